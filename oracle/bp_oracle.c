@@ -1,0 +1,330 @@
+/*
+ * FP64 CPU oracle for the SAR backprojection hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load the library built from it.  The HIP library never calls it.
+ *
+ * It restates, in C with explicit real/imaginary FP64 arithmetic and with
+ * floating-point contraction disabled (-ffp-contract=off), the Numba kernels
+ * of the reference:
+ *   cubic_weights        fast_bp/ffbp.py:17-21
+ *   sample1              fast_bp/ffbp.py:24-29
+ *   sample2              fast_bp/ffbp.py:32-40
+ *   backproject_points   fast_bp/ffbp.py:43-55
+ *   bp_polar             fast_bp/ffbp.py:58-76
+ *   regrid               fast_bp/ffbp.py:79-95
+ *   evaluate_image       fast_bp/ffbp.py:189-198
+ *   target_contributions fast_bp/ffbp.py:201-211
+ * The order of every operation follows the Python expressions in those
+ * lines (left-to-right evaluation, complex128 promotion of complex64 samples).
+ * Loops over output pixels run under OpenMP, like the reference's prange.
+ *
+ * Two interpolators have no reference counterpart and are specified here:
+ *   linear      : (1-t)*row[i] + t*row[i+1], valid when 0 <= i <= len-2.
+ *   kaiser-sinc : 8 taps at i-3..i+4, w = sinc(x)*I0(beta*sqrt(1-(x/4)^2))/I0(beta),
+ *                 beta = 6, normalised by the sum of the 8 weights, valid when
+ *                 3 <= i and i+4 <= len-1 (all taps inside), else 0.
+ * One documented deviation: regrid/evaluate_image return 0 where the source
+ * ground distance g is exactly 0 (Numba would raise ZeroDivisionError; the
+ * reference GPU kernels return 0, fast_bp/gpu/kernels.cu:359).
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { ORC_LAGRANGE4 = 0, ORC_LINEAR = 1, ORC_KAISER8 = 2 };
+
+#define KAISER_BETA 6.0
+#define KAISER_HALF 4
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ffbp.py:17-21, same association order. */
+void orc_cubic_weights(double t, double w[4]) {
+    w[0] = -t * (t - 1) * (t - 2) / 6;
+    w[1] = (t + 1) * (t - 1) * (t - 2) / 2;
+    w[2] = -(t + 1) * t * (t - 2) / 2;
+    w[3] = (t + 1) * t * (t - 1) / 6;
+}
+
+/* Modified Bessel function I0 by its power series (converges for all x). */
+static double bessel_i0(double x) {
+    double sum = 1.0, term = 1.0, q = 0.25 * x * x;
+    for (int k = 1; k < 200; ++k) {
+        term *= q / ((double)k * (double)k);
+        sum += term;
+        if (term < 1e-17 * sum) break;
+    }
+    return sum;
+}
+
+double orc_kaiser_sinc(double x) {
+    if (fabs(x) >= KAISER_HALF) return 0.0;
+    double s = (x == 0.0) ? 1.0 : sin(M_PI * x) / (M_PI * x);
+    double a = x / KAISER_HALF;
+    double win = bessel_i0(KAISER_BETA * sqrt(1.0 - a * a)) / bessel_i0(KAISER_BETA);
+    return s * win;
+}
+
+/* Weights of the 8-tap Kaiser-sinc at fraction t; taps i-3..i+4. */
+void orc_kaiser_weights(double t, double w[8]) {
+    double sum = 0.0;
+    for (int a = 0; a < 8; ++a) {
+        w[a] = orc_kaiser_sinc(t - (double)(a - 3));
+        sum += w[a];
+    }
+    for (int a = 0; a < 8; ++a) w[a] /= sum;
+}
+
+static inline void load_c(const void* base, int is_c128, size_t idx, double* re, double* im) {
+    if (is_c128) {
+        const double* p = (const double*)base;
+        *re = p[2 * idx];
+        *im = p[2 * idx + 1];
+    } else {
+        const float* p = (const float*)base;
+        *re = (double)p[2 * idx];
+        *im = (double)p[2 * idx + 1];
+    }
+}
+
+/* ffbp.py:24-29 (lagrange4) plus the linear and kaiser options. */
+void orc_sample1(const float* row, long len, double q, int interp, double out[2]) {
+    double fi = floor(q);
+    out[0] = 0.0;
+    out[1] = 0.0;
+    if (!(fi > -2147483648.0 && fi < 2147483647.0)) return;
+    long i = (long)fi;
+    double t = q - (double)i;
+    if (interp == ORC_LINEAR) {
+        if (i < 0 || i + 1 >= len) return;
+        double w0 = 1.0 - t, w1 = t;
+        out[0] = w0 * (double)row[2 * i] + w1 * (double)row[2 * (i + 1)];
+        out[1] = w0 * (double)row[2 * i + 1] + w1 * (double)row[2 * (i + 1) + 1];
+        return;
+    }
+    if (interp == ORC_KAISER8) {
+        if (i < 3 || i + 4 >= len) return;
+        double w[8];
+        orc_kaiser_weights(t, w);
+        double zr = 0.0, zi = 0.0;
+        for (int a = 0; a < 8; ++a) {
+            zr += w[a] * (double)row[2 * (i + a - 3)];
+            zi += w[a] * (double)row[2 * (i + a - 3) + 1];
+        }
+        out[0] = zr;
+        out[1] = zi;
+        return;
+    }
+    if (i < 1 || i + 2 >= len) return;
+    double w[4];
+    orc_cubic_weights(t, w);
+    /* w0*row[i-1] + w1*row[i] + w2*row[i+1] + w3*row[i+2], left to right. */
+    double zr = w[0] * (double)row[2 * (i - 1)];
+    double zi = w[0] * (double)row[2 * (i - 1) + 1];
+    zr = zr + w[1] * (double)row[2 * i];
+    zi = zi + w[1] * (double)row[2 * i + 1];
+    zr = zr + w[2] * (double)row[2 * (i + 1)];
+    zi = zi + w[2] * (double)row[2 * (i + 1) + 1];
+    zr = zr + w[3] * (double)row[2 * (i + 2)];
+    zi = zi + w[3] * (double)row[2 * (i + 2) + 1];
+    out[0] = zr;
+    out[1] = zi;
+}
+
+/* ffbp.py:32-40 (lagrange4) plus the kaiser option; img is [nr, nu]. */
+void orc_sample2(const void* img, int is_c128, long nr, long nu, double r, double u,
+                 int interp, double out[2]) {
+    out[0] = 0.0;
+    out[1] = 0.0;
+    double fr = floor(r), fu = floor(u);
+    if (!(fr > -2147483648.0 && fr < 2147483647.0)) return;
+    if (!(fu > -2147483648.0 && fu < 2147483647.0)) return;
+    long ir = (long)fr, iu = (long)fu;
+    if (interp == ORC_KAISER8) {
+        if (ir < 3 || iu < 3 || ir + 4 >= nr || iu + 4 >= nu) return;
+        double wr[8], wu[8];
+        orc_kaiser_weights(r - (double)ir, wr);
+        orc_kaiser_weights(u - (double)iu, wu);
+        double zr = 0.0, zi = 0.0;
+        for (int a = 0; a < 8; ++a) {
+            for (int b = 0; b < 8; ++b) {
+                double vr, vi;
+                load_c(img, is_c128, (size_t)(ir + a - 3) * (size_t)nu + (size_t)(iu + b - 3), &vr, &vi);
+                zr += vr * wr[a] * wu[b];
+                zi += vi * wr[a] * wu[b];
+            }
+        }
+        out[0] = zr;
+        out[1] = zi;
+        return;
+    }
+    if (ir < 1 || iu < 1 || ir + 2 >= nr || iu + 2 >= nu) return;
+    double wr[4], wu[4];
+    orc_cubic_weights(r - (double)ir, wr);
+    orc_cubic_weights(u - (double)iu, wu);
+    double zr = 0.0, zi = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        for (int b = 0; b < 4; ++b) {
+            double vr, vi;
+            load_c(img, is_c128, (size_t)(ir + a - 1) * (size_t)nu + (size_t)(iu + b - 1), &vr, &vi);
+            /* z += img[...]*wr[a]*wu[b]: (img*wr[a])*wu[b] */
+            zr = zr + (vr * wr[a]) * wu[b];
+            zi = zi + (vi * wr[a]) * wu[b];
+        }
+    }
+    out[0] = zr;
+    out[1] = zi;
+}
+
+/* total += weight*v*complex(cos(a), sin(a)): (weight*v) then complex product. */
+static inline void mac_rot(double* tr, double* ti, double w, const double v[2], double a) {
+    double wr = w * v[0], wi = w * v[1];
+    double c = cos(a), s = sin(a);
+    *tr = *tr + (wr * c - wi * s);
+    *ti = *ti + (wr * s + wi * c);
+}
+
+/* ffbp.py:43-55: direct BP at arbitrary xyz, not demodulated. */
+void orc_backproject_points(const float* data, long P, long S, const double* pos,
+                            const double* xyz, long M, double r_start, double dr, double k,
+                            const double* weight, const double* phase, int interp,
+                            double* out) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (long p = 0; p < M; ++p) {
+        double tr = 0.0, ti = 0.0;
+        for (long n = 0; n < P; ++n) {
+            double dx = xyz[3 * p] - pos[3 * n];
+            double dy = xyz[3 * p + 1] - pos[3 * n + 1];
+            double dz = xyz[3 * p + 2] - pos[3 * n + 2];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            double v[2];
+            orc_sample1(data + 2 * (size_t)n * (size_t)S, S, (r - r_start) / dr, interp, v);
+            double a = k * r + phase[n];
+            mac_rot(&tr, &ti, weight[n], v, a);
+        }
+        out[2 * p] = tr;
+        out[2 * p + 1] = ti;
+    }
+}
+
+/* ffbp.py:58-76: polar BP demodulated by exp(-j k r); pixels on z = 0. */
+void orc_bp_polar(const float* data, long P, long S, const double* pos, const double* origin,
+                  double r0, double dr, long nr, double u0, double du, long nu,
+                  double r_start, double data_dr, double k, const double* weight,
+                  const double* phase, int side, int interp, double* out) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (long ir = 0; ir < nr; ++ir) {
+        double r = r0 + (double)ir * dr;
+        double ground = sqrt(fmax(0.0, r * r - origin[2] * origin[2]));
+        for (long iu = 0; iu < nu; ++iu) {
+            double u = u0 + (double)iu * du;
+            double x = origin[0] + (double)side * ground * sqrt(fmax(0.0, 1 - u * u));
+            double y = origin[1] + ground * u;
+            double tr = 0.0, ti = 0.0;
+            for (long n = 0; n < P; ++n) {
+                double dx = x - pos[3 * n];
+                double dy = y - pos[3 * n + 1];
+                double d = sqrt(dx * dx + dy * dy + pos[3 * n + 2] * pos[3 * n + 2]);
+                double v[2];
+                orc_sample1(data + 2 * (size_t)n * (size_t)S, S, (d - r_start) / data_dr, interp, v);
+                double a = k * (d - r) + phase[n];
+                mac_rot(&tr, &ti, weight[n], v, a);
+            }
+            size_t o = (size_t)ir * (size_t)nu + (size_t)iu;
+            out[2 * o] = tr;
+            out[2 * o + 1] = ti;
+        }
+    }
+}
+
+/* ffbp.py:79-95: child polar image -> parent polar grid. */
+void orc_regrid(const void* img, int is_c128, long snr, long snu, const double* src_origin,
+                double sr0, double sdr, double su0, double sdu, const double* dst_origin,
+                double r0, double dr, long nr, double u0, double du, long nu, double k,
+                int side, int interp, double* out) {
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (long ir = 0; ir < nr; ++ir) {
+        double r = r0 + (double)ir * dr;
+        double gr = sqrt(fmax(0.0, r * r - dst_origin[2] * dst_origin[2]));
+        for (long iu = 0; iu < nu; ++iu) {
+            size_t o = (size_t)ir * (size_t)nu + (size_t)iu;
+            double u = u0 + (double)iu * du;
+            double x = dst_origin[0] + (double)side * gr * sqrt(fmax(0.0, 1 - u * u));
+            double y = dst_origin[1] + gr * u;
+            double dx = x - src_origin[0];
+            double dy = y - src_origin[1];
+            double g = sqrt(dx * dx + dy * dy);
+            if (g == 0.0) {
+                out[2 * o] = 0.0;
+                out[2 * o + 1] = 0.0;
+                continue;
+            }
+            double rr = sqrt(g * g + src_origin[2] * src_origin[2]);
+            double uu = dy / g;
+            double v[2];
+            orc_sample2(img, is_c128, snr, snu, (rr - sr0) / sdr, (uu - su0) / sdu, interp, v);
+            double a = k * (rr - r);
+            double c = cos(a), s = sin(a);
+            out[2 * o] = v[0] * c - v[1] * s;
+            out[2 * o + 1] = v[0] * s + v[1] * c;
+        }
+    }
+}
+
+/* ffbp.py:189-198: polar image -> arbitrary points, remodulated by exp(+j k r).
+ * The point height is ignored, exactly as in the reference (pixels on z=0). */
+void orc_evaluate_image(const void* img, int is_c128, long nr, long nu, const double* origin,
+                        double r0, double dr, double u0, double du, const double* xyz, long M,
+                        double k, int interp, double* out) {
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (long p = 0; p < M; ++p) {
+        double dx = xyz[3 * p] - origin[0];
+        double dy = xyz[3 * p + 1] - origin[1];
+        double g = sqrt(dx * dx + dy * dy);
+        if (g == 0.0) {
+            out[2 * p] = 0.0;
+            out[2 * p + 1] = 0.0;
+            continue;
+        }
+        double r = sqrt(g * g + origin[2] * origin[2]);
+        double u = dy / g;
+        double v[2];
+        orc_sample2(img, is_c128, nr, nu, (r - r0) / dr, (u - u0) / du, interp, v);
+        double c = cos(k * r), s = sin(k * r);
+        out[2 * p] = v[0] * c - v[1] * s;
+        out[2 * p + 1] = v[0] * s + v[1] * c;
+    }
+}
+
+/* ffbp.py:201-211: per-(point, pulse) unweighted contributions, [M, P]. */
+void orc_target_contributions(const float* data, long P, long S, const double* pos,
+                              const double* xyz, long M, double r_start, double dr, double k,
+                              int interp, double* out) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (long p = 0; p < M; ++p) {
+        for (long n = 0; n < P; ++n) {
+            double dx = xyz[3 * p] - pos[3 * n];
+            double dy = xyz[3 * p + 1] - pos[3 * n + 1];
+            double dz = xyz[3 * p + 2] - pos[3 * n + 2];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            double v[2];
+            orc_sample1(data + 2 * (size_t)n * (size_t)S, S, (r - r_start) / dr, interp, v);
+            double a = k * r;
+            double c = cos(a), s = sin(a);
+            size_t o = (size_t)p * (size_t)P + (size_t)n;
+            out[2 * o] = v[0] * c - v[1] * s;
+            out[2 * o + 1] = v[0] * s + v[1] * c;
+        }
+    }
+}
