@@ -1,0 +1,383 @@
+"""Throughput benchmark of the SAR FFBP / direct-BP hot path on one MI355X.
+
+Default workload (``c2_ffbp_full``): FFBP of an 8192-pulse x 8192-bin
+X-band stripmap scene (200 point targets + 20 dB clutter, seeded synthetic
+data), leaf sub-apertures of 16 pulses merged by factor 2 over 9 levels with
+8-tap Kaiser(beta=6)-sinc polar regridding, then evaluation onto a
+4096 x 4096 Cartesian grid at 0.25 m.  One step = leaves + all merge levels +
+crop + Cartesian evaluation, inputs already resident in HBM.
+
+value = effective Gpix*pulse/s = (output pixels x pulses) / step time, the
+reference's direct-equivalent throughput (fast_bp/gpu_backend.py:205).
+
+The JSON line also carries:
+* ``quality``: the split-precision image against the FP64 image of the same
+  algorithm computed on the GPU (NRMSE, peak-phase error at every target,
+  worst PSLR/ISLR delta), plus the FFBP-vs-direct-BP error on sampled pixels;
+* ``roofline``: achieved algorithmic FLOP/s of the dominant kernel (HIP events
+  around its launches on the launch stream) against the MI355X FP32 vector
+  peak -- backprojection is VALU-bound, it runs no MFMA;
+* ``cpu_baseline``: the FP64 C/OpenMP restatement of the reference's Numba
+  FFBP (the oracle) timed on the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gpix·pulse/s (BP, FFBP) at NRMSE≤1e-4 vs FP64; ms per autofocus grad step"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# Algorithmic operation counts (add/sub/mul = 1, FMA = 2, sqrt/sin/cos = 1).
+OPS_LEAF_LAGRANGE4 = 57      # per leaf pixel*pulse (4-point Lagrange)
+OPS_DIRECT_LINEAR = 35       # per pixel*pulse (linear interpolation)
+OPS_MERGE_KAISER8 = 742      # per parent pixel (two 8x8 Kaiser-sinc child regrids)
+OPS_MERGE_LAGRANGE4 = 290    # per parent pixel (two 4x4 Lagrange child regrids)
+OPS_EVAL_KAISER8 = 386       # per output pixel (one 8x8 regrid + remodulation)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="c2_ffbp_full",
+                   choices=["c2_ffbp_full", "c1_direct_small", "c3_direct_precision"])
+    p.add_argument("--mode", default="split")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-quality", action="store_true", help="skip the FP64 quality gate")
+    p.add_argument("--leaf", type=int, default=16)
+    p.add_argument("--range-spacing", type=float, default=0.25)
+    p.add_argument("--angular-oversample", type=float, default=2.5)
+    p.add_argument("--merge-interp", default="kaiser8")
+    return p.parse_args()
+
+
+class StageTimer:
+    """HIP events between consecutive launches on the current stream."""
+
+    def __init__(self):
+        from bp_project_amd.device import Event
+        self.Event = Event
+        self.marks = []
+
+    def mark(self, name):
+        e = self.Event()
+        e.record()
+        self.marks.append((name, e))
+
+    def durations(self):
+        from bp_project_amd.device import elapsed_ms
+        out = {}
+        for (_, a), (name, b) in zip(self.marks[:-1], self.marks[1:]):
+            out.setdefault(name, []).append(elapsed_ms(a, b))
+        return out
+
+
+def ffbp_workload(args, scene):
+    from bp_project_amd import _lib
+    from bp_project_amd.batched_resident_backend import BatchedResidentProgram
+    from bp_project_amd.device import DeviceArray, stream_handle
+    from bp_project_amd.gpu_backend import HipBaseline, build_ffbp_plan, geo_array
+    from bp_project_amd.scenes import polar_extent_for_grid
+
+    s = scene
+    g = s.grid
+    extent = polar_extent_for_grid(g, s.pos.mean(axis=0), margin=2.0)
+    t0 = time.perf_counter()
+    plan = build_ffbp_plan(s.pos, s.fc, extent, side=1, leaf_size=args.leaf,
+                           range_spacing=args.range_spacing,
+                           angular_oversample=args.angular_oversample)
+    log(f"plan: {plan.counts['nodes']} nodes, root {plan.root.nr}x{plan.root.nu}, crop "
+        f"{plan.shape}, {time.perf_counter() - t0:.2f} s")
+    backend = HipBaseline(interp="lagrange4", merge_interp=args.merge_interp)
+    dev = backend.upload(s.data, s.pos, s.weight, s.phase, plan)
+    vp = _lib.c_void_p
+
+    def make(mode):
+        prog = BatchedResidentProgram(backend, plan, dev, mode, s.r_start, s.data_dr, s.k)
+        cart = DeviceArray.empty(g.shape)
+        geo = geo_array(plan)
+        crop = prog.output
+        interp = _lib.interp_code(args.merge_interp)
+        code = _lib.mode_code(mode)
+
+        def evaluate():
+            _lib.call("bpmi_eval_grid", vp(crop.ptr), geo.ctypes.data_as(vp), plan.nr, plan.nu,
+                      plan.nu, g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, s.k, code, interp, vp(cart.ptr),
+                      stream_handle())
+
+        def step(timer=None):
+            for name, launcher, grid_, block, a in prog._schedule:
+                launcher(grid_, block, a)
+                if timer:
+                    timer.mark(name)
+            evaluate()
+            if timer:
+                timer.mark("eval_grid")
+        return prog, cart, step, geo
+
+    return plan, backend, dev, make
+
+
+def run_ffbp(args, scene):
+    from bp_project_amd.device import Event, elapsed_ms, synchronize
+    from bp_project_amd.metrics import fidelity, peak_phase_error, pslr_islr_deltas, target_peaks
+
+    plan, backend, dev, make = ffbp_workload(args, scene)
+    g = scene.grid
+    prog, cart, step, _ = make(args.mode)
+    for _ in range(args.warmup):
+        step()
+    synchronize()
+    timer = StageTimer()
+    start, end = Event(), Event()
+    synchronize()
+    start.record()
+    timer.mark("start")
+    for _ in range(args.steps):
+        step(timer)
+    end.record()
+    synchronize()
+    ms_total = elapsed_ms(start, end)
+    ms_step = ms_total / args.steps
+    stages = timer.durations()
+    # per-stage algorithmic work
+    flat = prog.flat
+    nodes_px = flat.node_int[:, 2].astype(np.int64) * flat.node_int[:, 3]
+    leaf_pp = int(plan.counts["leaf_pixel_pulse_products"])
+    merge_px = int(sum(nodes_px[i] for lvl in flat.levels[1:] for i in lvl))
+    ops_merge = OPS_MERGE_KAISER8 if args.merge_interp == "kaiser8" else OPS_MERGE_LAGRANGE4
+    work = {
+        f"batched_bp_polar_{args.mode}": leaf_pp * OPS_LEAF_LAGRANGE4,
+        f"batched_merge_regrid_{args.mode}": merge_px * ops_merge,
+        "eval_grid": g.nx * g.ny * OPS_EVAL_KAISER8,
+        "crop_float2": 0,
+    }
+    per_stage = {}
+    for name, vals in stages.items():
+        per_stage[name] = {"launches_per_step": len(vals) // args.steps,
+                           "ms_per_step": float(np.sum(vals) / args.steps),
+                           "mean_launch_ms": float(np.mean(vals))}
+    dominant = max((n for n in per_stage if n in work and work[n] > 0),
+                   key=lambda n: per_stage[n]["ms_per_step"])
+    dom_ms = per_stage[dominant]["ms_per_step"]
+    achieved = work[dominant] / (dom_ms * 1e-3) / 1e12
+    log("stages:", json.dumps(per_stage))
+    result = {
+        "ms_per_step": ms_step,
+        "value": g.nx * g.ny * scene.pulses / (ms_step * 1e-3) / 1e9,
+        "stages": per_stage,
+        "roofline": {
+            "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": dominant,
+            "ops_per_unit": (OPS_LEAF_LAGRANGE4 if "bp_polar" in dominant else
+                             ops_merge if "merge" in dominant else OPS_EVAL_KAISER8),
+            "unit_of_work": ("leaf pixel*pulse" if "bp_polar" in dominant else
+                             "parent pixel" if "merge" in dominant else "output pixel"),
+            "launches_per_step": per_stage[dominant]["launches_per_step"],
+            "note": "FP32 vector-ALU bound (no MFMA); peak = MI355X FP32 vector 157.3 TFLOP/s",
+        },
+        "config": {
+            "workload": "c2_ffbp_full", "pulses": scene.pulses, "bins": scene.bins,
+            "targets": len(scene.targets), "leaf_size": args.leaf, "merge_factor": 2,
+            "levels": len(flat.levels) - 1, "merge_interp": args.merge_interp,
+            "leaf_interp": "lagrange4", "range_spacing_m": args.range_spacing,
+            "angular_oversample": args.angular_oversample, "output": [g.ny, g.nx],
+            "pixel_m": g.dx, "mode": args.mode, "root_polar": [plan.root.nr, plan.root.nu],
+            "arena_gb": round(flat.arena_elems * 8 / 1e9, 2),
+        },
+    }
+    if not args.no_quality:
+        img = cart.get()
+        ref_prog, ref_cart, ref_step, _ = make("fp64")
+        ref_step()
+        synchronize()
+        ref = ref_cart.get()
+        ref_prog.close()
+        f = fidelity(ref, img)
+        peaks = target_peaks(np.abs(ref), g, scene.targets)
+        ph = peak_phase_error(ref, img, peaks)
+        pis = pslr_islr_deltas(ref, img, g, peaks[:12])
+        # FFBP vs direct BP (algorithm error) on 2048 sampled pixels
+        from bp_project_amd.ffbp import backproject_points
+        rng = np.random.default_rng(7)
+        sel = rng.choice(g.nx * g.ny, 2048, replace=False)
+        sel = np.unique(np.r_[sel, [p[0] * g.nx + p[1] for p in peaks]])
+        pts = g.points()[sel]
+        direct = backproject_points(scene.data, scene.pos, pts, scene.r_start, scene.data_dr,
+                                    scene.k, scene.weight, scene.phase, mode="fp64")
+        alg = float(np.linalg.norm(ref.ravel()[sel] - direct) / np.linalg.norm(direct))
+        result["quality"] = {
+            "reference": "FP64 FFBP of the same plan on the GPU (fp64 mode)",
+            "nrmse": f["relative_complex_error"], "ser_db": f["ser_db"],
+            "max_peak_phase_error_rad": float(max(ph)) if ph else None,
+            "targets_measured": len(ph),
+            "max_abs_pslr_islr_delta_db": pis["max_abs_delta_db"],
+            "ffbp_vs_direct_fp64_error_sampled": alg,
+            "passes_nrmse_1e-4": bool(f["relative_complex_error"] <= 1e-4),
+        }
+        log("quality:", json.dumps(result["quality"]))
+    prog.close()
+    return result
+
+
+def cpu_baseline_ffbp(args, scene):
+    """Oracle (FP64 C/OpenMP FFBP) on a 512 x 512 sub-patch with all pulses."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    from bp_project_amd.scenes import CartesianGrid, polar_extent_for_grid
+    g = scene.grid
+    n = 512
+    cx = g.x0 + g.dx * g.nx / 2
+    cy = g.y0 + g.dy * g.ny / 2
+    sub = CartesianGrid(x0=cx - n * g.dx / 2, dx=g.dx, nx=n, y0=cy - n * g.dy / 2, dy=g.dy, ny=n)
+    extent = polar_extent_for_grid(sub, scene.pos.mean(axis=0), margin=2.0)
+    t = time.perf_counter()
+    res, _ = oracle.form_ffbp(scene.data, scene.pos, scene.r_start, scene.data_dr, scene.fc,
+                              extent, side=1, leaf_size=args.leaf,
+                              range_spacing=args.range_spacing,
+                              angular_oversample=args.angular_oversample, weight=scene.weight,
+                              phase=scene.phase, policy="parity", leaf_interp="lagrange4",
+                              merge_interp=args.merge_interp)
+    oracle.evaluate_image(res.image, res.origin, res.r0, res.dr, res.u0, res.du, sub.points(),
+                          scene.k, interp=args.merge_interp)
+    dt = time.perf_counter() - t
+    rate = n * n * scene.pulses / dt / 1e9
+    return {"value": round(rate, 4), "unit": "Gpix·pulse/s", "cores": oracle.num_threads(),
+            "kind": "port", "seconds": round(dt, 2),
+            "sample": f"FP64 oracle FFBP (same leaf/merge parameters, {args.merge_interp} merges) "
+                      f"+ evaluate_image of a {n}x{n} sub-patch at {g.dx} m with all "
+                      f"{scene.pulses} pulses; effective px*pulse/s"}
+
+
+def run_direct(args, scene, name):
+    from bp_project_amd import _lib
+    from bp_project_amd.device import DeviceArray, Event, elapsed_ms, synchronize
+    from bp_project_amd.metrics import fidelity, peak_phase_error, target_peaks
+    vp = _lib.c_void_p
+    g = scene.grid
+    interp = "linear" if name == "c1_direct_small" else "lagrange4"
+    d = DeviceArray.from_host(scene.data)
+    p = DeviceArray.from_host(scene.pos)
+    out = DeviceArray.empty(g.shape)
+
+    def step(mode=args.mode, dst=out):
+        _lib.call("bpmi_bp_grid", vp(d.ptr), scene.bins, vp(p.ptr), None, None, 0, scene.pulses,
+                  g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, 0.0, scene.r_start, scene.data_dr, scene.k,
+                  _lib.mode_code(mode), _lib.interp_code(interp), 0, vp(dst.ptr), None)
+    for _ in range(args.warmup):
+        step()
+    synchronize()
+    start, end = Event(), Event()
+    start.record()
+    for _ in range(args.steps):
+        step()
+    end.record()
+    synchronize()
+    ms = elapsed_ms(start, end) / args.steps
+    pp = g.nx * g.ny * scene.pulses
+    ops = OPS_DIRECT_LINEAR if interp == "linear" else OPS_LEAF_LAGRANGE4
+    achieved = pp * ops / (ms * 1e-3) / 1e12
+    result = {"ms_per_step": ms, "value": pp / (ms * 1e-3) / 1e9,
+              "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                           "kernel": "bp_split_kernel", "ops_per_unit": ops,
+                           "unit_of_work": "pixel*pulse", "launches_per_step": 1,
+                           "note": "FP32 vector-ALU bound (no MFMA)"},
+              "config": {"workload": name, "pulses": scene.pulses, "bins": scene.bins,
+                         "output": [g.ny, g.nx], "pixel_m": g.dx, "interp": interp,
+                         "mode": args.mode}}
+    if not args.no_quality:
+        img = out.get()
+        ref_out = DeviceArray.empty(g.shape)
+        step("fp64", ref_out)
+        synchronize()
+        ref = ref_out.get()
+        f = fidelity(ref, img)
+        peaks = target_peaks(np.abs(ref), g, scene.targets)
+        ph = peak_phase_error(ref, img, peaks)
+        result["quality"] = {"reference": "FP64 direct BP on the GPU",
+                             "nrmse": f["relative_complex_error"],
+                             "max_peak_phase_error_rad": float(max(ph)) if ph else None,
+                             "passes_nrmse_1e-4": bool(f["relative_complex_error"] <= 1e-4)}
+        log("quality:", json.dumps(result["quality"]))
+    return result
+
+
+def cpu_baseline_direct(scene, name):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    g = scene.grid
+    interp = "linear" if name == "c1_direct_small" else "lagrange4"
+    rows = 256 if name == "c1_direct_small" else 2
+    pts = g.points()[: rows * g.nx]
+    t = time.perf_counter()
+    oracle.backproject_points(scene.data, scene.pos, pts, scene.r_start, scene.data_dr, scene.k,
+                              scene.weight, scene.phase, interp=interp)
+    dt = time.perf_counter() - t
+    return {"value": round(len(pts) * scene.pulses / dt / 1e9, 4), "unit": "Gpix·pulse/s",
+            "cores": oracle.num_threads(), "kind": "port", "seconds": round(dt, 2),
+            "sample": f"FP64 oracle direct BP of {rows} grid rows ({len(pts)} pixels) x "
+                      f"{scene.pulses} pulses"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of this path is a design note, not built)",
+              file=sys.stderr)
+        sys.exit(2)
+    from bp_project_amd.device import device_count, device_info
+    if device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    from bp_project_amd.scenes import workload_scene
+    t = time.perf_counter()
+    scene = workload_scene(args.workload)
+    log(f"scene {args.workload}: {scene.data.shape}, {len(scene.targets)} targets, "
+        f"{time.perf_counter() - t:.1f} s")
+    log("device:", json.dumps(device_info(0)))
+    if args.workload == "c2_ffbp_full":
+        res = run_ffbp(args, scene)
+        cpu = None if args.no_cpu else cpu_baseline_ffbp(args, scene)
+    else:
+        res = run_direct(args, scene, args.workload)
+        cpu = None if args.no_cpu else cpu_baseline_direct(scene, args.workload)
+    if cpu:
+        log("cpu baseline:", json.dumps(cpu))
+    line = {
+        "metric": METRIC, "value": round(res["value"], 3), "unit": "Gpix·pulse/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": res["config"], "roofline": res["roofline"], "cpu_baseline": cpu,
+    }
+    if "quality" in res:
+        line["quality"] = res["quality"]
+    if "stages" in res:
+        line["stages_ms_per_step"] = {k: round(v["ms_per_step"], 4) for k, v in res["stages"].items()}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,143 @@
+"""ctypes binding of libbpmi.so (the C ABI declared in include/bpmi.h).
+
+The library is loaded from the package directory only.  There is no CPU
+fallback: if the shared object is missing or no HIP device is present, every
+entry point raises.  Status codes map to the reference's exception types:
+BPMI_ERR_INVALID -> ValueError, anything else -> RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_double, c_float, c_int, c_longlong, c_size_t, c_void_p, c_char_p, POINTER
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libbpmi.so"
+
+OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
+MODE_FP64, MODE_FP32, MODE_MIXED, MODE_DFLOAT, MODE_SPLIT = 0, 1, 2, 3, 4
+MODE_CODES = {"fp64": MODE_FP64, "fp32": MODE_FP32, "mixed": MODE_MIXED,
+              "dfloat": MODE_DFLOAT, "split": MODE_SPLIT}
+INTERP_CODES = {"lagrange4": 0, "linear": 1, "kaiser8": 2}
+ACCUMULATE, OUT_C128 = 1, 2
+KAISER_TABLE_LEN = 4 * 2048 + 2
+
+_vp, _i, _d, _ll, _sz = c_void_p, c_int, c_double, c_longlong, c_size_t
+
+# name -> argument types (every function returns int status)
+SIGNATURES = {
+    "bpmi_version": [],
+    "bpmi_last_error": [c_char_p, _sz],
+    "bpmi_device_count": [POINTER(c_int)],
+    "bpmi_set_device": [_i],
+    "bpmi_get_device": [POINTER(c_int)],
+    "bpmi_device_info": [_i, c_char_p, _sz, c_char_p, _sz, POINTER(c_int), POINTER(c_int),
+                         POINTER(c_size_t)],
+    "bpmi_runtime_version": [POINTER(c_int)],
+    "bpmi_mem_info": [POINTER(c_size_t), POINTER(c_size_t)],
+    "bpmi_malloc": [POINTER(c_void_p), _sz],
+    "bpmi_free": [_vp],
+    "bpmi_host_alloc": [POINTER(c_void_p), _sz],
+    "bpmi_host_free": [_vp],
+    "bpmi_memcpy_h2d": [_vp, _vp, _sz, _vp],
+    "bpmi_memcpy_d2h": [_vp, _vp, _sz, _vp],
+    "bpmi_memcpy_d2d": [_vp, _vp, _sz, _vp],
+    "bpmi_zero": [_vp, _sz, _vp],
+    "bpmi_stream_create": [POINTER(c_void_p)],
+    "bpmi_stream_destroy": [_vp],
+    "bpmi_stream_synchronize": [_vp],
+    "bpmi_device_synchronize": [],
+    "bpmi_event_create": [POINTER(c_void_p)],
+    "bpmi_event_destroy": [_vp],
+    "bpmi_event_record": [_vp, _vp],
+    "bpmi_event_synchronize": [_vp],
+    "bpmi_event_elapsed": [POINTER(c_float), _vp, _vp],
+    "bpmi_graph_begin": [_vp],
+    "bpmi_graph_end": [_vp, POINTER(c_void_p)],
+    "bpmi_graph_launch": [_vp, _vp],
+    "bpmi_graph_destroy": [_vp],
+    "bpmi_bp_polar": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _d, _d, _i, _i, _i,
+                      _vp, _vp],
+    "bpmi_bp_polar_ex": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _d, _d, _i, _i, _i,
+                         _vp, _vp, _i, _i],
+    "bpmi_bp_grid": [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _i, _d, _d, _i, _d, _d, _d, _d, _i,
+                     _i, _i, _vp, _vp],
+    "bpmi_bp_points": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _ll, _d, _d, _d, _i, _i, _i, _vp,
+                       _vp],
+    "bpmi_target_contributions": [_vp, _i, _vp, _i, _vp, _ll, _d, _d, _d, _i, _i, _vp, _vp],
+    "bpmi_regrid": [_vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _i, _i, _i, _vp, _vp],
+    "bpmi_merge2": [_vp, _vp, _i, _i, _vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _i, _i, _vp, _vp],
+    "bpmi_add_inplace": [_vp, _vp, _ll, _vp],
+    "bpmi_cast_c128_c64": [_vp, _vp, _ll, _vp],
+    "bpmi_crop": [_vp, _i, _i, _i, _i, _i, _vp, _vp],
+    "bpmi_eval_points": [_vp, _vp, _i, _i, _i, _vp, _ll, _d, _i, _i, _vp, _vp],
+    "bpmi_eval_grid": [_vp, _vp, _i, _i, _i, _d, _d, _i, _d, _d, _i, _d, _i, _i, _vp, _vp],
+    "bpmi_ffbp_leaves": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _i, _i,
+                         _vp],
+    "bpmi_ffbp_merges": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _i, _vp],
+    "bpmi_split_tiles": [_i, _i, POINTER(c_int)],
+    "bpmi_fill_range_phase_lut": [_i, _d, _d, _d, _vp, _vp],
+    "bpmi_bp_polar_lut": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _d, _d, _vp, _vp,
+                          _vp],
+    "bpmi_set_kaiser_table": [_vp, _i],
+    "bpmi_probe_sincos_turns": [_vp, _vp, _vp, _ll, _vp],
+}
+
+_lib = None
+
+
+class BpmiError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libbpmi.so (no GPU is touched by loading)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"libbpmi.so not found at {p}: build it with `python -m bp_project_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(str(p))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().bpmi_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status, what=""):
+    if status == OK:
+        return
+    msg = last_error()
+    if status == ERR_INVALID:
+        raise ValueError(msg)
+    raise BpmiError(f"{what}: {msg}" if what else msg)
+
+
+def call(name, *args):
+    """Call a bpmi_* function and raise on a nonzero status."""
+    check(getattr(load(), name)(*args), name)
+
+
+def mode_code(mode):
+    try:
+        return MODE_CODES[mode]
+    except KeyError:
+        raise ValueError(f"mode must be one of {tuple(MODE_CODES)}") from None
+
+
+def interp_code(interp):
+    try:
+        return INTERP_CODES[interp]
+    except KeyError:
+        raise ValueError(f"interp must be one of {tuple(INTERP_CODES)}") from None

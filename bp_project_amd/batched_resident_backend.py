@@ -1,0 +1,126 @@
+"""Level-batched, preallocated FFBP (fast_bp/batched_resident_backend.py:23-157).
+
+All leaves run in one launch; one launch per subtree height merges every
+parent whose children are ready (both children regridded, each rounded to
+complex64, then an FP32 add); a final crop produces the requested extent.
+Node images live in one arena addressed by per-node offsets, and the node
+geometry/offset/block tables are uploaded once at construction.
+
+In ``split`` mode the leaf launch uses the LDS-tiled kernel with each leaf's
+origin as the geometry reference (1024-pixel tiles, block table of
+(node, tile) pairs); the other modes use one thread per pixel with blocks of
+256 pixels, like the reference's 128-pixel block table.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray
+from .plan import NI_NR, NI_NU, flatten_plan
+from .resident_backend import ResidentProgram
+
+_vp = _lib.c_void_p
+
+MERGE_BLOCK = 256
+
+
+def split_tiles(nr, nu):
+    t = ctypes.c_int()
+    _lib.call("bpmi_split_tiles", int(nr), int(nu), ctypes.byref(t))
+    return t.value
+
+
+class BatchedResidentProgram(ResidentProgram):
+    """ResidentProgram-compatible FFBP batched across each tree level."""
+
+    def __init__(self, backend, plan, device, mode, range_start, data_dr, k, chunk_pulses=64):
+        self._table_buffers = []
+        self._batch_stages = []
+        super().__init__(backend, plan, device, mode, range_start, data_dr, k,
+                         algorithm="ffbp", variant="fused", chunk_pulses=chunk_pulses)
+        self.variant = "batched_fused"
+        table_bytes = sum(int(a.nbytes) for a in self._table_buffers)
+        self.metadata.update({
+            "variant": self.variant,
+            "batch_stages": self._batch_stages,
+            "extra_table_bytes": table_bytes,
+            "extra_table_count": len(self._table_buffers),
+            "total_owned_device_bytes": self.metadata["total_owned_device_bytes"] + table_bytes,
+            "batch_mapping": "one workgroup per node pixel tile; nodes grouped by subtree height",
+            "fusion": "each parent's two child regrids rounded to complex64 then FP32 add",
+        })
+
+    def _table(self, host):
+        array = DeviceArray.from_host(np.ascontiguousarray(host))
+        self._table_buffers.append(array)
+        return array
+
+    def _build_node(self, root):
+        flat = flatten_plan(self.plan)
+        self.flat = flat
+        arena = self._new_buffer(flat.arena_elems)
+        self.arena = arena
+        self._node_table = self._table(flat.node_int)
+        self._geometry_table = self._table(flat.node_geo)
+        self._offset_table = self._table(flat.offsets.astype(np.int64))
+        d = self.device
+        mode = _lib.mode_code(self.mode)
+        for height, nodes in enumerate(flat.levels):
+            if height == 0:
+                if self.mode == "split":
+                    mapping = [(n, t) for n in nodes
+                               for t in range(split_tiles(flat.node_int[n, NI_NR],
+                                                          flat.node_int[n, NI_NU]))]
+                else:
+                    mapping = [(n, s) for n in nodes
+                               for s in range(0, int(flat.node_int[n, NI_NR]) *
+                                              int(flat.node_int[n, NI_NU]), MERGE_BLOCK)]
+                blocks = self._table(np.asarray(mapping, dtype=np.int32))
+                self._enqueue("bpmi_ffbp_leaves", len(mapping) * MERGE_BLOCK, (
+                    _vp(d.data.ptr), d.data.shape[1], _vp(d.pos64.ptr), _vp(d.weight64.ptr),
+                    _vp(d.phase64.ptr), _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
+                    _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(mapping),
+                    self.range_start, self.data_dr, self.k, mode, self._interp()),
+                    name=f"batched_bp_polar_{self.mode}")
+                kernel = f"batched_bp_polar_{self.mode}"
+            else:
+                mapping = [(n, s) for n in nodes
+                           for s in range(0, int(flat.node_int[n, NI_NR]) *
+                                          int(flat.node_int[n, NI_NU]), MERGE_BLOCK)]
+                blocks = self._table(np.asarray(mapping, dtype=np.int32))
+                self._enqueue("bpmi_ffbp_merges", len(mapping) * MERGE_BLOCK, (
+                    _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
+                    _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(mapping),
+                    self.k, mode, self._merge_interp()),
+                    name=f"batched_merge_regrid_{self.mode}")
+                kernel = f"batched_merge_regrid_{self.mode}"
+            pixels = int(sum(int(flat.node_int[n, NI_NR]) * int(flat.node_int[n, NI_NU])
+                             for n in nodes))
+            self.operations[-1].update({"height": height, "node_count": len(nodes),
+                                        "valid_pixels": pixels})
+            self._batch_stages.append({
+                "height": height, "kernel": kernel, "node_count": len(nodes),
+                "block_count": len(mapping), "valid_pixels": pixels,
+                "block_table_bytes": int(blocks.nbytes)})
+        root = flat.offsets[self.plan.root.index]
+        # the root image is a view into the arena
+        return DeviceArray((self.plan.root.nr * self.plan.root.nu,), np.complex64,
+                           arena._alloc, arena._offset + int(root) * 8)
+
+    def node_image(self, index):
+        """Device view of node ``index``'s polar image (valid after execute)."""
+        n = self.flat.node_int[index]
+        off = int(self.flat.offsets[index])
+        return DeviceArray((int(n[NI_NR]), int(n[NI_NU])), np.complex64, self.arena._alloc,
+                           self.arena._offset + off * 8)
+
+    def close(self):
+        super().close()
+        self._table_buffers.clear()
+        self._node_table = None
+        self._geometry_table = None
+        self._offset_table = None
+        self.arena = None

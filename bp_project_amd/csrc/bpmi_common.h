@@ -1,0 +1,280 @@
+// Shared device helpers for the gfx950 SAR backprojection library.
+//
+// Complex samples are float2 (complex64) in HBM.  Interpolators:
+//   LAGRANGE4 : the reference's 4-point Lagrange on taps i-1..i+2
+//               (fast_bp/ffbp.py:17-29), zero unless 1 <= i <= len-3;
+//   LINEAR    : (1-t) v[i] + t v[i+1], zero unless 0 <= i <= len-2;
+//   KAISER8   : 8-tap truncated sinc with a Kaiser (beta = 6) window on taps
+//               i-3..i+4, weights normalised to sum 1, zero unless
+//               3 <= i <= len-5.
+// Phase is handled in revolutions ("turns"): the FP64 argument is reduced by
+// subtracting its floor, and the FP32 hardware sine/cosine (whose input is in
+// revolutions) evaluates the reduced value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bpmi {
+
+enum Mode { MODE_FP64 = 0, MODE_FP32 = 1, MODE_MIXED = 2, MODE_DFLOAT = 3, MODE_SPLIT = 4 };
+enum Interp { INTERP_LAGRANGE4 = 0, INTERP_LINEAR = 1, INTERP_KAISER8 = 2 };
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+constexpr double INV_TWO_PI = 0.15915494309189533576888376337251;
+
+// Kaiser-sinc table: kernel value at x = i / KAISER_TABLE_RES for i in
+// [0, 4 * KAISER_TABLE_RES + 1]; symmetric, so only x >= 0 is stored.  The
+// interpolator reads it by linear interpolation (FP64 table in the FP64
+// paths, its FP32 rounding elsewhere) and normalises the 8 weights.
+constexpr int KAISER_TABLE_RES = 2048;
+constexpr int KAISER_TABLE_LEN = 4 * KAISER_TABLE_RES + 2;
+
+// The device table allocation holds the FP32 table followed by the FP64 one
+// (KAISER_TABLE_LEN floats is a multiple of 8 bytes).
+__device__ __forceinline__ const double* kaiser_table64(const float* t32) {
+    return reinterpret_cast<const double*>(t32 + KAISER_TABLE_LEN);
+}
+
+__device__ __forceinline__ double fract64(double x) { return x - floor(x); }
+__device__ __forceinline__ float fract32(float x) { return x - floorf(x); }
+
+// sin/cos of 2*pi*t with the hardware transcendental unit (input in turns).
+__device__ __forceinline__ void sincos_turns(float t, float* s, float* c) {
+    *s = __builtin_amdgcn_sinf(t);
+    *c = __builtin_amdgcn_cosf(t);
+}
+
+// exp(j 2 pi T) for an FP64 phase in turns: exact FP64 reduction, FP32 sincos.
+__device__ __forceinline__ float2 cis_turns64(double turns) {
+    float s, c;
+    sincos_turns((float)fract64(turns), &s, &c);
+    return make_float2(c, s);
+}
+
+// Reference association order of cubic_weights (fast_bp/ffbp.py:17-21).
+template <typename T>
+__device__ __forceinline__ void cubic_weights(T t, T w[4]) {
+    w[0] = -t * (t - T(1)) * (t - T(2)) / T(6);
+    w[1] = (t + T(1)) * (t - T(1)) * (t - T(2)) / T(2);
+    w[2] = -(t + T(1)) * t * (t - T(2)) / T(2);
+    w[3] = (t + T(1)) * t * (t - T(1)) / T(6);
+}
+
+// Kaiser-sinc weights from the device table (linear interpolation of the
+// tabulated kernel), normalised to unit sum.
+__device__ __forceinline__ void kaiser_weights_f32(const float* table, float t, float w[8]) {
+    float sum = 0.f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        float x = fabsf(t - (float)(a - 3)) * (float)KAISER_TABLE_RES;
+        int i = (int)x;
+        float f = x - (float)i;
+        i = min(i, KAISER_TABLE_LEN - 2);
+        float v = table[i] + f * (table[i + 1] - table[i]);
+        w[a] = v;
+        sum += v;
+    }
+    float inv = 1.f / sum;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) w[a] *= inv;
+}
+
+__device__ __forceinline__ void kaiser_weights_f64(const double* table, double t, double w[8]) {
+    double sum = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        double x = fabs(t - (double)(a - 3)) * (double)KAISER_TABLE_RES;
+        int i = (int)x;
+        double f = x - (double)i;
+        i = min(i, KAISER_TABLE_LEN - 2);
+        w[a] = table[i] + f * (table[i + 1] - table[i]);
+        sum += w[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) w[a] /= sum;
+}
+
+// Interpolator geometry: first tap offset, tap count, validity of index i.
+template <int INTERP> struct InterpShape;
+template <> struct InterpShape<INTERP_LAGRANGE4> {
+    static constexpr int FIRST = -1, TAPS = 4;
+    __device__ static bool valid(int i, int len) { return i >= 1 && i + 2 < len; }
+};
+template <> struct InterpShape<INTERP_LINEAR> {
+    static constexpr int FIRST = 0, TAPS = 2;
+    __device__ static bool valid(int i, int len) { return i >= 0 && i + 1 < len; }
+};
+template <> struct InterpShape<INTERP_KAISER8> {
+    static constexpr int FIRST = -3, TAPS = 8;
+    __device__ static bool valid(int i, int len) { return i >= 3 && i + 4 < len; }
+};
+
+// 1-D sample of a complex64 row, FP32 weights and accumulation.
+template <int INTERP>
+__device__ __forceinline__ float2 sample1_f32w(const float2* __restrict__ row, int len, int i,
+                                               float t, const float* ktab) {
+    float2 z = make_float2(0.f, 0.f);
+    if (!InterpShape<INTERP>::valid(i, len)) return z;
+    if (INTERP == INTERP_LINEAR) {
+        const float2 a = row[i], b = row[i + 1];
+        const float w0 = 1.f - t;
+        z.x = w0 * a.x + t * b.x;
+        z.y = w0 * a.y + t * b.y;
+    } else if (INTERP == INTERP_LAGRANGE4) {
+        float w[4];
+        cubic_weights(t, w);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float2 v = row[i + a - 1];
+            z.x += w[a] * v.x;
+            z.y += w[a] * v.y;
+        }
+    } else {
+        float w[8];
+        kaiser_weights_f32(ktab, t, w);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float2 v = row[i + a - 3];
+            z.x += w[a] * v.x;
+            z.y += w[a] * v.y;
+        }
+    }
+    return z;
+}
+
+// 1-D sample in FP64 (weights and accumulation), reference association order.
+template <int INTERP>
+__device__ __forceinline__ double2 sample1_f64(const float2* __restrict__ row, int len, double q,
+                                               const double* ktab64 = nullptr) {
+    double2 z = make_double2(0.0, 0.0);
+    const double fi = floor(q);
+    if (!(fi > -2147483000.0 && fi < 2147483000.0)) return z;
+    const int i = (int)fi;
+    if (!InterpShape<INTERP>::valid(i, len)) return z;
+    const double t = q - (double)i;
+    if (INTERP == INTERP_LINEAR) {
+        const float2 a = row[i], b = row[i + 1];
+        const double w0 = 1.0 - t;
+        z.x = w0 * (double)a.x + t * (double)b.x;
+        z.y = w0 * (double)a.y + t * (double)b.y;
+    } else if (INTERP == INTERP_LAGRANGE4) {
+        double w[4];
+        cubic_weights(t, w);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float2 v = row[i + a - 1];
+            z.x = z.x + w[a] * (double)v.x;
+            z.y = z.y + w[a] * (double)v.y;
+        }
+    } else {
+        double w[8];
+        kaiser_weights_f64(ktab64, t, w);
+        for (int a = 0; a < 8; ++a) {
+            const float2 v = row[i + a - 3];
+            z.x += w[a] * (double)v.x;
+            z.y += w[a] * (double)v.y;
+        }
+    }
+    return z;
+}
+
+// floor of an FP64 coordinate as int with a finite-range guard.
+__device__ __forceinline__ bool floor_index(double q, int* i) {
+    const double f = floor(q);
+    if (!(f > -2147483000.0 && f < 2147483000.0)) return false;
+    *i = (int)f;
+    return true;
+}
+
+__device__ __forceinline__ bool floor_index_f32(float q, int* i) {
+    const float f = floorf(q);
+    if (!(f > -2.0e9f && f < 2.0e9f)) return false;
+    *i = (int)f;
+    return true;
+}
+
+// Pixel of a polar node grid (fast_bp/ffbp.py:62-67): slant range r from the
+// node origin, u = sin(horizontal angle), pixel on z = 0.
+__device__ __forceinline__ void polar_pixel(double ox, double oy, double oz, double r, double u,
+                                            int side, double* x, double* y) {
+    const double ground = sqrt(fmax(0.0, r * r - oz * oz));
+    *x = ox + (double)side * ground * sqrt(fmax(0.0, 1.0 - u * u));
+    *y = oy + ground * u;
+}
+
+__device__ __forceinline__ void polar_pixel_f32(float ox, float oy, float oz, float r, float u,
+                                                int side, float* x, float* y) {
+    const float ground = sqrtf(fmaxf(0.0f, r * r - oz * oz));
+    *x = ox + (float)side * ground * sqrtf(fmaxf(0.0f, 1.0f - u * u));
+    *y = oy + ground * u;
+}
+
+// ---------------------------------------------------------------------------
+// Double-float (hi + lo FP32) arithmetic, error-free transformations with FMA
+// (Dekker / Knuth two-sum and FMA two-product).
+// ---------------------------------------------------------------------------
+struct df {
+    float hi, lo;
+};
+
+__device__ __forceinline__ df df_from(double x) {
+    df r;
+    r.hi = (float)x;
+    r.lo = (float)(x - (double)r.hi);
+    return r;
+}
+
+__device__ __forceinline__ df two_sum(float a, float b) {
+    df r;
+    r.hi = a + b;
+    const float bb = r.hi - a;
+    r.lo = (a - (r.hi - bb)) + (b - bb);
+    return r;
+}
+
+__device__ __forceinline__ df quick_two_sum(float a, float b) {
+    df r;
+    r.hi = a + b;
+    r.lo = b - (r.hi - a);
+    return r;
+}
+
+__device__ __forceinline__ df df_add(df a, df b) {
+    df s = two_sum(a.hi, b.hi);
+    s.lo += a.lo + b.lo;
+    return quick_two_sum(s.hi, s.lo);
+}
+
+__device__ __forceinline__ df df_sub(df a, df b) {
+    df nb;
+    nb.hi = -b.hi;
+    nb.lo = -b.lo;
+    return df_add(a, nb);
+}
+
+__device__ __forceinline__ df df_mul(df a, df b) {
+    const float p = a.hi * b.hi;
+    const float e = fmaf(a.hi, b.hi, -p);
+    const float lo = e + (a.hi * b.lo + a.lo * b.hi);
+    return quick_two_sum(p, lo);
+}
+
+__device__ __forceinline__ df df_sqr(df a) { return df_mul(a, a); }
+
+// sqrt by one Newton step from the FP32 root; the residual is formed exactly
+// with an FMA.
+__device__ __forceinline__ df df_sqrt(df a) {
+    if (a.hi <= 0.f) {
+        df z;
+        z.hi = 0.f;
+        z.lo = 0.f;
+        return z;
+    }
+    const float s = sqrtf(a.hi);
+    const float res = (fmaf(-s, s, a.hi) + a.lo);
+    const float corr = res / (2.f * s);
+    return quick_two_sum(s, corr);
+}
+
+}  // namespace bpmi

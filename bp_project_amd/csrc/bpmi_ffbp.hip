@@ -1,0 +1,662 @@
+// FFBP merge/regrid, polar->Cartesian evaluation, per-pulse contributions,
+// the phase-LUT direct variant and the small utility kernels.
+//
+// Precision policies per kernel:
+//   FP64   : FP64 geometry, weights, accumulation and sincos; complex64 store
+//            (regrid_fp64, fast_bp/gpu/kernels.cu:341-369).
+//   FP32   : every input rounded to FP32, FP32 arithmetic, sincosf
+//            (regrid_fp32, kernels.cu:371-398).
+//   others : FP64 geometry and sample coordinates, FP32 weights and
+//            accumulation, FP64 phase reduced to turns + FP32 sincos.
+#include "bpmi_internal.h"
+
+#include <cstring>
+
+namespace bpmi {
+
+static float* g_kaiser_table = nullptr;
+
+const float* kaiser_table_device() { return g_kaiser_table; }
+
+struct Geo {
+    double ox, oy, oz, r0, dr, u0, du;
+};
+
+__host__ __device__ inline Geo make_geo(const double* g) {
+    Geo r;
+    r.ox = g[0]; r.oy = g[1]; r.oz = g[2]; r.r0 = g[3]; r.dr = g[4]; r.u0 = g[5]; r.du = g[6];
+    return r;
+}
+
+// ---- 2-D sampling of a polar image [nr, ld] (first nu columns valid) -------
+template <int INTERP>
+__device__ __forceinline__ float2 sample2_f32w(const float2* __restrict__ img, int nr, int nu,
+                                               int ld, int ir, int iu, float tr, float tu,
+                                               const float* ktab) {
+    float2 z = make_float2(0.f, 0.f);
+    if (!InterpShape<INTERP>::valid(ir, nr) || !InterpShape<INTERP>::valid(iu, nu)) return z;
+    if (INTERP == INTERP_KAISER8) {
+        float wr[8], wu[8];
+        kaiser_weights_f32(ktab, tr, wr);
+        kaiser_weights_f32(ktab, tu, wu);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float2* row = img + (size_t)(ir + a - 3) * ld + (iu - 3);
+            float sx = 0.f, sy = 0.f;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const float2 v = row[b];
+                sx += wu[b] * v.x;
+                sy += wu[b] * v.y;
+            }
+            z.x += wr[a] * sx;
+            z.y += wr[a] * sy;
+        }
+        return z;
+    }
+    float wr[4], wu[4];
+    cubic_weights(tr, wr);
+    cubic_weights(tu, wu);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const float2* row = img + (size_t)(ir + a - 1) * ld + (iu - 1);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float2 v = row[b];
+            const float w = wr[a] * wu[b];
+            z.x += w * v.x;
+            z.y += w * v.y;
+        }
+    }
+    return z;
+}
+
+template <int INTERP>
+__device__ __forceinline__ double2 sample2_f64(const float2* __restrict__ img, int nr, int nu,
+                                               int ld, double qr, double qu, const float* ktab) {
+    double2 z = make_double2(0.0, 0.0);
+    int ir, iu;
+    if (!floor_index(qr, &ir) || !floor_index(qu, &iu)) return z;
+    if (!InterpShape<INTERP>::valid(ir, nr) || !InterpShape<INTERP>::valid(iu, nu)) return z;
+    const double tr = qr - (double)ir, tu = qu - (double)iu;
+    if (INTERP == INTERP_KAISER8) {
+        double wr[8], wu[8];
+        kaiser_weights_f64(kaiser_table64(ktab), tr, wr);
+        kaiser_weights_f64(kaiser_table64(ktab), tu, wu);
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) {
+                const float2 v = img[(size_t)(ir + a - 3) * ld + iu + b - 3];
+                const double w = wr[a] * wu[b];
+                z.x += w * (double)v.x;
+                z.y += w * (double)v.y;
+            }
+        return z;
+    }
+    double wr[4], wu[4];
+    cubic_weights(tr, wr);
+    cubic_weights(tu, wu);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float2 v = img[(size_t)(ir + a - 1) * ld + iu + b - 1];
+            const double w = wr[a] * wu[b];
+            z.x += w * (double)v.x;
+            z.y += w * (double)v.y;
+        }
+    return z;
+}
+
+// ---- one child regrid at destination pixel p (fast_bp/ffbp.py:79-95) -------
+template <int MODE, int INTERP>
+__device__ __forceinline__ float2 regrid_pixel(const float2* __restrict__ src, const Geo& sg,
+                                               int snr, int snu, const Geo& dg, int nu,
+                                               long long p, int side, double k,
+                                               const float* ktab) {
+    const int ir = (int)(p / nu);
+    const int iu = (int)(p - (long long)ir * nu);
+    if (MODE == MODE_FP32) {
+        const float r = (float)dg.r0 + (float)ir * (float)dg.dr;
+        const float u = (float)dg.u0 + (float)iu * (float)dg.du;
+        float x, y;
+        polar_pixel_f32((float)dg.ox, (float)dg.oy, (float)dg.oz, r, u, side, &x, &y);
+        const float dx = x - (float)sg.ox, dy = y - (float)sg.oy;
+        const float g = sqrtf(dx * dx + dy * dy);
+        if (g == 0.0f) return make_float2(0.f, 0.f);
+        const float soz = (float)sg.oz;
+        const float rr = sqrtf(g * g + soz * soz);
+        const float uu = dy / g;
+        const float qr = (rr - (float)sg.r0) / (float)sg.dr;
+        const float qu = (uu - (float)sg.u0) / (float)sg.du;
+        int i0, i1;
+        float2 v = make_float2(0.f, 0.f);
+        if (floor_index_f32(qr, &i0) && floor_index_f32(qu, &i1))
+            v = sample2_f32w<INTERP>(src, snr, snu, snu, i0, i1, qr - (float)i0, qu - (float)i1, ktab);
+        float s, c;
+        sincosf((float)k * (rr - r), &s, &c);
+        return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+    }
+    const double r = dg.r0 + (double)ir * dg.dr;
+    const double u = dg.u0 + (double)iu * dg.du;
+    double x, y;
+    polar_pixel(dg.ox, dg.oy, dg.oz, r, u, side, &x, &y);
+    const double dx = x - sg.ox, dy = y - sg.oy;
+    const double g = sqrt(dx * dx + dy * dy);
+    if (g == 0.0) return make_float2(0.f, 0.f);
+    const double rr = sqrt(g * g + sg.oz * sg.oz);
+    const double uu = dy / g;
+    const double qr = (rr - sg.r0) / sg.dr;
+    const double qu = (uu - sg.u0) / sg.du;
+    if (MODE == MODE_FP64) {
+        const double2 v = sample2_f64<INTERP>(src, snr, snu, snu, qr, qu, ktab);
+        double s, c;
+        sincos(k * (rr - r), &s, &c);
+        return make_float2((float)(v.x * c - v.y * s), (float)(v.x * s + v.y * c));
+    }
+    int i0, i1;
+    float2 v = make_float2(0.f, 0.f);
+    if (floor_index(qr, &i0) && floor_index(qu, &i1))
+        v = sample2_f32w<INTERP>(src, snr, snu, snu, i0, i1, (float)(qr - (double)i0),
+                                 (float)(qu - (double)i1), ktab);
+    const float2 rot = cis_turns64(k * INV_TWO_PI * (rr - r));
+    return make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+}
+
+template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void regrid_kernel(const float2* __restrict__ src, Geo sg,
+                                                     int snr, int snu, Geo dg, int nr, int nu,
+                                                     int side, double k, int accumulate,
+                                                     float2* __restrict__ out, const float* ktab) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nr * nu) return;
+    float2 v = regrid_pixel<MODE, INTERP>(src, sg, snr, snu, dg, nu, p, side, k, ktab);
+    if (accumulate) {
+        const float2 o = out[p];
+        v = make_float2(__fadd_rn(o.x, v.x), __fadd_rn(o.y, v.y));
+    }
+    out[p] = v;
+}
+
+// Both children rounded to complex64, then an unfused FP32 add
+// (fast_bp/gpu/resident_kernels.cu:1-4,53-54).
+template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void merge2_kernel(const float2* __restrict__ left, Geo lg,
+                                                     int lnr, int lnu,
+                                                     const float2* __restrict__ right, Geo rg,
+                                                     int rnr, int rnu, Geo dg, int nr, int nu,
+                                                     int side, double k, float2* __restrict__ out,
+                                                     const float* ktab) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nr * nu) return;
+    const float2 a = regrid_pixel<MODE, INTERP>(left, lg, lnr, lnu, dg, nu, p, side, k, ktab);
+    const float2 b = regrid_pixel<MODE, INTERP>(right, rg, rnr, rnu, dg, nu, p, side, k, ktab);
+    out[p] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+}
+
+// Level-batched merges: blocks = (node, first_pixel) pairs, 256 pixels each.
+template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void batched_merge_kernel(const int* __restrict__ node_int,
+                                                            const double* __restrict__ node_geo,
+                                                            const long long* __restrict__ offsets,
+                                                            float2* arena,
+                                                            const int* __restrict__ blocks,
+                                                            double k, const float* ktab) {
+    const int node = blocks[2 * blockIdx.x];
+    const long long p = (long long)blocks[2 * blockIdx.x + 1] + threadIdx.x;
+    const int* ni = node_int + 8 * node;
+    const int nr = ni[2], nu = ni[3], side = ni[4], l = ni[5], r = ni[6];
+    if (p >= (long long)nr * nu) return;
+    const Geo dg = make_geo(node_geo + 7 * node);
+    const Geo lg = make_geo(node_geo + 7 * l);
+    const Geo rg = make_geo(node_geo + 7 * r);
+    const int* li = node_int + 8 * l;
+    const int* ri = node_int + 8 * r;
+    const float2 a = regrid_pixel<MODE, INTERP>(arena + offsets[l], lg, li[2], li[3], dg, nu, p,
+                                                side, k, ktab);
+    const float2 b = regrid_pixel<MODE, INTERP>(arena + offsets[r], rg, ri[2], ri[3], dg, nu, p,
+                                                side, k, ktab);
+    arena[offsets[node] + p] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+}
+
+// ---- polar -> points (fast_bp/ffbp.py:189-198); point height ignored ------
+template <int MODE, int INTERP>
+__device__ __forceinline__ float2 eval_pixel(const float2* __restrict__ img, const Geo& g, int nr,
+                                             int nu, int ld, double x, double y, double k,
+                                             const float* ktab) {
+    if (MODE == MODE_FP32) {
+        const float dx = (float)x - (float)g.ox, dy = (float)y - (float)g.oy;
+        const float gg = sqrtf(dx * dx + dy * dy);
+        if (gg == 0.f) return make_float2(0.f, 0.f);
+        const float oz = (float)g.oz;
+        const float r = sqrtf(gg * gg + oz * oz);
+        const float u = dy / gg;
+        const float qr = (r - (float)g.r0) / (float)g.dr, qu = (u - (float)g.u0) / (float)g.du;
+        int i0, i1;
+        float2 v = make_float2(0.f, 0.f);
+        if (floor_index_f32(qr, &i0) && floor_index_f32(qu, &i1))
+            v = sample2_f32w<INTERP>(img, nr, nu, ld, i0, i1, qr - (float)i0, qu - (float)i1, ktab);
+        float s, c;
+        sincosf((float)k * r, &s, &c);
+        return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+    }
+    const double dx = x - g.ox, dy = y - g.oy;
+    const double gg = sqrt(dx * dx + dy * dy);
+    if (gg == 0.0) return make_float2(0.f, 0.f);
+    const double r = sqrt(gg * gg + g.oz * g.oz);
+    const double u = dy / gg;
+    const double qr = (r - g.r0) / g.dr, qu = (u - g.u0) / g.du;
+    if (MODE == MODE_FP64) {
+        const double2 v = sample2_f64<INTERP>(img, nr, nu, ld, qr, qu, ktab);
+        double s, c;
+        sincos(k * r, &s, &c);
+        return make_float2((float)(v.x * c - v.y * s), (float)(v.x * s + v.y * c));
+    }
+    int i0, i1;
+    float2 v = make_float2(0.f, 0.f);
+    if (floor_index(qr, &i0) && floor_index(qu, &i1))
+        v = sample2_f32w<INTERP>(img, nr, nu, ld, i0, i1, (float)(qr - (double)i0),
+                                 (float)(qu - (double)i1), ktab);
+    const float2 rot = cis_turns64(k * INV_TWO_PI * r);
+    return make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+}
+
+template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void eval_points_kernel(const float2* __restrict__ img, Geo g,
+                                                          int nr, int nu, int ld,
+                                                          const double* __restrict__ xyz,
+                                                          long long n, double k,
+                                                          float2* __restrict__ out,
+                                                          const float* ktab) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    out[p] = eval_pixel<MODE, INTERP>(img, g, nr, nu, ld, xyz[3 * p], xyz[3 * p + 1], k, ktab);
+}
+
+template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void eval_grid_kernel(const float2* __restrict__ img, Geo g,
+                                                        int nr, int nu, int ld, double x0,
+                                                        double dx, int nx, double y0, double dy,
+                                                        int ny, double k, float2* __restrict__ out,
+                                                        const float* ktab) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nx * ny) return;
+    const int iy = (int)(p / nx), ix = (int)(p - (long long)iy * nx);
+    out[p] = eval_pixel<MODE, INTERP>(img, g, nr, nu, ld, x0 + ix * dx, y0 + iy * dy, k, ktab);
+}
+
+// ---- per-(point, pulse) contributions (fast_bp/ffbp.py:201-211) ------------
+template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void contrib_kernel(const float2* __restrict__ data, int S,
+                                                      const double* __restrict__ pos, int P,
+                                                      const double* __restrict__ xyz,
+                                                      long long M, double rs, double ddr,
+                                                      double k, float2* __restrict__ out,
+                                                      const float* ktab) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * (long long)P) return;
+    const long long p = idx / P;
+    const int n = (int)(idx - p * P);
+    const double dx = xyz[3 * p] - pos[3 * n], dy = xyz[3 * p + 1] - pos[3 * n + 1],
+                 dz = xyz[3 * p + 2] - pos[3 * n + 2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double q = (d - rs) / ddr;
+    const float2* row = data + (size_t)n * S;
+    if (MODE == MODE_FP64) {
+        const double2 v = sample1_f64<INTERP>(row, S, q);
+        double s, c;
+        sincos(k * d, &s, &c);
+        out[idx] = make_float2((float)(v.x * c - v.y * s), (float)(v.x * s + v.y * c));
+        return;
+    }
+    int i;
+    float2 v = make_float2(0.f, 0.f);
+    if (floor_index(q, &i)) v = sample1_f32w<INTERP>(row, S, i, (float)(q - (double)i), ktab);
+    const float2 rot = cis_turns64(k * INV_TWO_PI * d);
+    out[idx] = make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+}
+
+// ---- phase-LUT direct BP (precision_probe.cu:106-164) ----------------------
+__global__ void fill_lut_kernel(int samples, double rs, double ddr, double k, float2* lut) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= samples) return;
+    double s, c;
+    sincos(k * (rs + (double)i * ddr), &s, &c);
+    lut[i] = make_float2((float)c, (float)s);
+}
+
+__global__ __launch_bounds__(256) void bp_polar_lut_kernel(const float2* __restrict__ data, int S,
+                                                           const double* __restrict__ pos,
+                                                           const double* __restrict__ weight,
+                                                           const double* __restrict__ phase,
+                                                           int pb, int pe, Geo g, int nr, int nu,
+                                                           int side, double rs, double ddr,
+                                                           double k, const float2* __restrict__ lut,
+                                                           float2* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nr * nu) return;
+    const int ir = (int)(p / nu), iu = (int)(p - (long long)ir * nu);
+    const double r = g.r0 + ir * g.dr, u = g.u0 + iu * g.du;
+    double x, y;
+    polar_pixel(g.ox, g.oy, g.oz, r, u, side, &x, &y);
+    double rsd, rcd;
+    sincos(-k * r, &rsd, &rcd);
+    const float ref_s = (float)rsd, ref_c = (float)rcd;
+    float2 total = out[p];
+    for (int n = pb; n < pe; ++n) {
+        const double dx = x - pos[3 * n], dy = y - pos[3 * n + 1], dz = pos[3 * n + 2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const double q = (d - rs) / ddr;
+        int i;
+        if (!floor_index(q, &i) || i < 1 || i + 2 >= S) continue;
+        const double f = q - (double)i;
+        float w4[4];
+        cubic_weights((float)f, w4);
+        const float2* row = data + (size_t)n * S;
+        float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float2 t = row[i + a - 1];
+            v.x += w4[a] * t.x;
+            v.y += w4[a] * t.y;
+        }
+        float is, ic;
+        sincosf((float)(k * ddr * f + (phase ? phase[n] : 0.0)), &is, &ic);
+        const float2 base = lut[i];
+        const float bc = base.x * ic - base.y * is;
+        const float bs = base.x * is + base.y * ic;
+        const float c = bc * ref_c - bs * ref_s;
+        const float s = bc * ref_s + bs * ref_c;
+        const float w = weight ? (float)weight[n] : 1.f;
+        total.x += w * (v.x * c - v.y * s);
+        total.y += w * (v.x * s + v.y * c);
+    }
+    out[p] = total;
+}
+
+// ---- utilities ---------------------------------------------------------------
+__global__ void add_kernel(float2* __restrict__ left, const float2* __restrict__ right, long long n) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float2 a = left[p], b = right[p];
+    left[p] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+}
+
+__global__ void cast_kernel(const double2* __restrict__ in, float2* __restrict__ out, long long n) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double2 z = in[p];
+    out[p] = make_float2((float)z.x, (float)z.y);
+}
+
+__global__ void crop_kernel(const float2* __restrict__ in, int src_nu, int rb, int ub, int nr,
+                            int nu, float2* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nr * nu) return;
+    const int ir = (int)(p / nu), iu = (int)(p - (long long)ir * nu);
+    out[p] = in[(size_t)(ir + rb) * src_nu + iu + ub];
+}
+
+__global__ void sincos_probe_kernel(const float* t, float* s, float* c, long long n) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    sincos_turns(t[p], &s[p], &c[p]);
+}
+
+// ---- dispatch helpers --------------------------------------------------------
+static int check_mode_interp(int mode, int interp) {
+    if (mode < 0 || mode > BPMI_MODE_SPLIT)
+        return set_error(BPMI_ERR_INVALID, "mode must be one of fp64, fp32, mixed, dfloat, split");
+    if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_KAISER8)
+        return set_error(BPMI_ERR_INVALID, "2-D interpolation supports lagrange4 and kaiser8");
+    if (interp == BPMI_INTERP_KAISER8 && !g_kaiser_table)
+        return set_error(BPMI_ERR_INVALID, "kaiser8 needs bpmi_set_kaiser_table() first");
+    return BPMI_OK;
+}
+
+// Calls F.template operator()<MODE, INTERP>() for the runtime (mode, interp).
+template <typename F>
+static void dispatch2(int mode, int interp, F&& f) {
+    const bool kai = interp == BPMI_INTERP_KAISER8;
+    switch (mode) {
+        case BPMI_MODE_FP64:
+            kai ? f.template operator()<MODE_FP64, INTERP_KAISER8>()
+                : f.template operator()<MODE_FP64, INTERP_LAGRANGE4>();
+            break;
+        case BPMI_MODE_FP32:
+            kai ? f.template operator()<MODE_FP32, INTERP_KAISER8>()
+                : f.template operator()<MODE_FP32, INTERP_LAGRANGE4>();
+            break;
+        default:
+            kai ? f.template operator()<MODE_MIXED, INTERP_KAISER8>()
+                : f.template operator()<MODE_MIXED, INTERP_LAGRANGE4>();
+            break;
+    }
+}
+
+}  // namespace bpmi
+
+using namespace bpmi;
+
+extern "C" {
+
+int bpmi_set_kaiser_table(const double* host_table, int len) {
+    if (!host_table || len != KAISER_TABLE_LEN)
+        return set_error(BPMI_ERR_INVALID, "kaiser table must hold %d doubles", KAISER_TABLE_LEN);
+    // one allocation: FP32 rounding of the table, then the FP64 table
+    const size_t f32_bytes = sizeof(float) * KAISER_TABLE_LEN;
+    const size_t bytes = f32_bytes + sizeof(double) * KAISER_TABLE_LEN;
+    unsigned char host[sizeof(float) * KAISER_TABLE_LEN + sizeof(double) * KAISER_TABLE_LEN];
+    float* h32 = (float*)host;
+    for (int i = 0; i < KAISER_TABLE_LEN; ++i) h32[i] = (float)host_table[i];
+    memcpy(host + f32_bytes, host_table, sizeof(double) * KAISER_TABLE_LEN);
+    if (!g_kaiser_table) {
+        int st = check_hip(hipMalloc((void**)&g_kaiser_table, bytes), "hipMalloc(kaiser table)");
+        if (st) return st;
+    }
+    return check_hip(hipMemcpy(g_kaiser_table, host, bytes, hipMemcpyHostToDevice),
+                     "hipMemcpy(kaiser table)");
+}
+
+int bpmi_regrid(const void* src, const double* src_geo, int snr, int snu, const double* dst_geo,
+                int nr, int nu, int side, double k, int mode, int interp, int flags, void* out,
+                void* stream) {
+    int st = check_mode_interp(mode, interp);
+    if (st) return st;
+    if (!src || !out || !src_geo || !dst_geo) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0 || snr <= 0 || snu <= 0)
+        return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    const Geo sg = make_geo(src_geo), dg = make_geo(dst_geo);
+    const dim3 grid(grid_1d((long long)nr * nu, 256));
+    const int acc = (flags & BPMI_ACCUMULATE) ? 1 : 0;
+    hipStream_t hs = (hipStream_t)stream;
+    const float* kt = g_kaiser_table;
+    const float2* s = (const float2*)src;
+    float2* o = (float2*)out;
+    auto l = [&]<int M, int I>() {
+        hipLaunchKernelGGL((regrid_kernel<M, I>), grid, dim3(256), 0, hs, s, sg, snr, snu, dg, nr,
+                           nu, side, k, acc, o, kt);
+    };
+    dispatch2(mode, interp, l);
+    return check_launch("regrid_kernel");
+}
+
+int bpmi_merge2(const void* left, const double* left_geo, int lnr, int lnu, const void* right,
+                const double* right_geo, int rnr, int rnu, const double* dst_geo, int nr, int nu,
+                int side, double k, int mode, int interp, void* out, void* stream) {
+    int st = check_mode_interp(mode, interp);
+    if (st) return st;
+    if (!left || !right || !out || !left_geo || !right_geo || !dst_geo)
+        return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0 || lnr <= 0 || lnu <= 0 || rnr <= 0 || rnu <= 0)
+        return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    const float2* a = (const float2*)left;
+    const float2* b = (const float2*)right;
+    const Geo ag = make_geo(left_geo), bg = make_geo(right_geo), dg = make_geo(dst_geo);
+    float2* o = (float2*)out;
+    const float* kt = g_kaiser_table;
+    const dim3 g(grid_1d((long long)nr * nu, 256));
+    hipStream_t hs = (hipStream_t)stream;
+    auto l = [&]<int M, int I>() {
+        hipLaunchKernelGGL((merge2_kernel<M, I>), g, dim3(256), 0, hs, a, ag, lnr, lnu, b, bg,
+                           rnr, rnu, dg, nr, nu, side, k, o, kt);
+    };
+    dispatch2(mode, interp, l);
+    return check_launch("merge2_kernel");
+}
+
+int bpmi_ffbp_merges(const int* node_int, const double* node_geo, const long long* offsets,
+                     void* arena, const int* blocks, int nblocks, double k, int mode, int interp,
+                     void* stream) {
+    int st = check_mode_interp(mode, interp);
+    if (st) return st;
+    if (nblocks <= 0) return BPMI_OK;
+    if (!node_int || !node_geo || !offsets || !arena || !blocks)
+        return set_error(BPMI_ERR_INVALID, "null table");
+    float2* ar = (float2*)arena;
+    const float* kt = g_kaiser_table;
+    hipStream_t hs = (hipStream_t)stream;
+    auto l = [&]<int M, int I>() {
+        hipLaunchKernelGGL((batched_merge_kernel<M, I>), dim3(nblocks), dim3(256), 0, hs, node_int,
+                           node_geo, offsets, ar, blocks, k, kt);
+    };
+    dispatch2(mode, interp, l);
+    return check_launch("batched_merge_kernel");
+}
+
+int bpmi_eval_points(const void* img, const double* geo, int nr, int nu, int ld, const double* xyz,
+                     long long npoints, double k, int mode, int interp, void* out, void* stream) {
+    int st = check_mode_interp(mode, interp);
+    if (st) return st;
+    if (npoints <= 0) return BPMI_OK;
+    if (!img || !geo || !xyz || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0 || ld < nu) return set_error(BPMI_ERR_INVALID, "bad image shape");
+    const float2* im = (const float2*)img;
+    const Geo g = make_geo(geo);
+    float2* o = (float2*)out;
+    const float* kt = g_kaiser_table;
+    const dim3 gr(grid_1d(npoints, 256));
+    hipStream_t hs = (hipStream_t)stream;
+    auto l = [&]<int M, int I>() {
+        hipLaunchKernelGGL((eval_points_kernel<M, I>), gr, dim3(256), 0, hs, im, g, nr, nu, ld,
+                           xyz, npoints, k, o, kt);
+    };
+    dispatch2(mode, interp, l);
+    return check_launch("eval_points_kernel");
+}
+
+int bpmi_eval_grid(const void* img, const double* geo, int nr, int nu, int ld, double x0,
+                   double dx, int nx, double y0, double dy, int ny, double k, int mode, int interp,
+                   void* out, void* stream) {
+    int st = check_mode_interp(mode, interp);
+    if (st) return st;
+    if (!img || !geo || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0 || ld < nu || nx <= 0 || ny <= 0)
+        return set_error(BPMI_ERR_INVALID, "bad image shape");
+    const float2* im = (const float2*)img;
+    const Geo g = make_geo(geo);
+    float2* o = (float2*)out;
+    const float* kt = g_kaiser_table;
+    const dim3 gr(grid_1d((long long)nx * ny, 256));
+    hipStream_t hs = (hipStream_t)stream;
+    auto l = [&]<int M, int I>() {
+        hipLaunchKernelGGL((eval_grid_kernel<M, I>), gr, dim3(256), 0, hs, im, g, nr, nu, ld, x0,
+                           dx, nx, y0, dy, ny, k, o, kt);
+    };
+    dispatch2(mode, interp, l);
+    return check_launch("eval_grid_kernel");
+}
+
+int bpmi_target_contributions(const void* data, int samples, const double* pos, int pulses,
+                              const double* xyz, long long npoints, double range_start,
+                              double data_dr, double k, int mode, int interp, void* out,
+                              void* stream) {
+    if (!data || !pos || !xyz || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (samples <= 0 || pulses < 0 || npoints < 0) return set_error(BPMI_ERR_INVALID, "bad sizes");
+    if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
+        return set_error(BPMI_ERR_INVALID, "contributions support lagrange4 and linear");
+    const long long n = npoints * (long long)pulses;
+    if (n == 0) return BPMI_OK;
+    const dim3 g(grid_1d(n, 256));
+    hipStream_t hs = (hipStream_t)stream;
+    const bool f64 = mode == BPMI_MODE_FP64;
+    if (interp == BPMI_INTERP_LAGRANGE4) {
+        if (f64)
+            hipLaunchKernelGGL((contrib_kernel<MODE_FP64, INTERP_LAGRANGE4>), g, dim3(256), 0, hs,
+                               (const float2*)data, samples, pos, pulses, xyz, npoints,
+                               range_start, data_dr, k, (float2*)out, nullptr);
+        else
+            hipLaunchKernelGGL((contrib_kernel<MODE_MIXED, INTERP_LAGRANGE4>), g, dim3(256), 0,
+                               hs, (const float2*)data, samples, pos, pulses, xyz, npoints,
+                               range_start, data_dr, k, (float2*)out, nullptr);
+    } else {
+        if (f64)
+            hipLaunchKernelGGL((contrib_kernel<MODE_FP64, INTERP_LINEAR>), g, dim3(256), 0, hs,
+                               (const float2*)data, samples, pos, pulses, xyz, npoints,
+                               range_start, data_dr, k, (float2*)out, nullptr);
+        else
+            hipLaunchKernelGGL((contrib_kernel<MODE_MIXED, INTERP_LINEAR>), g, dim3(256), 0, hs,
+                               (const float2*)data, samples, pos, pulses, xyz, npoints,
+                               range_start, data_dr, k, (float2*)out, nullptr);
+    }
+    return check_launch("contrib_kernel");
+}
+
+int bpmi_fill_range_phase_lut(int samples, double range_start, double data_dr, double k, void* lut,
+                              void* stream) {
+    if (samples <= 0 || !lut) return set_error(BPMI_ERR_INVALID, "bad LUT arguments");
+    hipLaunchKernelGGL(fill_lut_kernel, dim3(grid_1d(samples, 256)), dim3(256), 0,
+                       (hipStream_t)stream, samples, range_start, data_dr, k, (float2*)lut);
+    return check_launch("fill_lut_kernel");
+}
+
+int bpmi_bp_polar_lut(const void* data, int samples, const double* pos, const double* weight,
+                      const double* phase, int pulse_begin, int pulse_end, const double* geo,
+                      int nr, int nu, int side, double range_start, double data_dr, double k,
+                      const void* lut, void* out, void* stream) {
+    if (!data || !pos || !geo || !lut || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    if (pulse_begin < 0 || pulse_end < pulse_begin) return set_error(BPMI_ERR_INVALID, "bad pulse range");
+    hipLaunchKernelGGL(bp_polar_lut_kernel, dim3(grid_1d((long long)nr * nu, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float2*)data, samples, pos, weight, phase,
+                       pulse_begin, pulse_end, make_geo(geo), nr, nu, side, range_start, data_dr,
+                       k, (const float2*)lut, (float2*)out);
+    return check_launch("bp_polar_lut_kernel");
+}
+
+int bpmi_add_inplace(void* left, const void* right, long long n, void* stream) {
+    if (n <= 0) return BPMI_OK;
+    if (!left || !right) return set_error(BPMI_ERR_INVALID, "null argument");
+    hipLaunchKernelGGL(add_kernel, dim3(grid_1d(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (float2*)left, (const float2*)right, n);
+    return check_launch("add_kernel");
+}
+
+int bpmi_cast_c128_c64(const void* in, void* out, long long n, void* stream) {
+    if (n <= 0) return BPMI_OK;
+    if (!in || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    hipLaunchKernelGGL(cast_kernel, dim3(grid_1d(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const double2*)in, (float2*)out, n);
+    return check_launch("cast_kernel");
+}
+
+int bpmi_crop(const void* in, int src_nu, int r_begin, int u_begin, int nr, int nu, void* out,
+              void* stream) {
+    if (!in || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    if (r_begin < 0 || u_begin < 0 || u_begin + nu > src_nu)
+        return set_error(BPMI_ERR_INVALID, "crop window outside the source");
+    hipLaunchKernelGGL(crop_kernel, dim3(grid_1d((long long)nr * nu, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float2*)in, src_nu, r_begin, u_begin, nr, nu,
+                       (float2*)out);
+    return check_launch("crop_kernel");
+}
+
+int bpmi_probe_sincos_turns(const float* t, float* s, float* c, long long n, void* stream) {
+    if (n <= 0) return BPMI_OK;
+    if (!t || !s || !c) return set_error(BPMI_ERR_INVALID, "null argument");
+    hipLaunchKernelGGL(sincos_probe_kernel, dim3(grid_1d(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, t, s, c, n);
+    return check_launch("sincos_probe_kernel");
+}
+
+}  // extern "C"

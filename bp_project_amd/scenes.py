@@ -229,7 +229,8 @@ def stripmap_scene(pulses, bins, n_targets, grid_n, grid_spacing, seed, *, groun
         rbin = r_start + data_dr * idx
         vals = amp[t] * np.exp(-1j * k * d)[:, None] * np.sinc((rbin - d[:, None]) / res)
         rows = np.broadcast_to(np.arange(pulses)[:, None], idx.shape)
-        np.add.at(data, (rows[valid], idx[valid]), vals[valid].astype(np.complex64))
+        # (row, bin) pairs are distinct within one target, so += is exact
+        data[rows[valid], idx[valid]] += vals[valid].astype(np.complex64)
     phase_error = np.zeros(pulses)
     if phase_walk_rms > 0:
         walk = np.cumsum(rng.standard_normal(pulses))

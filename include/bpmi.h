@@ -1,0 +1,217 @@
+/*
+ * libbpmi -- MI355X (gfx950) SAR backprojection and FFBP, C ABI.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses;
+ * "host" arrays are read during the call.  Every function returns 0 (BPMI_OK)
+ * on success and a nonzero status otherwise; bpmi_last_error() returns the
+ * message of the calling thread's last failure.  Work is enqueued on the
+ * given stream (NULL = the null stream) and never synchronises unless the
+ * function name says so.  The library allocates nothing implicitly: the
+ * caller owns every buffer.
+ *
+ * Complex data are interleaved complex64 (float2) unless stated.  Node
+ * geometry ``geo`` is a host array of 7 doubles (ox, oy, oz, r0, dr, u0, du):
+ * origin, first slant range and step, first u = sin(angle) and step
+ * (fast_bp/ffbp.py:98-111).  Polar images are [nr, nu] with u fastest.
+ *
+ * Python binding: bp_project_amd/_lib.py (ctypes).  The reference functions
+ * each entry point replaces are cited beside it.
+ */
+#ifndef BPMI_H
+#define BPMI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BPMI_VERSION 1
+
+/* status codes */
+#define BPMI_OK 0
+#define BPMI_ERR_INVALID 1     /* bad argument (Python maps to ValueError) */
+#define BPMI_ERR_HIP 2         /* HIP runtime error (RuntimeError) */
+#define BPMI_ERR_UNSUPPORTED 3 /* valid but unsupported combination */
+
+/* arithmetic modes (fast_bp/gpu_backend.py:57 MODES plus two additions) */
+#define BPMI_MODE_FP64 0   /* FP64 geometry, interpolation, phase, accumulation */
+#define BPMI_MODE_FP32 1   /* FP32 everything (fast_bp/gpu/kernels.cu fp32) */
+#define BPMI_MODE_MIXED 2  /* FP64 range and phase reduction, FP32 interp/accum */
+#define BPMI_MODE_DFLOAT 3 /* double-float (hi+lo FP32) range and phase */
+#define BPMI_MODE_SPLIT 4  /* LDS-tiled: FP64 per-tile reference + FP32 residual */
+
+/* interpolators */
+#define BPMI_INTERP_LAGRANGE4 0 /* fast_bp/ffbp.py:17-40 */
+#define BPMI_INTERP_LINEAR 1
+#define BPMI_INTERP_KAISER8 2   /* 8-tap Kaiser(beta=6)-windowed sinc */
+
+/* flags for BP outputs */
+#define BPMI_ACCUMULATE 1 /* out += result (else out = result) */
+#define BPMI_OUT_C128 2   /* out is complex128 (FP64 mode accumulator) */
+
+/* ---- runtime ----------------------------------------------------------- */
+int bpmi_version(void);
+int bpmi_last_error(char* buf, size_t n);
+int bpmi_device_count(int* count);
+int bpmi_set_device(int device);
+int bpmi_get_device(int* device);
+int bpmi_device_info(int device, char* name, size_t name_len, char* arch, size_t arch_len,
+                     int* cu_count, int* clock_khz, size_t* total_mem);
+int bpmi_runtime_version(int* version);
+int bpmi_mem_info(size_t* free_bytes, size_t* total_bytes);
+int bpmi_malloc(void** ptr, size_t bytes);
+int bpmi_free(void* ptr);
+int bpmi_host_alloc(void** ptr, size_t bytes);
+int bpmi_host_free(void* ptr);
+int bpmi_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int bpmi_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int bpmi_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* Kernel-based zero fill (safe inside graph capture). ptr 16-byte aligned. */
+int bpmi_zero(void* ptr, size_t bytes, void* stream);
+int bpmi_stream_create(void** stream);
+int bpmi_stream_destroy(void* stream);
+int bpmi_stream_synchronize(void* stream);
+int bpmi_device_synchronize(void);
+int bpmi_event_create(void** event);
+int bpmi_event_destroy(void* event);
+int bpmi_event_record(void* event, void* stream);
+int bpmi_event_synchronize(void* event);
+int bpmi_event_elapsed(float* ms, void* start, void* end);
+/* Graph capture of everything enqueued on ``stream`` between begin and end
+ * (resident_backend.py:250-301 capture_graph/replay). */
+int bpmi_graph_begin(void* stream);
+int bpmi_graph_end(void* stream, void** graph_exec);
+int bpmi_graph_launch(void* graph_exec, void* stream);
+int bpmi_graph_destroy(void* graph_exec);
+
+/* ---- direct backprojection --------------------------------------------- */
+/* Common inputs: data complex64 [pulses, samples]; pos f64 [pulses, 3];
+ * weight, phase f64 [pulses] (NULL = 1 and 0).  Pulses [pulse_begin,
+ * pulse_end) are summed.  range_start/data_dr define the range of sample i
+ * as range_start + i*data_dr; k = 4*pi*fc/c with signed fc. */
+
+/* Polar grid of one node, demodulated by exp(-j k r) (pixels on z = 0).
+ * Replaces bp_polar (fast_bp/ffbp.py:58-76), bp_polar_{mode} and
+ * direct_chunk_{mode} (fast_bp/gpu/kernels.cu:139-339). */
+int bpmi_bp_polar(const void* data, int samples, const double* pos, const double* weight,
+                  const double* phase, int pulse_begin, int pulse_end, const double* geo,
+                  int nr, int nu, int side, double range_start, double data_dr, double k,
+                  int mode, int interp, int flags, void* out, void* stream);
+
+/* bpmi_bp_polar with an explicit SPLIT-mode geometry reference
+ * (origin_ref = 0: each tile's centre; 1: the node origin, for sub-apertures
+ * short relative to the slant range such as FFBP leaves) and tile width
+ * (tile_cols_log2 = 3..6, or -1 to choose from the grid spacing).  Other
+ * modes ignore both. */
+int bpmi_bp_polar_ex(const void* data, int samples, const double* pos, const double* weight,
+                     const double* phase, int pulse_begin, int pulse_end, const double* geo,
+                     int nr, int nu, int side, double range_start, double data_dr, double k,
+                     int mode, int interp, int flags, void* out, void* stream, int origin_ref,
+                     int tile_cols_log2);
+
+/* Regular Cartesian grid on z = z0, image [ny, nx] with x fastest, not
+ * demodulated: exp(+j (k d + phase)).  Cartesian direct BP
+ * (backproject_points at grid pixels, fast_bp/ffbp.py:43-55). */
+int bpmi_bp_grid(const void* data, int samples, const double* pos, const double* weight,
+                 const double* phase, int pulse_begin, int pulse_end, double x0, double dx,
+                 int nx, double y0, double dy, int ny, double z0, double range_start,
+                 double data_dr, double k, int mode, int interp, int flags, void* out,
+                 void* stream);
+
+/* Arbitrary points xyz f64 [npoints, 3] (device).  Replaces
+ * backproject_points (fast_bp/ffbp.py:43-55). */
+int bpmi_bp_points(const void* data, int samples, const double* pos, const double* weight,
+                   const double* phase, int pulse_begin, int pulse_end, const double* xyz,
+                   long long npoints, double range_start, double data_dr, double k, int mode,
+                   int interp, int flags, void* out, void* stream);
+
+/* Per-(point, pulse) unweighted contributions out[npoints, pulses] complex64
+ * (fast_bp/ffbp.py:201-211 target_contributions). */
+int bpmi_target_contributions(const void* data, int samples, const double* pos, int pulses,
+                              const double* xyz, long long npoints, double range_start,
+                              double data_dr, double k, int mode, int interp, void* out,
+                              void* stream);
+
+/* ---- FFBP merge / regrid ----------------------------------------------- */
+/* One child polar image -> destination grid, out = v exp(j k (rr - r)) or
+ * out += that with BPMI_ACCUMULATE.  Replaces regrid (fast_bp/ffbp.py:79-95)
+ * and regrid_{mode} (fast_bp/gpu/kernels.cu:341-429). */
+int bpmi_regrid(const void* src, const double* src_geo, int snr, int snu,
+                const double* dst_geo, int nr, int nu, int side, double k, int mode, int interp,
+                int flags, void* out, void* stream);
+
+/* Both children in one pass, each rounded to complex64 before the FP32 sum
+ * (merge_regrid_{mode}, fast_bp/gpu/resident_kernels.cu:35-157). */
+int bpmi_merge2(const void* left, const double* left_geo, int lnr, int lnu, const void* right,
+                const double* right_geo, int rnr, int rnu, const double* dst_geo, int nr, int nu,
+                int side, double k, int mode, int interp, void* out, void* stream);
+
+/* left += right, complex64 (add_float2_inplace, kernels.cu:431-437). */
+int bpmi_add_inplace(void* left, const void* right, long long n, void* stream);
+/* complex128 -> complex64 (cast_double2_to_float2, kernels.cu:439-444). */
+int bpmi_cast_c128_c64(const void* in, void* out, long long n, void* stream);
+/* out[nr, nu] = in[r_begin + i, u_begin + j] with row length src_nu
+ * (crop_float2, kernels.cu:446-454). */
+int bpmi_crop(const void* in, int src_nu, int r_begin, int u_begin, int nr, int nu, void* out,
+              void* stream);
+
+/* Polar image (rows of length ld >= nu, so a crop is a view) -> points,
+ * remodulated by exp(+j k r) (evaluate_image, fast_bp/ffbp.py:189-198). */
+int bpmi_eval_points(const void* img, const double* geo, int nr, int nu, int ld,
+                     const double* xyz, long long npoints, double k, int mode, int interp,
+                     void* out, void* stream);
+/* Same onto a regular grid on z = 0, image [ny, nx] x fastest
+ * (the Cartesian ground image of fast_bp/run.py:118-125). */
+int bpmi_eval_grid(const void* img, const double* geo, int nr, int nu, int ld, double x0,
+                   double dx, int nx, double y0, double dy, int ny, double k, int mode, int interp,
+                   void* out, void* stream);
+
+/* ---- level-batched FFBP (batched_resident_backend.py:72-150) ----------- */
+/* node_int int32 [nodes, 8] = (a, b, nr, nu, side, left, right, height);
+ * node_geo f64 [nodes, 7]; offsets int64 [nodes] element offsets of each
+ * node image in ``arena`` (complex64).  ``blocks`` int32 [nblocks, 2] =
+ * (node, first_pixel) pairs; each block covers 256 pixels of one node.
+ * In BPMI_MODE_SPLIT the leaf table instead holds (node, tile) pairs, with
+ * bpmi_split_tiles() tiles of 1024 pixels per node.
+ * All table pointers are device pointers. */
+int bpmi_ffbp_leaves(const void* data, int samples, const double* pos, const double* weight,
+                     const double* phase, const int* node_int, const double* node_geo,
+                     const long long* offsets, void* arena, const int* blocks, int nblocks,
+                     double range_start, double data_dr, double k, int mode, int interp,
+                     void* stream);
+int bpmi_ffbp_merges(const int* node_int, const double* node_geo, const long long* offsets,
+                     void* arena, const int* blocks, int nblocks, double k, int mode, int interp,
+                     void* stream);
+/* Number of split-mode tiles (1024 pixels: 64 u-columns x 16 r-rows) of a
+ * leaf node of nr x nu pixels. */
+int bpmi_split_tiles(int nr, int nu, int* tiles);
+
+/* ---- phase-LUT direct BP (resident_precision.py:68-175) ---------------- */
+/* lut[i] = exp(j k (range_start + i data_dr)), complex64, FP64 evaluation
+ * (fill_range_phase_lut, precision_probe.cu:106-115). */
+int bpmi_fill_range_phase_lut(int samples, double range_start, double data_dr, double k,
+                              void* lut, void* stream);
+/* Mixed direct BP with the LUT phase decomposition
+ * (direct_chunk_mixed_phase_lut, precision_probe.cu:118-164); out += chunk. */
+int bpmi_bp_polar_lut(const void* data, int samples, const double* pos, const double* weight,
+                      const double* phase, int pulse_begin, int pulse_end, const double* geo,
+                      int nr, int nu, int side, double range_start, double data_dr, double k,
+                      const void* lut, void* out, void* stream);
+
+/* ---- Kaiser table and probes ------------------------------------------- */
+/* Upload the Kaiser-sinc kernel table used by BPMI_INTERP_KAISER8: host
+ * array of BPMI_KAISER_TABLE_LEN doubles, the kernel at x = i/2048.  FP64
+ * paths read it as is, the others its FP32 rounding (linear interpolation
+ * between entries, 8 weights normalised to unit sum).  Must be called once
+ * before any kaiser8 launch. */
+#define BPMI_KAISER_TABLE_LEN (4 * 2048 + 2)
+int bpmi_set_kaiser_table(const double* host_table, int len);
+/* sin/cos(2*pi*t) with the hardware transcendental path (device arrays). */
+int bpmi_probe_sincos_turns(const float* t, float* s, float* c, long long n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BPMI_H */

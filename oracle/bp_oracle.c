@@ -22,9 +22,11 @@
  *
  * Two interpolators have no reference counterpart and are specified here:
  *   linear      : (1-t)*row[i] + t*row[i+1], valid when 0 <= i <= len-2.
- *   kaiser-sinc : 8 taps at i-3..i+4, w = sinc(x)*I0(beta*sqrt(1-(x/4)^2))/I0(beta),
- *                 beta = 6, normalised by the sum of the 8 weights, valid when
- *                 3 <= i and i+4 <= len-1 (all taps inside), else 0.
+ *   kaiser-sinc : 8 taps at i-3..i+4.  The kernel
+ *                 K(x) = sinc(x)*I0(beta*sqrt(1-(x/4)^2))/I0(beta), beta = 6,
+ *                 is tabulated in FP64 at x = j/2048 (j = 0..8193) and read by
+ *                 linear interpolation; the 8 weights are normalised by their
+ *                 sum.  Valid when 3 <= i and i+4 <= len-1, else 0.
  * One documented deviation: regrid/evaluate_image return 0 where the source
  * ground distance g is exactly 0 (Numba would raise ZeroDivisionError; the
  * reference GPU kernels return 0, fast_bp/gpu/kernels.cu:359).
@@ -76,11 +78,36 @@ double orc_kaiser_sinc(double x) {
     return s * win;
 }
 
+#define KAISER_RES 2048
+#define KAISER_LEN (4 * KAISER_RES + 2)
+static double g_kaiser_table[KAISER_LEN];
+static int g_kaiser_ready = 0;
+
+static void kaiser_table_init(void) {
+    #pragma omp critical(orc_kaiser_init)
+    {
+        if (!g_kaiser_ready) {
+            for (int j = 0; j < KAISER_LEN; ++j) g_kaiser_table[j] = orc_kaiser_sinc((double)j / KAISER_RES);
+            g_kaiser_ready = 1;
+        }
+    }
+}
+
+double orc_kaiser_table_value(int j) {
+    if (!g_kaiser_ready) kaiser_table_init();
+    return (j >= 0 && j < KAISER_LEN) ? g_kaiser_table[j] : 0.0;
+}
+
 /* Weights of the 8-tap Kaiser-sinc at fraction t; taps i-3..i+4. */
 void orc_kaiser_weights(double t, double w[8]) {
+    if (!g_kaiser_ready) kaiser_table_init();
     double sum = 0.0;
     for (int a = 0; a < 8; ++a) {
-        w[a] = orc_kaiser_sinc(t - (double)(a - 3));
+        double x = fabs(t - (double)(a - 3)) * KAISER_RES;
+        int j = (int)x;
+        double f = x - (double)j;
+        if (j > KAISER_LEN - 2) j = KAISER_LEN - 2;
+        w[a] = g_kaiser_table[j] + f * (g_kaiser_table[j + 1] - g_kaiser_table[j]);
         sum += w[a];
     }
     for (int a = 0; a < 8; ++a) w[a] /= sum;

@@ -83,6 +83,8 @@ def _declare(L):
     L.orc_kaiser_sinc.argtypes = [_d]
     L.orc_kaiser_sinc.restype = _d
     L.orc_kaiser_weights.argtypes = [_d, _dp]
+    L.orc_kaiser_table_value.argtypes = [_i]
+    L.orc_kaiser_table_value.restype = _d
     L.orc_sample1.argtypes = [_vp, _l, _d, _i, _dp]
     L.orc_sample2.argtypes = [_vp, _i, _l, _l, _d, _d, _i, _dp]
     L.orc_backproject_points.argtypes = [_vp, _l, _l, _vp, _vp, _l, _d, _d, _d, _vp, _vp, _i, _vp]

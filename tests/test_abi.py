@@ -1,0 +1,89 @@
+"""The C-ABI library loads and exports every symbol include/bpmi.h declares;
+argument validation happens on the host (no GPU needed for these calls)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from bp_project_amd import _lib
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "bpmi.h"
+
+
+def header_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^int\s+(bpmi_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = header_functions()
+    assert len(names) > 40
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with bpmi.h"
+    assert lib.bpmi_version() == 1
+
+
+def test_host_side_validation_and_error_mapping():
+    lib = _lib.load()
+    geo = (ctypes.c_double * 7)()
+    st = lib.bpmi_bp_polar(None, 16, None, None, None, 0, 1, geo, 4, 4, 1, 0.0, 1.0, 1.0, 0, 0, 0,
+                           None, None)
+    assert st == _lib.ERR_INVALID
+    assert "non-null" in _lib.last_error()
+    with pytest.raises(ValueError, match="non-null"):
+        _lib.check(st)
+    with pytest.raises(ValueError, match="kaiser table"):
+        _lib.call("bpmi_set_kaiser_table", None, 3)
+    t = ctypes.c_int()
+    _lib.call("bpmi_split_tiles", 100, 130, ctypes.byref(t))
+    assert t.value == 3 * 7          # 64-column x 16-row tiles
+    with pytest.raises(ValueError, match="empty image grids"):
+        _lib.call("bpmi_split_tiles", 0, 5, ctypes.byref(t))
+    with pytest.raises(ValueError, match="mode must be one of"):
+        _lib.mode_code("fp16")
+
+
+def test_program_argument_errors_are_raised_before_device_use():
+    from bp_project_amd.resident_backend import ResidentProgram
+
+    class Dummy:
+        block_size = 128
+    with pytest.raises(ValueError, match="mode must be one of"):
+        ResidentProgram(Dummy(), None, None, "fp16", 0, 1, 1)
+    with pytest.raises(ValueError, match="algorithm must be direct or ffbp"):
+        ResidentProgram(Dummy(), None, None, "fp64", 0, 1, 1, algorithm="x")
+    with pytest.raises(ValueError, match="variant must be original or fused"):
+        ResidentProgram(Dummy(), None, None, "fp64", 0, 1, 1, variant="x")
+    with pytest.raises(ValueError, match="fused variant is only defined for FFBP"):
+        ResidentProgram(Dummy(), None, None, "fp64", 0, 1, 1, algorithm="direct", variant="fused")
+    with pytest.raises(ValueError, match="chunk_pulses must be positive"):
+        ResidentProgram(Dummy(), None, None, "fp64", 0, 1, 1, chunk_pulses=0)
+
+
+def test_lut_attachment_on_a_mock_program():
+    """resident_precision._attach_lut_to_schedule is pure Python
+    (fast_bp/resident_precision.py:44-65)."""
+    from bp_project_amd.resident_precision import _attach_lut_to_schedule
+
+    class Mock:
+        _graph = None
+    m = Mock()
+    args = tuple(range(18))
+    k = object()
+    m._schedule = [("direct_chunk_mixed", k, (1,), (128,), args)]
+    m.operations = [{"name": "direct_chunk_mixed", "kernel": k, "args": args}]
+
+    class Lut:
+        ptr = 1234
+    _attach_lut_to_schedule(m, Lut())
+    name, launcher, _, _, new_args = m._schedule[0]
+    assert name == "direct_chunk_mixed_phase_lut"
+    assert new_args[:14] == args[:14] and new_args[15] == 17
+    assert m.operations[0]["args"] is new_args
+    m._graph = object()
+    with pytest.raises(RuntimeError, match="before graph capture"):
+        _attach_lut_to_schedule(m, Lut())

@@ -1,0 +1,56 @@
+"""Metric conventions checked against the reference's published closed-form
+self-checks (sar_quality/evaluate.py:188-194, vancouver_bp/quality.py:242-278)."""
+import numpy as np
+from scipy.integrate import quad
+from scipy.optimize import brentq, minimize_scalar
+
+from bp_project_amd import metrics
+
+
+def test_impulse_metrics_sinc(golden):
+    ref = golden["metric_selfcheck"]["impulse_metrics_sinc"]
+    x = np.linspace(-40, 40, 80001)
+    r = metrics.impulse_metrics(x, np.sinc(x))
+    assert abs(r["irw_m"] - ref["irw_m"]) < 1e-9
+    assert abs(r["pslr_db"] - ref["pslr_db"]) < 1e-9
+    assert abs(r["islr_db"] - ref["islr_db"]) < 1e-9
+    assert abs(r["irw_m"] - .885893) < .001
+    assert not metrics.focus_metrics(np.zeros(100, complex))["valid"]
+
+
+def test_profile_metrics_sinc(golden):
+    ref = golden["metric_selfcheck"]["profile_metrics_sinc"]
+    scale = 5.0
+    x = np.arange(-100.0, 100.0 + .005, .01)
+    z = np.sinc(x / scale) * np.exp(1j * (0.13 * x + .7))
+    got = metrics.profile_metrics(x, z)
+    assert got["sidelobes_valid"]
+    for key in ("irw_m", "pslr_db", "islr_db"):
+        assert abs(got[key] - ref[key]) < 1e-9
+    half = brentq(lambda u: np.sinc(u) ** 2 - .5, 0, 1)
+    first = minimize_scalar(lambda u: -np.sinc(u) ** 2, bounds=(1, 2), method="bounded")
+    main = quad(lambda u: np.sinc(u) ** 2, -1, 1)[0]
+    side = 2 * quad(lambda u: np.sinc(u) ** 2, 1, 20, limit=300)[0]
+    assert abs(got["irw_m"] - 2 * half * scale) < 1e-4
+    assert abs(got["pslr_db"] - 10 * np.log10(-first.fun)) < .001
+    assert abs(got["islr_db"] - 10 * np.log10(side / main)) < .002
+    assert not metrics.profile_metrics(x, np.exp(-.5 * (x / 3) ** 2))["sidelobes_valid"]
+    assert not metrics.profile_metrics(x, np.ones_like(x))["valid"]
+
+
+def test_fidelity_and_entropy_closed_forms():
+    rng = np.random.default_rng(1)
+    ref = rng.standard_normal((20, 30)) + 1j * rng.standard_normal((20, 30))
+    f = metrics.fidelity(ref, ref * (1 + 1e-3))
+    assert abs(f["relative_complex_error"] - 1e-3) < 1e-12
+    assert f["peak_shift_pixels"] == [0, 0]
+    f = metrics.fidelity(ref, ref * np.exp(0.01j))
+    assert abs(np.deg2rad(f["phase_bias_deg"]) - 0.01) < 1e-9
+    assert f["phase_std_deg_above_minus30db"] < 1e-9
+    # entropy of a uniform image is log(N); of a single bright pixel, 0
+    assert abs(metrics.image_entropy(np.ones(64)) - np.log(64)) < 1e-12
+    z = np.zeros(64, complex)
+    z[3] = 2
+    assert metrics.image_entropy(z) == 0
+    fm = metrics.focus_metrics(np.ones((8, 8)))
+    assert abs(fm["entropy_nats"] - np.log(64)) < 1e-12
