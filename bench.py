@@ -66,6 +66,8 @@ def parse():
     p.add_argument("--range-spacing", type=float, default=0.25)
     p.add_argument("--angular-oversample", type=float, default=2.5)
     p.add_argument("--merge-interp", default="kaiser8")
+    p.add_argument("--precision-study", action="store_true",
+                   help="direct workloads: compare fp32/mixed/dfloat/split against fp64")
     return p.parse_args()
 
 
@@ -306,20 +308,43 @@ def run_direct(args, scene, name):
               "config": {"workload": name, "pulses": scene.pulses, "bins": scene.bins,
                          "output": [g.ny, g.nx], "pixel_m": g.dx, "interp": interp,
                          "mode": args.mode}}
-    if not args.no_quality:
+    if not args.no_quality or args.precision_study:
+        from bp_project_amd.metrics import pslr_islr_deltas
         img = out.get()
         ref_out = DeviceArray.empty(g.shape)
+        t0 = time.perf_counter()
         step("fp64", ref_out)
         synchronize()
+        log(f"fp64 reference image: {time.perf_counter() - t0:.2f} s")
         ref = ref_out.get()
-        f = fidelity(ref, img)
         peaks = target_peaks(np.abs(ref), g, scene.targets)
-        ph = peak_phase_error(ref, img, peaks)
-        result["quality"] = {"reference": "FP64 direct BP on the GPU",
-                             "nrmse": f["relative_complex_error"],
-                             "max_peak_phase_error_rad": float(max(ph)) if ph else None,
-                             "passes_nrmse_1e-4": bool(f["relative_complex_error"] <= 1e-4)}
+
+        def assess(image):
+            f = fidelity(ref, image)
+            ph = peak_phase_error(ref, image, peaks)
+            pis = pslr_islr_deltas(ref, image, g, peaks[:16])
+            return {"nrmse": f["relative_complex_error"],
+                    "max_peak_phase_error_rad": float(max(ph)) if ph else None,
+                    "max_abs_pslr_islr_delta_db": pis["max_abs_delta_db"],
+                    "passes_nrmse_1e-4": bool(f["relative_complex_error"] <= 1e-4)}
+        result["quality"] = dict(reference="FP64 direct BP on the GPU", **assess(img))
         log("quality:", json.dumps(result["quality"]))
+        if args.precision_study:
+            study = {}
+            for mode in ("fp32", "mixed", "dfloat", "split"):
+                dst = DeviceArray.empty(g.shape)
+                t0 = time.perf_counter()
+                start, end = Event(), Event()
+                start.record()
+                step(mode, dst)
+                end.record()
+                synchronize()
+                ms_mode = elapsed_ms(start, end)
+                study[mode] = dict(ms=round(ms_mode, 3),
+                                   gpix_pulse_per_s=round(pp / (ms_mode * 1e-3) / 1e9, 2),
+                                   **assess(dst.get()))
+                log(f"precision study {mode}: {json.dumps(study[mode])}")
+            result["precision_study"] = study
     return result
 
 
@@ -374,6 +399,8 @@ def main():
     }
     if "quality" in res:
         line["quality"] = res["quality"]
+    if "precision_study" in res:
+        line["precision_study"] = res["precision_study"]
     if "stages" in res:
         line["stages_ms_per_step"] = {k: round(v["ms_per_step"], 4) for k, v in res["stages"].items()}
     print(json.dumps(line), flush=True)

@@ -75,10 +75,18 @@ SIGNATURES = {
                          _vp],
     "bpmi_ffbp_merges": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _i, _vp],
     "bpmi_split_tiles": [_i, _i, POINTER(c_int)],
+    "bpmi_split_tiles_ex": [_i, _i, _i, POINTER(c_int)],
+    "bpmi_ffbp_leaves_multi": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp],
+    "bpmi_eval_grid_multi": [_vp, _vp, _i, _i, _i, _vp],
     "bpmi_fill_range_phase_lut": [_i, _d, _d, _d, _vp, _vp],
     "bpmi_bp_polar_lut": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _d, _d, _vp, _vp,
                           _vp],
     "bpmi_set_kaiser_table": [_vp, _i],
+    "bpmi_adjoint_workspace_bytes": [_ll, _i, POINTER(c_size_t)],
+    "bpmi_bp_grid_adjoint": [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _i, _d, _d, _i, _d, _d, _d,
+                             _d, _i, _vp, _vp, _sz, _vp, _vp, _vp],
+    "bpmi_bp_points_adjoint": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _ll, _d, _d, _d, _i, _vp, _vp,
+                               _sz, _vp, _vp, _vp],
     "bpmi_probe_sincos_turns": [_vp, _vp, _vp, _ll, _vp],
 }
 

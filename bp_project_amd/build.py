@@ -6,6 +6,10 @@ All translation units in ``csrc/`` are compiled for gfx950 only and linked
 into ``bp_project_amd/libbpmi.so``.  Packed FP32 instructions are disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``); FP64 FMA has the
 same issue rate as unpacked FP32 on CDNA4, which the kernels exploit.
+``-ffp-contract=on`` fuses multiply-adds only within one expression, so the
+same source rounds identically in every kernel that inlines it (batched and
+per-node FFBP stay bitwise equal) and the double-float error-free
+transformations are not fused across statements.
 """
 from __future__ import annotations
 
@@ -34,7 +38,7 @@ def sources():
 
 
 def flags():
-    return [f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared",
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=on",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument", f"-I{INCLUDE}", f"-I{CSRC}"]
 

@@ -52,6 +52,18 @@ __device__ __forceinline__ float2 cis_turns64(double turns) {
     return make_float2(c, s);
 }
 
+// 1/sqrt(x) in FP64 from the hardware estimate plus two Newton steps
+// (x > 0, normal): relative error at the FP64 rounding level, no IEEE
+// square-root or division sequence.
+__device__ __forceinline__ double rsqrt64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h, y * y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-h, y * y, 0.5);
+    return fma(y, e, y);
+}
+
 // Reference association order of cubic_weights (fast_bp/ffbp.py:17-21).
 template <typename T>
 __device__ __forceinline__ void cubic_weights(T t, T w[4]) {
@@ -218,6 +230,8 @@ struct df {
     float hi, lo;
 };
 
+// The error-free transformations below must not be contracted into FMAs.
+#pragma clang fp contract(off)
 __device__ __forceinline__ df df_from(double x) {
     df r;
     r.hi = (float)x;
@@ -261,6 +275,7 @@ __device__ __forceinline__ df df_mul(df a, df b) {
 }
 
 __device__ __forceinline__ df df_sqr(df a) { return df_mul(a, a); }
+#pragma clang fp contract(on)
 
 // sqrt by one Newton step from the FP32 root; the residual is formed exactly
 // with an FMA.

@@ -250,6 +250,7 @@ struct LeafTables {
     const long long* offsets;
     float2* arena;
     const int* blocks;
+    const bpmi_node_source* sources;  // optional per-node inputs (many patches)
 };
 
 __device__ __forceinline__ void leaf_args(const LeafTables& lt, int node, BpArgs* a, SrcArgs* s) {
@@ -258,6 +259,17 @@ __device__ __forceinline__ void leaf_args(const LeafTables& lt, int node, BpArgs
     a->pulse_begin = ni[0];
     a->pulse_end = ni[1];
     a->flags = 0;
+    if (lt.sources) {
+        const bpmi_node_source& src = lt.sources[node];
+        a->data = (const float2*)src.data;
+        a->pos = src.pos;
+        a->weight = src.weight;
+        a->phase = src.phase;
+        a->range_start = src.range_start;
+        a->data_dr = src.data_dr;
+        a->k = src.k;
+        a->samples = src.samples;
+    }
     a->out = lt.arena + lt.offsets[node];
     s->a0 = g[0]; s->a1 = g[1]; s->a2 = g[2];
     s->b0 = g[3]; s->b1 = g[4]; s->b2 = g[5]; s->b3 = g[6];
@@ -829,6 +841,65 @@ extern "C" int bpmi_bp_points(const void* data, int samples, const double* pos,
     return check_launch("bp_pixel_kernel(points)");
 }
 
+extern "C" int bpmi_split_tiles_ex(int nr, int nu, int tile_cols_log2, int* tiles) {
+    if (!tiles) return set_error(BPMI_ERR_INVALID, "tiles is null");
+    if (nr <= 0 || nu <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    if (tile_cols_log2 < 3 || tile_cols_log2 > 6)
+        return set_error(BPMI_ERR_INVALID, "tile_cols_log2 must be 3..6");
+    const long long t = tiles_for(nr, nu, tile_cols_log2);
+    if (t > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "node too large");
+    *tiles = (int)t;
+    return BPMI_OK;
+}
+
+extern "C" int bpmi_ffbp_leaves_multi(const bpmi_node_source* sources, const int* node_int,
+                                      const double* node_geo, const long long* offsets,
+                                      void* arena, const int* blocks, int nblocks, int mode,
+                                      int interp, int tile_cols_log2, void* stream) {
+    if (nblocks <= 0) return BPMI_OK;
+    if (!sources || !node_int || !node_geo || !offsets || !arena || !blocks)
+        return set_error(BPMI_ERR_INVALID, "null table");
+    if (mode < 0 || mode > BPMI_MODE_SPLIT)
+        return set_error(BPMI_ERR_INVALID, "mode must be one of fp64, fp32, mixed, dfloat, split");
+    if (interp < 0 || interp > BPMI_INTERP_KAISER8) return set_error(BPMI_ERR_INVALID, "unknown interpolator");
+    if (interp == BPMI_INTERP_KAISER8 && !kaiser_table_device())
+        return set_error(BPMI_ERR_INVALID, "kaiser8 needs bpmi_set_kaiser_table() first");
+    BpArgs a{nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0.0, 1.0, 0.0, 0, nullptr};
+    LeafTables lt{node_int, node_geo, offsets, (float2*)arena, blocks, sources};
+    hipStream_t hs = (hipStream_t)stream;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (tile_cols_log2 < 3 || tile_cols_log2 > 6)
+            return set_error(BPMI_ERR_INVALID, "tile_cols_log2 must be 3..6");
+        if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
+            return set_error(BPMI_ERR_UNSUPPORTED, "split mode supports lagrange4 and linear interpolation");
+        SrcArgs s{};
+        if (interp == BPMI_INTERP_LAGRANGE4)
+            hipLaunchKernelGGL((bp_split_kernel<INTERP_LAGRANGE4, SRC_POLAR, true>), dim3(nblocks),
+                               dim3(SP_THREADS), 0, hs, a, s, tile_cols_log2, 0, lt);
+        else
+            hipLaunchKernelGGL((bp_split_kernel<INTERP_LINEAR, SRC_POLAR, true>), dim3(nblocks),
+                               dim3(SP_THREADS), 0, hs, a, s, tile_cols_log2, 0, lt);
+        return check_launch("bp_split_kernel(multi-patch leaves)");
+    }
+    const float* kt = kaiser_table_device();
+    const dim3 g(nblocks);
+    auto l = [&]<int M>() {
+        if (interp == BPMI_INTERP_LAGRANGE4)
+            hipLaunchKernelGGL((batched_leaf_kernel<M, INTERP_LAGRANGE4>), g, dim3(256), 0, hs, a, lt, kt);
+        else if (interp == BPMI_INTERP_LINEAR)
+            hipLaunchKernelGGL((batched_leaf_kernel<M, INTERP_LINEAR>), g, dim3(256), 0, hs, a, lt, kt);
+        else
+            hipLaunchKernelGGL((batched_leaf_kernel<M, INTERP_KAISER8>), g, dim3(256), 0, hs, a, lt, kt);
+    };
+    switch (mode) {
+        case BPMI_MODE_FP64: l.template operator()<MODE_FP64>(); break;
+        case BPMI_MODE_FP32: l.template operator()<MODE_FP32>(); break;
+        case BPMI_MODE_MIXED: l.template operator()<MODE_MIXED>(); break;
+        default: l.template operator()<MODE_DFLOAT>(); break;
+    }
+    return check_launch("batched_leaf_kernel(multi-patch)");
+}
+
 extern "C" int bpmi_split_tiles(int nr, int nu, int* tiles) {
     if (!tiles) return set_error(BPMI_ERR_INVALID, "tiles is null");
     if (nr <= 0 || nu <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
@@ -850,7 +921,7 @@ extern "C" int bpmi_ffbp_leaves(const void* data, int samples, const double* pos
         return set_error(BPMI_ERR_INVALID, "null table");
     BpArgs a{(const float2*)data, pos, weight, phase, samples, 0, 0, range_start, data_dr, k, 0,
              nullptr};
-    LeafTables lt{node_int, node_geo, offsets, (float2*)arena, blocks};
+    LeafTables lt{node_int, node_geo, offsets, (float2*)arena, blocks, nullptr};
     hipStream_t hs = (hipStream_t)stream;
     if (mode == BPMI_MODE_SPLIT) {
         // blocks = (node, tile) pairs; tiles are 64 u-columns x 16 r-rows and
@@ -883,4 +954,227 @@ extern "C" int bpmi_ffbp_leaves(const void* data, int samples, const double* pos
         default: l.template operator()<MODE_DFLOAT>(); break;
     }
     return check_launch("batched_leaf_kernel");
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (image -> per-pulse gradients) of the Cartesian / point BP
+// ---------------------------------------------------------------------------
+// Forward model: I(x) = sum_n w_n s_n(q_n(x)) exp(j (k d_n(x) + phase_n)),
+// d_n = |x - p_n|, q_n = (d_n - range_start) / data_dr.  Given g = dL/dI in
+// PyTorch's convention (dL = Re sum_x conj(g_x) dI_x), per pulse n:
+//   dL/dphase_n = Re sum_x conj(g_x) j c_xn,          c_xn = w s e^{j theta}
+//   dL/dp_n     = Re sum_x conj(g_x) w e^{j theta} (s'/data_dr + j k s) (p_n - x)/d
+// Range and phase are recomputed (FP64 geometry, FP32 interpolation and
+// products); nothing is stored per (pixel, pulse).  Each workgroup reduces
+// its 1024 pixels per pulse into partials[tile][pulse][5] =
+// (Re A, Im A, Gx, Gy, Gz) with A = sum conj(g) c; a second kernel sums the
+// tiles in a fixed order in FP64, so results are bitwise reproducible.
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_PPT = 4;
+constexpr int ADJ_TILE = ADJ_THREADS * ADJ_PPT;
+
+template <int INTERP>
+__device__ __forceinline__ void sample1_with_derivative(const float2* __restrict__ row, int len,
+                                                        int i, float t, float2* v, float2* dv) {
+    *v = make_float2(0.f, 0.f);
+    *dv = make_float2(0.f, 0.f);
+    if (!InterpShape<INTERP>::valid(i, len)) return;
+    if (INTERP == INTERP_LINEAR) {
+        const float2 a = row[i], b = row[i + 1];
+        v->x = a.x + t * (b.x - a.x);
+        v->y = a.y + t * (b.y - a.y);
+        dv->x = b.x - a.x;
+        dv->y = b.y - a.y;
+    } else {
+        float w[4], dw[4];
+        cubic_weights(t, w);
+        const float t2 = t * t;
+        dw[0] = -(3.f * t2 - 6.f * t + 2.f) * (1.f / 6.f);
+        dw[1] = (3.f * t2 - 4.f * t - 1.f) * 0.5f;
+        dw[2] = -(3.f * t2 - 2.f * t - 2.f) * 0.5f;
+        dw[3] = (3.f * t2 - 1.f) * (1.f / 6.f);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float2 s = row[i + a - 1];
+            v->x += w[a] * s.x;
+            v->y += w[a] * s.y;
+            dv->x += dw[a] * s.x;
+            dv->y += dw[a] * s.y;
+        }
+    }
+}
+
+template <int INTERP, int SRC>
+__global__ __launch_bounds__(ADJ_THREADS) void bp_adjoint_kernel(BpArgs a, SrcArgs s,
+                                                                 const float2* __restrict__ grad,
+                                                                 float* __restrict__ partials) {
+    __shared__ float red[4][8];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const long long base = (long long)blockIdx.x * ADJ_TILE;
+    const long long count = src_count(s, SRC);
+    const int S = a.samples;
+    const int P = a.pulse_end - a.pulse_begin;
+    const double kt = a.k * INV_TWO_PI;
+    const float k32 = (float)a.k;
+    const float invdr = (float)(1.0 / a.data_dr);
+    double px[ADJ_PPT], py[ADJ_PPT], pz[ADJ_PPT];
+    float2 g[ADJ_PPT];
+    bool val[ADJ_PPT];
+#pragma unroll
+    for (int k = 0; k < ADJ_PPT; ++k) {
+        const long long p = base + t + (long long)k * ADJ_THREADS;
+        val[k] = p < count;
+        double r;
+        src_pixel64<SRC>(s, val[k] ? p : 0, &px[k], &py[k], &pz[k], &r);
+        g[k] = val[k] ? grad[p] : make_float2(0.f, 0.f);
+    }
+    for (int n = a.pulse_begin; n < a.pulse_end; ++n) {
+        const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
+        const float2* row = a.data + (size_t)n * S;
+        const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
+        const float w = (float)pulse_weight(a.weight, n);
+        float are = 0.f, aim = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+        for (int k = 0; k < ADJ_PPT; ++k) {
+            if (!val[k]) continue;
+            const double ex = px[k] - ax, ey = py[k] - ay, ez = pz[k] - az;
+            const double d = sqrt(ex * ex + ey * ey + ez * ez);
+            const double q = (d - a.range_start) / a.data_dr;
+            int i;
+            if (!floor_index(q, &i)) continue;
+            float2 v, dv;
+            sample1_with_derivative<INTERP>(row, S, i, (float)(q - (double)i), &v, &dv);
+            float sn, cs;
+            sincos_turns((float)fract64(kt * d + ph), &sn, &cs);
+            // conj(g) * w e^{j theta}
+            const float hr = w * (g[k].x * cs + g[k].y * sn);
+            const float hi = w * (g[k].x * sn - g[k].y * cs);
+            // A += conj(g) w e^{j theta} v
+            are += hr * v.x - hi * v.y;
+            aim += hr * v.y + hi * v.x;
+            // b = conj(g) w e^{j theta} (dv/dr + j k v) / d ; only Re(b) is needed
+            const float ur = dv.x * invdr - k32 * v.y;
+            const float ui = dv.y * invdr + k32 * v.x;
+            const float invd = (float)(1.0 / d);
+            const float br = (hr * ur - hi * ui) * invd;
+            // (p_n - x) = -(ex, ey, ez)
+            gx -= br * (float)ex;
+            gy -= br * (float)ey;
+            gz -= br * (float)ez;
+        }
+        float vals[5] = {are, aim, gx, gy, gz};
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            float v = vals[j];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            vals[j] = v;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) red[wave][j] = vals[j];
+        }
+        __syncthreads();
+        if (t < 5) {
+            const float sum = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+            partials[((size_t)blockIdx.x * P + (n - a.pulse_begin)) * 5 + t] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// Fixed-order FP64 reduction over tiles: out_phase[n] = -Im A, out_pos[n] = G.
+__global__ void adjoint_reduce_kernel(const float* __restrict__ partials, int tiles, int P,
+                                      double* __restrict__ grad_phase,
+                                      double* __restrict__ grad_pos) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P) return;
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < tiles; ++b) {
+        const float* p = partials + ((size_t)b * P + n) * 5;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) s[j] += (double)p[j];
+    }
+    // dL/dphase = Re(j A) = -Im A
+    if (grad_phase) grad_phase[n] = -s[1];
+    if (grad_pos) {
+        grad_pos[3 * n] = s[2];
+        grad_pos[3 * n + 1] = s[3];
+        grad_pos[3 * n + 2] = s[4];
+    }
+}
+
+static int adjoint_launch(int interp, int src, const BpArgs& a, const SrcArgs& s, long long count,
+                          const void* grad, void* partials, size_t partials_bytes,
+                          double* grad_phase, double* grad_pos, hipStream_t hs) {
+    if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
+        return set_error(BPMI_ERR_UNSUPPORTED, "adjoint supports lagrange4 and linear");
+    if (!grad || !partials) return set_error(BPMI_ERR_INVALID, "grad and partials must be non-null");
+    const long long tiles = (count + ADJ_TILE - 1) / ADJ_TILE;
+    const int P = a.pulse_end - a.pulse_begin;
+    if (P <= 0) return BPMI_OK;
+    const size_t need = (size_t)tiles * (size_t)P * 5 * sizeof(float);
+    if (partials_bytes < need)
+        return set_error(BPMI_ERR_INVALID, "partials workspace too small: need %zu bytes", need);
+    if (tiles > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "image too large");
+    const dim3 g((unsigned)tiles);
+    if (src == SRC_GRID) {
+        if (interp == BPMI_INTERP_LAGRANGE4)
+            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LAGRANGE4, SRC_GRID>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
+        else
+            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LINEAR, SRC_GRID>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
+    } else {
+        if (interp == BPMI_INTERP_LAGRANGE4)
+            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LAGRANGE4, SRC_POINTS>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
+        else
+            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LINEAR, SRC_POINTS>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
+    }
+    int st = check_launch("bp_adjoint_kernel");
+    if (st) return st;
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(grid_1d(P, 256)), dim3(256), 0, hs,
+                       (const float*)partials, (int)tiles, P, grad_phase, grad_pos);
+    return check_launch("adjoint_reduce_kernel");
+}
+
+extern "C" int bpmi_adjoint_workspace_bytes(long long npixels, int pulses, size_t* bytes) {
+    if (!bytes) return set_error(BPMI_ERR_INVALID, "bytes is null");
+    if (npixels < 0 || pulses < 0) return set_error(BPMI_ERR_INVALID, "bad sizes");
+    *bytes = (size_t)((npixels + ADJ_TILE - 1) / ADJ_TILE) * (size_t)pulses * 5 * sizeof(float);
+    return BPMI_OK;
+}
+
+extern "C" int bpmi_bp_grid_adjoint(const void* data, int samples, const double* pos,
+                                    const double* weight, const double* phase, int pulse_begin,
+                                    int pulse_end, double x0, double dx, int nx, double y0,
+                                    double dy, int ny, double z0, double range_start,
+                                    double data_dr, double k, int interp, const void* grad_image,
+                                    void* partials, size_t partials_bytes, double* grad_phase,
+                                    double* grad_pos, void* stream) {
+    int st = validate_common(data, samples, pos, pulse_begin, pulse_end, BPMI_MODE_MIXED, interp,
+                             0, (void*)grad_image);
+    if (st) return st;
+    if (nx <= 0 || ny <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    BpArgs a{(const float2*)data, pos, weight, phase, samples, pulse_begin, pulse_end,
+             range_start, data_dr, k, 0, nullptr};
+    SrcArgs s{0.0, 0.0, z0, x0, dx, y0, dy, ny, nx, 1, nullptr, 0};
+    return adjoint_launch(interp, SRC_GRID, a, s, (long long)nx * ny, grad_image, partials,
+                          partials_bytes, grad_phase, grad_pos, (hipStream_t)stream);
+}
+
+extern "C" int bpmi_bp_points_adjoint(const void* data, int samples, const double* pos,
+                                      const double* weight, const double* phase, int pulse_begin,
+                                      int pulse_end, const double* xyz, long long npoints,
+                                      double range_start, double data_dr, double k, int interp,
+                                      const void* grad_points, void* partials,
+                                      size_t partials_bytes, double* grad_phase,
+                                      double* grad_pos, void* stream) {
+    int st = validate_common(data, samples, pos, pulse_begin, pulse_end, BPMI_MODE_MIXED, interp,
+                             0, (void*)grad_points);
+    if (st) return st;
+    if (npoints <= 0) return BPMI_OK;
+    if (!xyz) return set_error(BPMI_ERR_INVALID, "xyz must be non-null");
+    BpArgs a{(const float2*)data, pos, weight, phase, samples, pulse_begin, pulse_end,
+             range_start, data_dr, k, 0, nullptr};
+    SrcArgs s{0, 0, 0, 0, 0, 0, 0, 0, 0, 1, xyz, npoints};
+    return adjoint_launch(interp, SRC_POINTS, a, s, npoints, grad_points, partials,
+                          partials_bytes, grad_phase, grad_pos, (hipStream_t)stream);
 }

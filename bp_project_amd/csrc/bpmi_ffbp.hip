@@ -273,6 +273,19 @@ __global__ __launch_bounds__(256) void eval_points_kernel(const float2* __restri
 }
 
 template <int MODE, int INTERP>
+__global__ __launch_bounds__(256) void eval_grid_multi_kernel(const bpmi_eval_job* __restrict__ jobs,
+                                                              const int* __restrict__ blocks,
+                                                              const float* ktab) {
+    const bpmi_eval_job& j = jobs[blocks[2 * blockIdx.x]];
+    const long long p = (long long)blocks[2 * blockIdx.x + 1] + threadIdx.x;
+    if (p >= (long long)j.nx * j.ny) return;
+    const int iy = (int)(p / j.nx), ix = (int)(p - (long long)iy * j.nx);
+    const Geo g = make_geo(j.geo);
+    ((float2*)j.out)[p] = eval_pixel<MODE, INTERP>((const float2*)j.img, g, j.nr, j.nu, j.ld,
+                                                   j.x0 + ix * j.dx, j.y0 + iy * j.dy, j.k, ktab);
+}
+
+template <int MODE, int INTERP>
 __global__ __launch_bounds__(256) void eval_grid_kernel(const float2* __restrict__ img, Geo g,
                                                         int nr, int nu, int ld, double x0,
                                                         double dx, int nx, double y0, double dy,
@@ -313,6 +326,282 @@ __global__ __launch_bounds__(256) void contrib_kernel(const float2* __restrict__
     if (floor_index(q, &i)) v = sample1_f32w<INTERP>(row, S, i, (float)(q - (double)i), ktab);
     const float2 rot = cis_turns64(k * INV_TWO_PI * d);
     out[idx] = make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+}
+
+
+// ---------------------------------------------------------------------------
+// Fast (split-mode) regrid: FP64 geometry without IEEE square roots or
+// divisions (hardware reciprocal square root / reciprocal + Newton steps),
+// the destination pixel evaluated once per parent pixel, and Kaiser-sinc
+// weights from per-tap polynomials (degree 8 in s = 2t - 1, fitted by the
+// library to the tabulated kernel at upload; max weight error ~5e-7).
+// ---------------------------------------------------------------------------
+__constant__ float c_kaiser_poly[8][9];
+
+__device__ __forceinline__ void kaiser_weights_poly(float t, float w[8]) {
+    const float s = 2.f * t - 1.f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        float v = c_kaiser_poly[a][8];
+#pragma unroll
+        for (int j = 7; j >= 0; --j) v = v * s + c_kaiser_poly[a][j];
+        w[a] = v;
+    }
+}
+
+__device__ __forceinline__ double rcp64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+struct FastSrc {
+    const float2* img;
+    double ox, oy, oz2, r0, inv_dr, u0, inv_du;
+    int nr, nu, ld;
+};
+
+__device__ __forceinline__ FastSrc make_fast_src(const float2* img, const double* g, int nr, int nu,
+                                                 int ld) {
+    FastSrc f;
+    f.img = img;
+    f.ox = g[0]; f.oy = g[1]; f.oz2 = g[2] * g[2];
+    f.r0 = g[3]; f.inv_dr = rcp64(g[4]);
+    f.u0 = g[5]; f.inv_du = rcp64(g[6]);
+    f.nr = nr; f.nu = nu; f.ld = ld;
+    return f;
+}
+
+template <int INTERP>
+__device__ __forceinline__ float2 sample2_fast(const FastSrc& f, double qr, double qu) {
+    float2 z = make_float2(0.f, 0.f);
+    int ir, iu;
+    if (!floor_index(qr, &ir) || !floor_index(qu, &iu)) return z;
+    if (!InterpShape<INTERP>::valid(ir, f.nr) || !InterpShape<INTERP>::valid(iu, f.nu)) return z;
+    const float tr = (float)(qr - (double)ir), tu = (float)(qu - (double)iu);
+    if (INTERP == INTERP_KAISER8) {
+        float wr[8], wu[8];
+        kaiser_weights_poly(tr, wr);
+        kaiser_weights_poly(tu, wu);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float2* row = f.img + (size_t)(ir + a - 3) * f.ld + (iu - 3);
+            float sx = 0.f, sy = 0.f;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const float2 v = row[b];
+                sx += wu[b] * v.x;
+                sy += wu[b] * v.y;
+            }
+            z.x += wr[a] * sx;
+            z.y += wr[a] * sy;
+        }
+        return z;
+    }
+    float wr[4], wu[4];
+    cubic_weights(tr, wr);
+    cubic_weights(tu, wu);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const float2* row = f.img + (size_t)(ir + a - 1) * f.ld + (iu - 1);
+        float sx = 0.f, sy = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float2 v = row[b];
+            sx += wu[b] * v.x;
+            sy += wu[b] * v.y;
+        }
+        z.x += wr[a] * sx;
+        z.y += wr[a] * sy;
+    }
+    return z;
+}
+
+// Destination polar pixel p of node geometry g (fast_bp/ffbp.py:62-67).
+__device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long long p, int nu,
+                                                double* x, double* y, double* r) {
+    const int ir = (int)(p / nu);
+    const int iu = (int)(p - (long long)ir * nu);
+    const double rr = g[3] + (double)ir * g[4];
+    const double u = g[5] + (double)iu * g[6];
+    const double g2 = rr * rr - g[2] * g[2];
+    const double ground = g2 > 0.0 ? g2 * rsqrt64(g2) : 0.0;
+    const double c2 = 1.0 - u * u;
+    const double cu = c2 > 0.0 ? c2 * rsqrt64(c2) : 0.0;
+    *x = g[0] + (double)side * ground * cu;
+    *y = g[1] + ground * u;
+    *r = rr;
+}
+
+// Child regrid at destination point (x, y) with destination slant range r.
+template <int INTERP>
+__device__ __forceinline__ float2 regrid_fast(const FastSrc& f, double x, double y, double r,
+                                              double kt) {
+    const double dx = x - f.ox, dy = y - f.oy;
+    const double s2 = dx * dx + dy * dy;
+    if (!(s2 > 0.0)) return make_float2(0.f, 0.f);
+    const double uu = dy * rsqrt64(s2);
+    const double rr2 = s2 + f.oz2;
+    const double rr = rr2 * rsqrt64(rr2);
+    const float2 v = sample2_fast<INTERP>(f, (rr - f.r0) * f.inv_dr, (uu - f.u0) * f.inv_du);
+    const float2 rot = cis_turns64(kt * (rr - r));
+    return make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+}
+
+// Polar image -> point (x, y), remodulated by exp(+j k r) (ffbp.py:189-198).
+template <int INTERP>
+__device__ __forceinline__ float2 eval_fast(const FastSrc& f, double x, double y, double kt) {
+    const double dx = x - f.ox, dy = y - f.oy;
+    const double s2 = dx * dx + dy * dy;
+    if (!(s2 > 0.0)) return make_float2(0.f, 0.f);
+    const double u = dy * rsqrt64(s2);
+    const double r2 = s2 + f.oz2;
+    const double r = r2 * rsqrt64(r2);
+    const float2 v = sample2_fast<INTERP>(f, (r - f.r0) * f.inv_dr, (u - f.u0) * f.inv_du);
+    const float2 rot = cis_turns64(kt * r);
+    return make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void regrid_fast_kernel(const float2* __restrict__ src, Geo sg,
+                                                          int snr, int snu, Geo dg, int nr, int nu,
+                                                          int side, double k, int accumulate,
+                                                          float2* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nr * nu) return;
+    const double sgv[7] = {sg.ox, sg.oy, sg.oz, sg.r0, sg.dr, sg.u0, sg.du};
+    const double dgv[7] = {dg.ox, dg.oy, dg.oz, dg.r0, dg.dr, dg.u0, dg.du};
+    double x, y, r;
+    dest_pixel_fast(dgv, side, p, nu, &x, &y, &r);
+    const FastSrc f = make_fast_src(src, sgv, snr, snu, snu);
+    float2 v = regrid_fast<INTERP>(f, x, y, r, k * INV_TWO_PI);
+    if (accumulate) {
+        const float2 o = out[p];
+        v = make_float2(__fadd_rn(o.x, v.x), __fadd_rn(o.y, v.y));
+    }
+    out[p] = v;
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void merge2_fast_kernel(const float2* __restrict__ left, Geo lg,
+                                                          int lnr, int lnu,
+                                                          const float2* __restrict__ right,
+                                                          Geo rg, int rnr, int rnu, Geo dg,
+                                                          int nr, int nu, int side, double k,
+                                                          float2* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nr * nu) return;
+    const double lgv[7] = {lg.ox, lg.oy, lg.oz, lg.r0, lg.dr, lg.u0, lg.du};
+    const double rgv[7] = {rg.ox, rg.oy, rg.oz, rg.r0, rg.dr, rg.u0, rg.du};
+    const double dgv[7] = {dg.ox, dg.oy, dg.oz, dg.r0, dg.dr, dg.u0, dg.du};
+    double x, y, r;
+    dest_pixel_fast(dgv, side, p, nu, &x, &y, &r);
+    const double kt = k * INV_TWO_PI;
+    const float2 a = regrid_fast<INTERP>(make_fast_src(left, lgv, lnr, lnu, lnu), x, y, r, kt);
+    const float2 b = regrid_fast<INTERP>(make_fast_src(right, rgv, rnr, rnu, rnu), x, y, r, kt);
+    out[p] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void batched_merge_fast_kernel(
+        const int* __restrict__ node_int, const double* __restrict__ node_geo,
+        const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
+        double k) {
+    const int node = blocks[2 * blockIdx.x];
+    const long long p = (long long)blocks[2 * blockIdx.x + 1] + threadIdx.x;
+    const int* ni = node_int + 8 * node;
+    const int nr = ni[2], nu = ni[3], side = ni[4], l = ni[5], r = ni[6];
+    if (p >= (long long)nr * nu) return;
+    double x, y, rr;
+    dest_pixel_fast(node_geo + 7 * node, side, p, nu, &x, &y, &rr);
+    const int* li = node_int + 8 * l;
+    const int* ri = node_int + 8 * r;
+    const double kt = k * INV_TWO_PI;
+    const float2 a = regrid_fast<INTERP>(
+        make_fast_src(arena + offsets[l], node_geo + 7 * l, li[2], li[3], li[3]), x, y, rr, kt);
+    const float2 b = regrid_fast<INTERP>(
+        make_fast_src(arena + offsets[r], node_geo + 7 * r, ri[2], ri[3], ri[3]), x, y, rr, kt);
+    arena[offsets[node] + p] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void eval_points_fast_kernel(const float2* __restrict__ img, Geo g,
+                                                               int nr, int nu, int ld,
+                                                               const double* __restrict__ xyz,
+                                                               long long n, double k,
+                                                               float2* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double gv[7] = {g.ox, g.oy, g.oz, g.r0, g.dr, g.u0, g.du};
+    out[p] = eval_fast<INTERP>(make_fast_src(img, gv, nr, nu, ld), xyz[3 * p], xyz[3 * p + 1],
+                               k * INV_TWO_PI);
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void eval_grid_fast_kernel(const float2* __restrict__ img, Geo g,
+                                                             int nr, int nu, int ld, double x0,
+                                                             double dx, int nx, double y0,
+                                                             double dy, int ny, double k,
+                                                             float2* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nx * ny) return;
+    const int iy = (int)(p / nx), ix = (int)(p - (long long)iy * nx);
+    const double gv[7] = {g.ox, g.oy, g.oz, g.r0, g.dr, g.u0, g.du};
+    out[p] = eval_fast<INTERP>(make_fast_src(img, gv, nr, nu, ld), x0 + ix * dx, y0 + iy * dy,
+                               k * INV_TWO_PI);
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void eval_grid_multi_fast_kernel(
+        const bpmi_eval_job* __restrict__ jobs, const int* __restrict__ blocks) {
+    const bpmi_eval_job& j = jobs[blocks[2 * blockIdx.x]];
+    const long long p = (long long)blocks[2 * blockIdx.x + 1] + threadIdx.x;
+    if (p >= (long long)j.nx * j.ny) return;
+    const int iy = (int)(p / j.nx), ix = (int)(p - (long long)iy * j.nx);
+    ((float2*)j.out)[p] = eval_fast<INTERP>(make_fast_src((const float2*)j.img, j.geo, j.nr, j.nu, j.ld),
+                                            j.x0 + ix * j.dx, j.y0 + iy * j.dy, j.k * INV_TWO_PI);
+}
+
+// Chebyshev interpolation (9 nodes) of each normalised 8-tap Kaiser weight
+// w_a(t), t in [0,1), read from the FP64 table exactly as the FP64 paths do,
+// converted to a power series in s = 2t - 1.
+static void fit_kaiser_poly(const double* table, float out[8][9]) {
+    constexpr int N = 9;
+    double f[8][N];
+    double sn[N];
+    for (int j = 0; j < N; ++j) {
+        sn[j] = cos(M_PI * (j + 0.5) / N);
+        const double t = 0.5 * (sn[j] + 1.0);
+        double w[8], sum = 0.0;
+        for (int a = 0; a < 8; ++a) {
+            double x = fabs(t - (double)(a - 3)) * KAISER_TABLE_RES;
+            int i = (int)x;
+            double fr = x - (double)i;
+            if (i > KAISER_TABLE_LEN - 2) i = KAISER_TABLE_LEN - 2;
+            w[a] = table[i] + fr * (table[i + 1] - table[i]);
+            sum += w[a];
+        }
+        for (int a = 0; a < 8; ++a) f[a][j] = w[a] / sum;
+    }
+    // power-basis coefficients of the Chebyshev polynomials T_0..T_8
+    double T[N][N] = {};
+    T[0][0] = 1.0;
+    T[1][1] = 1.0;
+    for (int k = 2; k < N; ++k)
+        for (int m = 0; m < N; ++m)
+            T[k][m] = (m > 0 ? 2.0 * T[k - 1][m - 1] : 0.0) - T[k - 2][m];
+    for (int a = 0; a < 8; ++a) {
+        double poly[N] = {};
+        for (int k = 0; k < N; ++k) {
+            double c = 0.0;
+            for (int j = 0; j < N; ++j) c += f[a][j] * cos(M_PI * k * (j + 0.5) / N);
+            c *= (k == 0 ? 1.0 : 2.0) / N;
+            for (int m = 0; m < N; ++m) poly[m] += c * T[k][m];
+        }
+        for (int m = 0; m < N; ++m) out[a][m] = (float)poly[m];
+    }
 }
 
 // ---- phase-LUT direct BP (precision_probe.cu:106-164) ----------------------
@@ -453,6 +742,11 @@ int bpmi_set_kaiser_table(const double* host_table, int len) {
         int st = check_hip(hipMalloc((void**)&g_kaiser_table, bytes), "hipMalloc(kaiser table)");
         if (st) return st;
     }
+    float poly[8][9];
+    fit_kaiser_poly(host_table, poly);
+    int st = check_hip(hipMemcpyToSymbol(HIP_SYMBOL(c_kaiser_poly), poly, sizeof(poly)),
+                       "hipMemcpyToSymbol(kaiser polynomials)");
+    if (st) return st;
     return check_hip(hipMemcpy(g_kaiser_table, host, bytes, hipMemcpyHostToDevice),
                      "hipMemcpy(kaiser table)");
 }
@@ -472,6 +766,15 @@ int bpmi_regrid(const void* src, const double* src_geo, int snr, int snu, const 
     const float* kt = g_kaiser_table;
     const float2* s = (const float2*)src;
     float2* o = (float2*)out;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (interp == BPMI_INTERP_KAISER8)
+            hipLaunchKernelGGL(regrid_fast_kernel<INTERP_KAISER8>, grid, dim3(256), 0, hs, s, sg,
+                               snr, snu, dg, nr, nu, side, k, acc, o);
+        else
+            hipLaunchKernelGGL(regrid_fast_kernel<INTERP_LAGRANGE4>, grid, dim3(256), 0, hs, s, sg,
+                               snr, snu, dg, nr, nu, side, k, acc, o);
+        return check_launch("regrid_fast_kernel");
+    }
     auto l = [&]<int M, int I>() {
         hipLaunchKernelGGL((regrid_kernel<M, I>), grid, dim3(256), 0, hs, s, sg, snr, snu, dg, nr,
                            nu, side, k, acc, o, kt);
@@ -496,6 +799,15 @@ int bpmi_merge2(const void* left, const double* left_geo, int lnr, int lnu, cons
     const float* kt = g_kaiser_table;
     const dim3 g(grid_1d((long long)nr * nu, 256));
     hipStream_t hs = (hipStream_t)stream;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (interp == BPMI_INTERP_KAISER8)
+            hipLaunchKernelGGL(merge2_fast_kernel<INTERP_KAISER8>, g, dim3(256), 0, hs, a, ag, lnr,
+                               lnu, b, bg, rnr, rnu, dg, nr, nu, side, k, o);
+        else
+            hipLaunchKernelGGL(merge2_fast_kernel<INTERP_LAGRANGE4>, g, dim3(256), 0, hs, a, ag,
+                               lnr, lnu, b, bg, rnr, rnu, dg, nr, nu, side, k, o);
+        return check_launch("merge2_fast_kernel");
+    }
     auto l = [&]<int M, int I>() {
         hipLaunchKernelGGL((merge2_kernel<M, I>), g, dim3(256), 0, hs, a, ag, lnr, lnu, b, bg,
                            rnr, rnu, dg, nr, nu, side, k, o, kt);
@@ -515,6 +827,15 @@ int bpmi_ffbp_merges(const int* node_int, const double* node_geo, const long lon
     float2* ar = (float2*)arena;
     const float* kt = g_kaiser_table;
     hipStream_t hs = (hipStream_t)stream;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (interp == BPMI_INTERP_KAISER8)
+            hipLaunchKernelGGL(batched_merge_fast_kernel<INTERP_KAISER8>, dim3(nblocks), dim3(256),
+                               0, hs, node_int, node_geo, offsets, ar, blocks, k);
+        else
+            hipLaunchKernelGGL(batched_merge_fast_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
+                               dim3(256), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
+        return check_launch("batched_merge_fast_kernel");
+    }
     auto l = [&]<int M, int I>() {
         hipLaunchKernelGGL((batched_merge_kernel<M, I>), dim3(nblocks), dim3(256), 0, hs, node_int,
                            node_geo, offsets, ar, blocks, k, kt);
@@ -536,6 +857,15 @@ int bpmi_eval_points(const void* img, const double* geo, int nr, int nu, int ld,
     const float* kt = g_kaiser_table;
     const dim3 gr(grid_1d(npoints, 256));
     hipStream_t hs = (hipStream_t)stream;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (interp == BPMI_INTERP_KAISER8)
+            hipLaunchKernelGGL(eval_points_fast_kernel<INTERP_KAISER8>, gr, dim3(256), 0, hs, im, g,
+                               nr, nu, ld, xyz, npoints, k, o);
+        else
+            hipLaunchKernelGGL(eval_points_fast_kernel<INTERP_LAGRANGE4>, gr, dim3(256), 0, hs, im,
+                               g, nr, nu, ld, xyz, npoints, k, o);
+        return check_launch("eval_points_fast_kernel");
+    }
     auto l = [&]<int M, int I>() {
         hipLaunchKernelGGL((eval_points_kernel<M, I>), gr, dim3(256), 0, hs, im, g, nr, nu, ld,
                            xyz, npoints, k, o, kt);
@@ -558,12 +888,46 @@ int bpmi_eval_grid(const void* img, const double* geo, int nr, int nu, int ld, d
     const float* kt = g_kaiser_table;
     const dim3 gr(grid_1d((long long)nx * ny, 256));
     hipStream_t hs = (hipStream_t)stream;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (interp == BPMI_INTERP_KAISER8)
+            hipLaunchKernelGGL(eval_grid_fast_kernel<INTERP_KAISER8>, gr, dim3(256), 0, hs, im, g,
+                               nr, nu, ld, x0, dx, nx, y0, dy, ny, k, o);
+        else
+            hipLaunchKernelGGL(eval_grid_fast_kernel<INTERP_LAGRANGE4>, gr, dim3(256), 0, hs, im, g,
+                               nr, nu, ld, x0, dx, nx, y0, dy, ny, k, o);
+        return check_launch("eval_grid_fast_kernel");
+    }
     auto l = [&]<int M, int I>() {
         hipLaunchKernelGGL((eval_grid_kernel<M, I>), gr, dim3(256), 0, hs, im, g, nr, nu, ld, x0,
                            dx, nx, y0, dy, ny, k, o, kt);
     };
     dispatch2(mode, interp, l);
     return check_launch("eval_grid_kernel");
+}
+
+int bpmi_eval_grid_multi(const bpmi_eval_job* jobs, const int* blocks, int nblocks, int mode,
+                         int interp, void* stream) {
+    int st = check_mode_interp(mode, interp);
+    if (st) return st;
+    if (nblocks <= 0) return BPMI_OK;
+    if (!jobs || !blocks) return set_error(BPMI_ERR_INVALID, "null table");
+    const float* kt = g_kaiser_table;
+    hipStream_t hs = (hipStream_t)stream;
+    if (mode == BPMI_MODE_SPLIT) {
+        if (interp == BPMI_INTERP_KAISER8)
+            hipLaunchKernelGGL(eval_grid_multi_fast_kernel<INTERP_KAISER8>, dim3(nblocks),
+                               dim3(256), 0, hs, jobs, blocks);
+        else
+            hipLaunchKernelGGL(eval_grid_multi_fast_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
+                               dim3(256), 0, hs, jobs, blocks);
+        return check_launch("eval_grid_multi_fast_kernel");
+    }
+    auto l = [&]<int M, int I>() {
+        hipLaunchKernelGGL((eval_grid_multi_kernel<M, I>), dim3(nblocks), dim3(256), 0, hs, jobs,
+                           blocks, kt);
+    };
+    dispatch2(mode, interp, l);
+    return check_launch("eval_grid_multi_kernel");
 }
 
 int bpmi_target_contributions(const void* data, int samples, const double* pos, int pulses,

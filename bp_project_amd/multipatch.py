@@ -1,0 +1,285 @@
+"""Batched FFBP serving of many independent, uneven patches.
+
+All leaves of all patches run in one launch (per-node input table), the
+merges of every patch run one launch per subtree height, and every patch's
+Cartesian output is evaluated in one launch; the whole batch is captured
+in one HIP graph, so a batch is one graph launch with no host round trips.
+The split-mode leaf tile width is chosen by the autotuner.
+
+Counterpart of nothing in the reference (it images one ROI at a time,
+fast_bp/batched_resident_backend.py); it composes the reference-parity
+kernels: results are bitwise equal to running each patch's
+``BatchedResidentProgram`` + evaluation alone with the same tile width.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, Event, Graph, Stream, current_stream, elapsed_ms, stream_handle, synchronize
+from .gpu_backend import build_ffbp_plan, ensure_kaiser_table
+from .plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, flatten_plan
+from .scenes import CartesianGrid, polar_extent_for_grid, stripmap_scene
+
+_vp = _lib.c_void_p
+BLOCK = 256
+
+NODE_SOURCE = np.dtype([("data", "<u8"), ("pos", "<u8"), ("weight", "<u8"), ("phase", "<u8"),
+                        ("range_start", "<f8"), ("data_dr", "<f8"), ("k", "<f8"),
+                        ("samples", "<i4"), ("reserved", "<i4")])
+EVAL_JOB = np.dtype([("img", "<u8"), ("out", "<u8"), ("geo", "<f8", (7,)), ("x0", "<f8"),
+                     ("dx", "<f8"), ("y0", "<f8"), ("dy", "<f8"), ("k", "<f8"), ("nr", "<i4"),
+                     ("nu", "<i4"), ("ld", "<i4"), ("nx", "<i4"), ("ny", "<i4"),
+                     ("reserved", "<i4")])
+assert NODE_SOURCE.itemsize == 64 and EVAL_JOB.itemsize == 136
+
+
+def patch_mix(count=64, seed=20260921, scale=1.0):
+    """Seeded mix of uneven patches: pulses from {256..4096}, bins from
+    {1024, 2048, 4096}, square grids of 128..1024 pixels at 0.5 m, 10..100
+    targets, 20 dB clutter.  ``scale`` < 1 shrinks every size (tests)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        pulses = int(rng.choice([256, 512, 1024, 2048, 4096]))
+        bins = int(rng.choice([1024, 2048, 4096]))
+        grid_n = int(rng.choice([128, 256, 384, 512, 640, 768, 896, 1024]))
+        targets = int(rng.integers(10, 101))
+        if scale != 1.0:
+            pulses = max(32, int(pulses * scale))
+            bins = max(256, int(bins * scale))
+            grid_n = max(16, int(grid_n * scale))
+            targets = max(2, int(targets * scale))
+        # a bin count must cover the grid's slant extent
+        bins = max(bins, int(grid_n * 0.5 * 1.3 / (299792458.0 / (2 * 180e6))) + 256)
+        specs.append(dict(pulses=pulses, bins=bins, n_targets=targets, grid_n=grid_n,
+                          grid_spacing=0.5, seed=seed * 1000 + i))
+    return specs
+
+
+def make_patches(specs):
+    return [stripmap_scene(**s) for s in specs]
+
+
+class PatchPlan:
+    def __init__(self, scene, leaf_size, range_spacing, angular_oversample):
+        self.scene = scene
+        extent = polar_extent_for_grid(scene.grid, scene.pos.mean(axis=0), margin=2.0)
+        self.plan = build_ffbp_plan(scene.pos, scene.fc, extent, side=1, leaf_size=leaf_size,
+                                    range_spacing=range_spacing,
+                                    angular_oversample=angular_oversample)
+        self.flat = flatten_plan(self.plan)
+
+
+class MultiPatchProgram:
+    """Level-batched FFBP + Cartesian evaluation of many patches."""
+
+    def __init__(self, scenes, mode="split", leaf_size=16, range_spacing=0.5,
+                 angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
+                 tile_cols_log2=6):
+        if not scenes:
+            raise ValueError("no patches")
+        ks = {round(s.k, 9) for s in scenes}
+        if len(ks) != 1:
+            raise ValueError("all patches of a batch must share the carrier wavenumber")
+        if "kaiser8" in (leaf_interp, merge_interp):
+            ensure_kaiser_table()
+        self.mode = mode
+        self.leaf_interp = leaf_interp
+        self.merge_interp = merge_interp
+        self.k = float(scenes[0].k)
+        self.patches = [PatchPlan(s, leaf_size, range_spacing, angular_oversample) for s in scenes]
+        self._inputs = []
+        node_int, node_geo, sizes, sources = [], [], [], []
+        self.node_base = []
+        base = 0
+        for pp in self.patches:
+            s = pp.scene
+            d = DeviceArray.from_host(s.data)
+            p = DeviceArray.from_host(s.pos)
+            w = DeviceArray.from_host(np.ascontiguousarray(s.weight, np.float64))
+            ph = DeviceArray.from_host(np.ascontiguousarray(s.phase, np.float64))
+            self._inputs += [d, p, w, ph]
+            ni = pp.flat.node_int.copy()
+            for col in (NI_LEFT, NI_RIGHT):
+                ni[:, col] = np.where(ni[:, col] >= 0, ni[:, col] + base, -1)
+            node_int.append(ni)
+            node_geo.append(pp.flat.node_geo)
+            sizes.append(ni[:, NI_NR].astype(np.int64) * ni[:, NI_NU])
+            src = np.zeros(len(ni), NODE_SOURCE)
+            src["data"], src["pos"], src["weight"], src["phase"] = d.ptr, p.ptr, w.ptr, ph.ptr
+            src["range_start"], src["data_dr"], src["k"] = s.r_start, s.data_dr, s.k
+            src["samples"] = s.bins
+            sources.append(src)
+            self.node_base.append(base)
+            base += len(ni)
+        self.node_int = np.ascontiguousarray(np.concatenate(node_int), np.int32)
+        self.node_geo = np.ascontiguousarray(np.concatenate(node_geo))
+        sizes = np.concatenate(sizes)
+        offsets = np.zeros(len(sizes), np.int64)
+        cursor = 0
+        for i, n in enumerate(sizes):
+            offsets[i] = cursor
+            cursor += -(-int(n) // 64) * 64
+        self.offsets = offsets
+        self.arena = DeviceArray.empty(cursor)
+        self.t_node_int = DeviceArray.from_host(self.node_int)
+        self.t_node_geo = DeviceArray.from_host(self.node_geo)
+        self.t_offsets = DeviceArray.from_host(offsets)
+        self.t_sources = DeviceArray.from_host(np.concatenate(sources).view(np.uint8))
+        self.set_tile_width(tile_cols_log2)
+        heights = self.node_int[:, NI_HEIGHT]
+        self.merge_tables = []
+        for h in range(1, int(heights.max()) + 1):
+            nodes = np.flatnonzero(heights == h)
+            mapping = [(int(n), s) for n in nodes for s in range(0, int(sizes[n]), BLOCK)]
+            self.merge_tables.append(DeviceArray.from_host(np.asarray(mapping, np.int32)))
+        # Cartesian evaluation jobs (crop as a view into each root)
+        jobs = np.zeros(len(self.patches), EVAL_JOB)
+        self.outputs = []
+        eval_blocks = []
+        for j, pp in enumerate(self.patches):
+            plan, g = pp.plan, pp.scene.grid
+            root = self.node_base[j] + plan.root.index
+            out = DeviceArray.empty(g.shape)
+            self.outputs.append(out)
+            img = self.arena.ptr + 8 * (int(offsets[root]) + plan.crop_r[0] * plan.root.nu
+                                        + plan.crop_u[0])
+            jobs[j]["img"], jobs[j]["out"] = img, out.ptr
+            jobs[j]["geo"] = (plan.origin[0], plan.origin[1], plan.origin[2], plan.r0, plan.dr,
+                              plan.u0, plan.du)
+            jobs[j]["x0"], jobs[j]["dx"], jobs[j]["y0"], jobs[j]["dy"] = g.x0, g.dx, g.y0, g.dy
+            jobs[j]["k"] = self.k
+            jobs[j]["nr"], jobs[j]["nu"], jobs[j]["ld"] = plan.nr, plan.nu, plan.root.nu
+            jobs[j]["nx"], jobs[j]["ny"] = g.nx, g.ny
+            eval_blocks += [(j, s) for s in range(0, g.nx * g.ny, BLOCK)]
+        self.t_jobs = DeviceArray.from_host(jobs.view(np.uint8))
+        self.t_eval_blocks = DeviceArray.from_host(np.asarray(eval_blocks, np.int32))
+        self.n_eval_blocks = len(eval_blocks)
+        self._graph = None
+        self._stream = None
+        self.launch_count = 2 + len(self.merge_tables)
+
+    def set_tile_width(self, tile_cols_log2):
+        """(Re)build the leaf block table for a split-mode tile width."""
+        self.tile_cols_log2 = int(tile_cols_log2)
+        leaves = np.flatnonzero(self.node_int[:, NI_HEIGHT] == 0)
+        mapping = []
+        if self.mode == "split":
+            import ctypes
+            t = ctypes.c_int()
+            for n in leaves:
+                _lib.call("bpmi_split_tiles_ex", int(self.node_int[n, NI_NR]),
+                          int(self.node_int[n, NI_NU]), self.tile_cols_log2, ctypes.byref(t))
+                mapping += [(int(n), i) for i in range(t.value)]
+        else:
+            for n in leaves:
+                size = int(self.node_int[n, NI_NR]) * int(self.node_int[n, NI_NU])
+                mapping += [(int(n), s) for s in range(0, size, BLOCK)]
+        self.leaf_blocks = DeviceArray.from_host(np.asarray(mapping, np.int32))
+        self.n_leaf_blocks = len(mapping)
+        if getattr(self, "_graph", None) is not None:
+            self._graph.close()
+            self._graph = None
+
+    @property
+    def work(self):
+        """(output pixels x pulses) summed over patches."""
+        return sum(pp.scene.grid.nx * pp.scene.grid.ny * pp.scene.pulses for pp in self.patches)
+
+    def execute(self):
+        mode = _lib.mode_code(self.mode)
+        s = stream_handle()
+        _lib.call("bpmi_ffbp_leaves_multi", _vp(self.t_sources.ptr), _vp(self.t_node_int.ptr),
+                  _vp(self.t_node_geo.ptr), _vp(self.t_offsets.ptr), _vp(self.arena.ptr),
+                  _vp(self.leaf_blocks.ptr), self.n_leaf_blocks, mode,
+                  _lib.interp_code(self.leaf_interp), self.tile_cols_log2, s)
+        for table in self.merge_tables:
+            _lib.call("bpmi_ffbp_merges", _vp(self.t_node_int.ptr), _vp(self.t_node_geo.ptr),
+                      _vp(self.t_offsets.ptr), _vp(self.arena.ptr), _vp(table.ptr),
+                      table.shape[0], self.k, mode, _lib.interp_code(self.merge_interp), s)
+        _lib.call("bpmi_eval_grid_multi", _vp(self.t_jobs.ptr), _vp(self.t_eval_blocks.ptr),
+                  self.n_eval_blocks, mode, _lib.interp_code(self.merge_interp), s)
+        return self.outputs
+
+    def capture_graph(self):
+        if self._graph is None:
+            self._stream = Stream()
+            current_stream().synchronize()
+            with self._stream:
+                self.execute()
+            self._stream.synchronize()
+            self._graph = Graph.capture(self._stream, self.execute)
+        return True
+
+    def replay(self):
+        if self._graph is None:
+            raise RuntimeError("capture_graph() must succeed before replay()")
+        self._graph.launch()
+        return self.outputs
+
+    def autotune(self, candidates=(4, 5, 6), force=False):
+        """Pick the split-mode leaf tile width (log2 columns) by timing."""
+        from .autotune import make_key, tune
+        if self.mode != "split":
+            return self.tile_cols_log2, {}
+        work = int(sum(pp.plan.counts["leaf_pixel_pulse_products"] for pp in self.patches))
+        key = make_key("ffbp_leaves_multi", self.mode, self.leaf_interp, work)
+
+        def run(tc):
+            self.set_tile_width(tc)
+            self.execute()
+        best, times = tune(key, list(candidates), run, force=force)
+        self.set_tile_width(best)
+        return best, times
+
+    def close(self):
+        if self._graph is not None:
+            self._graph.close()
+        self._graph = None
+        self.outputs = []
+        self._inputs = []
+        self.arena = None
+
+
+def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
+                         angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8"):
+    """Per-patch programs (the same kernels, one patch at a time): returns the
+    outputs, each patch's isolated event time and its completion time when
+    the patches are launched back to back."""
+    from .batched_resident_backend import BatchedResidentProgram
+    from .gpu_backend import HipBaseline, geo_array
+    backend = HipBaseline(interp=leaf_interp, merge_interp=merge_interp)
+    progs = []
+    for s in scenes:
+        pp = PatchPlan(s, leaf_size, range_spacing, angular_oversample)
+        dev = backend.upload(s.data, s.pos, s.weight, s.phase, pp.plan)
+        prog = BatchedResidentProgram(backend, pp.plan, dev, mode, s.r_start, s.data_dr, s.k)
+        out = DeviceArray.empty(s.grid.shape)
+        progs.append((pp, prog, out, geo_array(pp.plan)))
+
+    def run_one(entry):
+        pp, prog, out, geo = entry
+        g = pp.scene.grid
+        prog.execute()
+        _lib.call("bpmi_eval_grid", _vp(prog.output.ptr), geo.ctypes.data_as(_vp), pp.plan.nr,
+                  pp.plan.nu, pp.plan.nu, g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, pp.scene.k,
+                  _lib.mode_code(mode), _lib.interp_code(merge_interp), _vp(out.ptr),
+                  stream_handle())
+    for e in progs:
+        run_one(e)
+    synchronize()
+    start = Event()
+    start.record()
+    marks = []
+    for e in progs:
+        a, b = Event(), Event()
+        a.record()
+        run_one(e)
+        b.record()
+        marks.append((a, b))
+    synchronize()
+    isolated = [elapsed_ms(a, b) for a, b in marks]
+    completion = [elapsed_ms(start, b) for _, b in marks]
+    outputs = [e[2] for e in progs]
+    return outputs, isolated, completion, progs

@@ -133,6 +133,28 @@ int bpmi_target_contributions(const void* data, int samples, const double* pos, 
                               double data_dr, double k, int mode, int interp, void* out,
                               void* stream);
 
+/* ---- adjoint: image gradient -> per-pulse gradients --------------------- */
+/* No reference counterpart (the reference autofocus, fast_bp/autofocus.py:37-79,
+ * is a phase-gradient estimate built on target_contributions).  For the
+ * Cartesian/points BP above (any mode's forward model), given grad = dL/dI
+ * (complex64, PyTorch convention dL = Re sum conj(grad) dI), computes
+ * grad_phase[n] = dL/dphase_n and grad_pos[n, 3] = dL/dpos_n (FP64, device)
+ * for pulses [pulse_begin, pulse_end).  ``partials`` is caller-provided
+ * scratch of bpmi_adjoint_workspace_bytes() bytes.  Deterministic. */
+int bpmi_adjoint_workspace_bytes(long long npixels, int pulses, size_t* bytes);
+int bpmi_bp_grid_adjoint(const void* data, int samples, const double* pos, const double* weight,
+                         const double* phase, int pulse_begin, int pulse_end, double x0,
+                         double dx, int nx, double y0, double dy, int ny, double z0,
+                         double range_start, double data_dr, double k, int interp,
+                         const void* grad_image, void* partials, size_t partials_bytes,
+                         double* grad_phase, double* grad_pos, void* stream);
+int bpmi_bp_points_adjoint(const void* data, int samples, const double* pos, const double* weight,
+                           const double* phase, int pulse_begin, int pulse_end, const double* xyz,
+                           long long npoints, double range_start, double data_dr, double k,
+                           int interp, const void* grad_points, void* partials,
+                           size_t partials_bytes, double* grad_phase, double* grad_pos,
+                           void* stream);
+
 /* ---- FFBP merge / regrid ----------------------------------------------- */
 /* One child polar image -> destination grid, out = v exp(j k (rr - r)) or
  * out += that with BPMI_ACCUMULATE.  Replaces regrid (fast_bp/ffbp.py:79-95)
@@ -186,6 +208,44 @@ int bpmi_ffbp_merges(const int* node_int, const double* node_geo, const long lon
 /* Number of split-mode tiles (1024 pixels: 64 u-columns x 16 r-rows) of a
  * leaf node of nr x nu pixels. */
 int bpmi_split_tiles(int nr, int nu, int* tiles);
+
+/* ---- many independent FFBP patches in one launch per level -------------- */
+/* Per-node input description for leaves of different acquisitions (no
+ * reference counterpart: the reference batches the nodes of one image,
+ * batched_resident_backend.py:72-150).  64 bytes, device array [nodes]. */
+typedef struct bpmi_node_source {
+    const void* data;       /* complex64 [pulses, samples] of the node's patch */
+    const double* pos;      /* f64 [pulses, 3] */
+    const double* weight;   /* f64 [pulses] or NULL */
+    const double* phase;    /* f64 [pulses] or NULL */
+    double range_start;
+    double data_dr;
+    double k;
+    int samples;
+    int reserved;
+} bpmi_node_source;
+
+/* Leaves of many patches (node tables as for bpmi_ffbp_leaves; pulse
+ * indices a, b are relative to the node's own patch).  ``tile_cols_log2``
+ * (3..6) sets the split-mode tile width; the block table then holds
+ * (node, tile) pairs with bpmi_split_tiles_ex() tiles per node. */
+int bpmi_ffbp_leaves_multi(const bpmi_node_source* sources, const int* node_int,
+                           const double* node_geo, const long long* offsets, void* arena,
+                           const int* blocks, int nblocks, int mode, int interp,
+                           int tile_cols_log2, void* stream);
+int bpmi_split_tiles_ex(int nr, int nu, int tile_cols_log2, int* tiles);
+
+/* One Cartesian evaluation job per patch (64-bit fields first). */
+typedef struct bpmi_eval_job {
+    const void* img;        /* complex64 polar image, rows of length ld */
+    void* out;              /* complex64 [ny, nx] */
+    double geo[7];          /* ox, oy, oz, r0, dr, u0, du of the image */
+    double x0, dx, y0, dy, k;
+    int nr, nu, ld, nx, ny, reserved;
+} bpmi_eval_job;
+/* blocks = (job, first_pixel) pairs of 256 output pixels, device arrays. */
+int bpmi_eval_grid_multi(const bpmi_eval_job* jobs, const int* blocks, int nblocks, int mode,
+                         int interp, void* stream);
 
 /* ---- phase-LUT direct BP (resident_precision.py:68-175) ---------------- */
 /* lut[i] = exp(j k (range_start + i data_dr)), complex64, FP64 evaluation

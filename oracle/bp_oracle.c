@@ -355,3 +355,76 @@ void orc_target_contributions(const float* data, long P, long S, const double* p
         }
     }
 }
+
+/* FP64 adjoint of backproject_points (no reference counterpart; checked by
+ * central finite differences of orc_backproject_points).  With g = dL/dI in
+ * the convention dL = Re sum_x conj(g_x) dI_x (complex128 [M]):
+ *   gphi[n]   = Re sum_x conj(g_x) j c_xn,   c_xn = w_n s_n(q) e^{j(k d + phase_n)}
+ *   gpos[n,:] = Re sum_x conj(g_x) w_n e^{j theta} (s'_n(q)/dr + j k s_n(q)) (p_n - x)/d
+ * s' is the derivative of the interpolant with respect to q. */
+static void sample1_deriv(const float* row, long len, double q, int interp, double v[2], double dv[2]) {
+    v[0] = v[1] = dv[0] = dv[1] = 0.0;
+    double fi = floor(q);
+    if (!(fi > -2147483648.0 && fi < 2147483647.0)) return;
+    long i = (long)fi;
+    double t = q - (double)i;
+    if (interp == ORC_LINEAR) {
+        if (i < 0 || i + 1 >= len) return;
+        double ar = row[2 * i], ai = row[2 * i + 1], br = row[2 * i + 2], bi = row[2 * i + 3];
+        v[0] = ar + t * (br - ar);
+        v[1] = ai + t * (bi - ai);
+        dv[0] = br - ar;
+        dv[1] = bi - ai;
+        return;
+    }
+    if (i < 1 || i + 2 >= len) return;
+    double w[4], dw[4];
+    orc_cubic_weights(t, w);
+    dw[0] = -(3 * t * t - 6 * t + 2) / 6;
+    dw[1] = (3 * t * t - 4 * t - 1) / 2;
+    dw[2] = -(3 * t * t - 2 * t - 2) / 2;
+    dw[3] = (3 * t * t - 1) / 6;
+    for (int a = 0; a < 4; ++a) {
+        double sr = row[2 * (i + a - 1)], si = row[2 * (i + a - 1) + 1];
+        v[0] += w[a] * sr;
+        v[1] += w[a] * si;
+        dv[0] += dw[a] * sr;
+        dv[1] += dw[a] * si;
+    }
+}
+
+void orc_backproject_points_adjoint(const float* data, long P, long S, const double* pos,
+                                    const double* xyz, long M, double r_start, double dr, double k,
+                                    const double* weight, const double* phase, int interp,
+                                    const double* grad, double* gphi, double* gpos) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (long n = 0; n < P; ++n) {
+        double are = 0, aim = 0, gx = 0, gy = 0, gz = 0;
+        for (long p = 0; p < M; ++p) {
+            double ex = xyz[3 * p] - pos[3 * n];
+            double ey = xyz[3 * p + 1] - pos[3 * n + 1];
+            double ez = xyz[3 * p + 2] - pos[3 * n + 2];
+            double d = sqrt(ex * ex + ey * ey + ez * ez);
+            double v[2], dv[2];
+            sample1_deriv(data + 2 * (size_t)n * (size_t)S, S, (d - r_start) / dr, interp, v, dv);
+            double th = k * d + phase[n];
+            double c = cos(th), s = sin(th);
+            double gr = grad[2 * p], gi = grad[2 * p + 1];
+            /* h = conj(g) w e^{j th} */
+            double hr = weight[n] * (gr * c + gi * s);
+            double hi = weight[n] * (gr * s - gi * c);
+            are += hr * v[0] - hi * v[1];
+            aim += hr * v[1] + hi * v[0];
+            double ur = dv[0] / dr - k * v[1];
+            double ui = dv[1] / dr + k * v[0];
+            double br = (hr * ur - hi * ui) / d;
+            gx -= br * ex;
+            gy -= br * ey;
+            gz -= br * ez;
+        }
+        gphi[n] = -aim;
+        gpos[3 * n] = gx;
+        gpos[3 * n + 1] = gy;
+        gpos[3 * n + 2] = gz;
+    }
+}

@@ -94,6 +94,8 @@ def _declare(L):
                              _d, _i, _i, _vp]
     L.orc_evaluate_image.argtypes = [_vp, _i, _l, _l, _vp, _d, _d, _d, _d, _vp, _l, _d, _i, _vp]
     L.orc_target_contributions.argtypes = [_vp, _l, _l, _vp, _vp, _l, _d, _d, _d, _i, _vp]
+    L.orc_backproject_points_adjoint.argtypes = [_vp, _l, _l, _vp, _vp, _l, _d, _d, _d, _vp, _vp,
+                                                 _i, _vp, _vp, _vp]
 
 
 def _ptr(a):
@@ -238,6 +240,40 @@ def target_contributions(data, pos, xyz, r_start, dr, k, interp="lagrange4"):
                                    float(r_start), float(dr), float(k), _interp(interp),
                                    _ptr(out))
     return out
+
+
+def backproject_points_adjoint(data, pos, xyz, r_start, dr, k, weight, phase, grad,
+                               interp="lagrange4"):
+    """FP64 adjoint of backproject_points: (dL/dphase [P], dL/dpos [P, 3]) for
+    dL = Re sum conj(grad) dI.  Pinned by central finite differences."""
+    data = _c64(data)
+    P, S = data.shape
+    pos = _f64(pos, (P, 3))
+    xyz = _f64(xyz).reshape(-1, 3)
+    weight = _f64(weight, (P,))
+    phase = _f64(phase, (P,))
+    grad = np.ascontiguousarray(grad, dtype=np.complex128).reshape(len(xyz))
+    gphi = np.zeros(P)
+    gpos = np.zeros((P, 3))
+    lib().orc_backproject_points_adjoint(_ptr(data), P, S, _ptr(pos), _ptr(xyz), len(xyz),
+                                         float(r_start), float(dr), float(k), _ptr(weight),
+                                         _ptr(phase), _interp(interp), _ptr(grad), _ptr(gphi),
+                                         _ptr(gpos))
+    return gphi, gpos
+
+
+def entropy_and_grad(image):
+    """Image entropy E = -sum p ln p, p = |I|^2 / sum |I|^2, and its gradient
+    g = dE/dI in the convention dE = Re sum conj(g) dI, i.e.
+    g = -2 (ln p + E) I / S (closed form)."""
+    I = np.asarray(image, np.complex128)
+    a2 = np.abs(I) ** 2
+    S = a2.sum()
+    p = a2 / S
+    lp = np.log(np.where(p > 0, p, 1.0))
+    E = float(-(p * lp).sum())
+    g = -2.0 * (lp + E) * I / S
+    return E, g
 
 
 def polar_points(origin, r, u, side=1):
