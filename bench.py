@@ -58,7 +58,8 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="c2_ffbp_full",
-                   choices=["c2_ffbp_full", "c1_direct_small", "c3_direct_precision"])
+                   choices=["c2_ffbp_full", "c1_direct_small", "c3_direct_precision",
+                            "c4_autofocus", "c5_multipatch"])
     p.add_argument("--mode", default="split")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-quality", action="store_true", help="skip the FP64 quality gate")
@@ -247,7 +248,7 @@ def cpu_baseline_ffbp(args, scene):
     import oracle
     from bp_project_amd.scenes import CartesianGrid, polar_extent_for_grid
     g = scene.grid
-    n = 512
+    n = 2048
     cx = g.x0 + g.dx * g.nx / 2
     cy = g.y0 + g.dy * g.ny / 2
     sub = CartesianGrid(x0=cx - n * g.dx / 2, dx=g.dx, nx=n, y0=cy - n * g.dy / 2, dy=g.dy, ny=n)
@@ -348,6 +349,107 @@ def run_direct(args, scene, name):
     return result
 
 
+def run_autofocus(args, scene):
+    """Config 4: 30 Adam iterations of entropy autofocus through the HIP adjoint,
+    against the per-step cost of FP64 BP + central-difference gradients."""
+    import torch
+    from bp_project_amd.autofocus import GridOperator, entropy, gradient_autofocus
+    from bp_project_amd.device import Event, elapsed_ms, synchronize
+    op = GridOperator.from_scene(scene, mode=args.mode, interp="lagrange4")
+    res = gradient_autofocus(op, scene.phase, scene.pos, iterations=30, lr_phase=0.05,
+                             lr_pos=5e-4)
+    steps_ms = res["ms_per_iteration"][2:] or res["ms_per_iteration"]
+    ms_step = float(np.median(steps_ms))
+    # FP64 BP with complex128 output: the image a finite-difference autofocus needs
+    phase = torch.from_numpy(scene.phase).cuda()
+    pos = torch.from_numpy(scene.pos).cuda()
+    op.image64(phase, pos)
+    synchronize()
+    a, b = Event(), Event()
+    a.record()
+    for _ in range(3):
+        op.image64(phase, pos)
+    b.record()
+    synchronize()
+    fp64_ms = elapsed_ms(a, b) / 3
+    # central differences: 2 images per parameter, 4 parameters (phase, xyz) per pulse
+    fd_step_ms = 2 * 4 * scene.pulses * fp64_ms
+    with torch.no_grad():
+        true_img = op.image(torch.from_numpy(scene.phase - scene.meta["phase_error"]).cuda(),
+                            torch.from_numpy(scene.true_pos).cuda())
+        true_entropy = float(entropy(true_img))
+    g = scene.grid
+    pp = g.nx * g.ny * scene.pulses
+    return {
+        "ms_per_step": ms_step,
+        "value": pp / (ms_step * 1e-3) / 1e9,
+        "autofocus": {
+            "ms_per_gradient_step": round(ms_step, 3),
+            "iterations": len(res["ms_per_iteration"]),
+            "entropy_initial": res["entropy"][0], "entropy_final": res["best_entropy"],
+            "entropy_with_true_phase": true_entropy,
+            "fp64_bp_image_ms": round(fp64_ms, 3),
+            "fd_fp64_gradient_step_ms_extrapolated": round(fd_step_ms, 1),
+            "speedup_vs_fd_fp64": round(fd_step_ms / ms_step, 1),
+            "note": "FD baseline = 2 x 4 x pulses FP64 GPU BP images per gradient step "
+                    "(measured image time x count); adjoint gradients equal the FD ones "
+                    "(tests/test_gpu_autofocus.py), so both reach the same entropy"},
+        "roofline": None,
+        "config": {"workload": "c4_autofocus", "pulses": scene.pulses, "bins": scene.bins,
+                   "output": [g.ny, g.nx], "pixel_m": g.dx, "mode": args.mode,
+                   "optimizer": "Adam(phase lr 0.05, pos lr 5e-4), 30 iterations"},
+    }
+
+
+def run_multipatch(args):
+    """Config 5: 64 uneven patches, one HIP graph per batch vs per-patch launches."""
+    from bp_project_amd.device import Event, elapsed_ms, synchronize
+    from bp_project_amd.multipatch import MultiPatchProgram, make_patches, patch_mix, sequential_reference
+    t = time.perf_counter()
+    scenes = make_patches(patch_mix(count=64))
+    log(f"64 patches generated in {time.perf_counter() - t:.1f} s")
+    prog = MultiPatchProgram(scenes, mode=args.mode, merge_interp="kaiser8")
+    best, times = prog.autotune()
+    log(f"autotuned leaf tile width 2^{best}: {times}")
+    prog.capture_graph()
+    for _ in range(args.warmup):
+        prog.replay()
+    synchronize()
+    a, b = Event(), Event()
+    a.record()
+    for _ in range(args.steps):
+        prog.replay()
+    b.record()
+    synchronize()
+    ms_batch = elapsed_ms(a, b) / args.steps
+    batch_out = [o.get() for o in prog.outputs]
+    seq_out, isolated, completion, progs = sequential_reference(scenes, mode=args.mode,
+                                                                 merge_interp="kaiser8")
+    same = all(np.array_equal(x, y.get()) for x, y in zip(batch_out, seq_out))
+    work = prog.work
+    seq_total = completion[-1]
+    return {
+        "ms_per_step": ms_batch,
+        "value": work / (ms_batch * 1e-3) / 1e9,
+        "multipatch": {
+            "patches": len(scenes), "pixel_pulse_products": work,
+            "batched_graph_ms": round(ms_batch, 3),
+            "batched_latency_p50_ms": round(ms_batch, 3), "batched_latency_p99_ms": round(ms_batch, 3),
+            "sequential_total_ms": round(seq_total, 3),
+            "sequential_latency_p50_ms": round(float(np.percentile(completion, 50)), 3),
+            "sequential_latency_p99_ms": round(float(np.percentile(completion, 99)), 3),
+            "sequential_isolated_p50_ms": round(float(np.percentile(isolated, 50)), 3),
+            "sequential_isolated_p99_ms": round(float(np.percentile(isolated, 99)), 3),
+            "sequential_gpix_pulse_per_s": round(work / (seq_total * 1e-3) / 1e9, 2),
+            "batched_equals_sequential_bitwise": bool(same),
+            "autotuned_leaf_tile_cols_log2": best, "autotune_ms": times,
+            "launches_per_batch": prog.launch_count},
+        "roofline": None,
+        "config": {"workload": "c5_multipatch", "patches": len(scenes), "mode": args.mode,
+                   "merge_interp": "kaiser8", "graph": True},
+    }
+
+
 def cpu_baseline_direct(scene, name):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -378,13 +480,20 @@ def main():
         sys.exit(1)
     from bp_project_amd.scenes import workload_scene
     t = time.perf_counter()
-    scene = workload_scene(args.workload)
-    log(f"scene {args.workload}: {scene.data.shape}, {len(scene.targets)} targets, "
-        f"{time.perf_counter() - t:.1f} s")
+    scene = None if args.workload == "c5_multipatch" else workload_scene(args.workload)
+    if scene is not None:
+        log(f"scene {args.workload}: {scene.data.shape}, {len(scene.targets)} targets, "
+            f"{time.perf_counter() - t:.1f} s")
     log("device:", json.dumps(device_info(0)))
     if args.workload == "c2_ffbp_full":
         res = run_ffbp(args, scene)
         cpu = None if args.no_cpu else cpu_baseline_ffbp(args, scene)
+    elif args.workload == "c4_autofocus":
+        res = run_autofocus(args, scene)
+        cpu = None
+    elif args.workload == "c5_multipatch":
+        res = run_multipatch(args)
+        cpu = None
     else:
         res = run_direct(args, scene, args.workload)
         cpu = None if args.no_cpu else cpu_baseline_direct(scene, args.workload)
@@ -399,6 +508,9 @@ def main():
     }
     if "quality" in res:
         line["quality"] = res["quality"]
+    for extra in ("autofocus", "multipatch"):
+        if extra in res:
+            line[extra] = res[extra]
     if "precision_study" in res:
         line["precision_study"] = res["precision_study"]
     if "stages" in res:

@@ -19,6 +19,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray
+from .gpu_backend import leaf_tile_width
 from .plan import NI_NR, NI_NU, flatten_plan
 from .resident_backend import ResidentProgram
 
@@ -27,9 +28,9 @@ _vp = _lib.c_void_p
 MERGE_BLOCK = 256
 
 
-def split_tiles(nr, nu):
+def split_tiles(nr, nu, tile_cols_log2=6):
     t = ctypes.c_int()
-    _lib.call("bpmi_split_tiles", int(nr), int(nu), ctypes.byref(t))
+    _lib.call("bpmi_split_tiles_ex", int(nr), int(nu), int(tile_cols_log2), ctypes.byref(t))
     return t.value
 
 
@@ -71,20 +72,37 @@ class BatchedResidentProgram(ResidentProgram):
         for height, nodes in enumerate(flat.levels):
             if height == 0:
                 if self.mode == "split":
+                    tc = self.leaf_tile_cols_log2 = leaf_tile_width(
+                        self.plan, d._host_position, self.data_dr)
                     mapping = [(n, t) for n in nodes
                                for t in range(split_tiles(flat.node_int[n, NI_NR],
-                                                          flat.node_int[n, NI_NU]))]
+                                                          flat.node_int[n, NI_NU], tc))]
+                    from .multipatch import NODE_SOURCE
+                    src = np.zeros(flat.node_count, NODE_SOURCE)
+                    src["data"], src["pos"] = d.data.ptr, d.pos64.ptr
+                    src["weight"], src["phase"] = d.weight64.ptr, d.phase64.ptr
+                    src["range_start"], src["data_dr"], src["k"] = (self.range_start,
+                                                                    self.data_dr, self.k)
+                    src["samples"] = d.data.shape[1]
+                    sources = self._table(src.view(np.uint8))
+                    blocks = self._table(np.asarray(mapping, dtype=np.int32))
+                    self._enqueue("bpmi_ffbp_leaves_multi", len(mapping) * MERGE_BLOCK, (
+                        _vp(sources.ptr), _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
+                        _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr),
+                        len(mapping), mode, self._interp(), tc),
+                        name=f"batched_bp_polar_{self.mode}")
                 else:
                     mapping = [(n, s) for n in nodes
                                for s in range(0, int(flat.node_int[n, NI_NR]) *
                                               int(flat.node_int[n, NI_NU]), MERGE_BLOCK)]
-                blocks = self._table(np.asarray(mapping, dtype=np.int32))
-                self._enqueue("bpmi_ffbp_leaves", len(mapping) * MERGE_BLOCK, (
-                    _vp(d.data.ptr), d.data.shape[1], _vp(d.pos64.ptr), _vp(d.weight64.ptr),
-                    _vp(d.phase64.ptr), _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
-                    _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(mapping),
-                    self.range_start, self.data_dr, self.k, mode, self._interp()),
-                    name=f"batched_bp_polar_{self.mode}")
+                    blocks = self._table(np.asarray(mapping, dtype=np.int32))
+                    self._enqueue("bpmi_ffbp_leaves", len(mapping) * MERGE_BLOCK, (
+                        _vp(d.data.ptr), d.data.shape[1], _vp(d.pos64.ptr), _vp(d.weight64.ptr),
+                        _vp(d.phase64.ptr), _vp(self._node_table.ptr),
+                        _vp(self._geometry_table.ptr), _vp(self._offset_table.ptr),
+                        _vp(arena.ptr), _vp(blocks.ptr), len(mapping), self.range_start,
+                        self.data_dr, self.k, mode, self._interp()),
+                        name=f"batched_bp_polar_{self.mode}")
                 kernel = f"batched_bp_polar_{self.mode}"
             else:
                 mapping = [(n, s) for n in nodes

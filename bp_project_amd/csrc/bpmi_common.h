@@ -65,12 +65,21 @@ __device__ __forceinline__ double rsqrt64(double x) {
 }
 
 // Reference association order of cubic_weights (fast_bp/ffbp.py:17-21).
-template <typename T>
-__device__ __forceinline__ void cubic_weights(T t, T w[4]) {
-    w[0] = -t * (t - T(1)) * (t - T(2)) / T(6);
-    w[1] = (t + T(1)) * (t - T(1)) * (t - T(2)) / T(2);
-    w[2] = -(t + T(1)) * t * (t - T(2)) / T(2);
-    w[3] = (t + T(1)) * t * (t - T(1)) / T(6);
+// FP64: the reference's divisions; FP32: multiplication by the rounded
+// reciprocal (a 1-ulp weight difference, no IEEE division sequence).
+__device__ __forceinline__ void cubic_weights(double t, double w[4]) {
+    w[0] = -t * (t - 1.0) * (t - 2.0) / 6.0;
+    w[1] = (t + 1.0) * (t - 1.0) * (t - 2.0) / 2.0;
+    w[2] = -(t + 1.0) * t * (t - 2.0) / 2.0;
+    w[3] = (t + 1.0) * t * (t - 1.0) / 6.0;
+}
+
+__device__ __forceinline__ void cubic_weights(float t, float w[4]) {
+    constexpr float sixth = 1.0f / 6.0f;
+    w[0] = -t * (t - 1.0f) * (t - 2.0f) * sixth;
+    w[1] = (t + 1.0f) * (t - 1.0f) * (t - 2.0f) * 0.5f;
+    w[2] = -(t + 1.0f) * t * (t - 2.0f) * 0.5f;
+    w[3] = (t + 1.0f) * t * (t - 1.0f) * sixth;
 }
 
 // Kaiser-sinc weights from the device table (linear interpolation of the

@@ -80,6 +80,33 @@ def origin_reference(position, a, b, r_min):
     return int(spread <= 0.02 * max(float(r_min), 1e-9))
 
 
+SPLIT_WINDOW_SLOTS = 32   # LDS range window of the split kernel (samples per pulse)
+
+
+def leaf_tile_width(plan, position, data_dr):
+    """log2 tile width (16..64 u-columns) of the split-mode leaf kernel for a
+    plan: the least padded u-columns over all leaves whose tile rows, widened
+    by the leaf sub-aperture, still fit the LDS range window (a pulse that does
+    not fit takes the exact-geometry path, so this is a speed choice only).
+    Eager, fused and batched FFBP all use it, so they stay bitwise equal."""
+    leaves = [n for n in iter_nodes(plan.root) if n.is_leaf]
+    position = np.asarray(position, dtype=np.float64)
+    reach = 0.0
+    for n in leaves:
+        p = position[n.a:n.b]
+        reach = max(reach, float(np.sqrt(((p - p.mean(axis=0)) ** 2).sum(axis=1)).max()))
+    best, best_cols = 6, None
+    for tc in (4, 5, 6):
+        rows = 1024 >> tc
+        slots = (rows * leaves[0].dr + 2 * reach) / float(data_dr) + 4
+        if slots > SPLIT_WINDOW_SLOTS:
+            continue
+        cols = sum(-(-n.nu // (1 << tc)) * (1 << tc) for n in leaves)
+        if best_cols is None or cols < best_cols:
+            best, best_cols = tc, cols
+    return best
+
+
 class HipBaseline:
     """Launch the libbpmi kernels (counterpart of fast_bp/gpu_backend.py:227-561)."""
 
@@ -156,7 +183,8 @@ class HipBaseline:
         return result, 1000.0 * (time.perf_counter() - start)
 
     # -- building blocks ---------------------------------------------------------
-    def leaf(self, node, device, mode, range_start, data_dr, k, out=None, stream=None):
+    def leaf(self, node, device, mode, range_start, data_dr, k, out=None, stream=None,
+             tile_cols_log2=-1):
         """Leaf polar BP of one node into ``out`` (new array if None)."""
         out = DeviceArray.empty((node.nr, node.nu)) if out is None else out
         geo = geo_array(node)
@@ -167,7 +195,7 @@ class HipBaseline:
                       _vp(d.weight64.ptr), _vp(d.phase64.ptr), node.a, node.b, _gp(geo), node.nr,
                       node.nu, node.side, range_start, data_dr, k, _lib.MODE_SPLIT,
                       _lib.interp_code(self.interp), 0, _vp(out.ptr), stream_handle(stream),
-                      oref, -1)
+                      oref, tile_cols_log2)
         else:
             _lib.call("bpmi_bp_polar", _vp(d.data.ptr), d.data.shape[1], _vp(d.pos64.ptr),
                       _vp(d.weight64.ptr), _vp(d.phase64.ptr), node.a, node.b, _gp(geo), node.nr,
@@ -207,13 +235,13 @@ class HipBaseline:
         return out
 
     # -- eager schedules -----------------------------------------------------------
-    def _execute_node(self, node, device, mode, range_start, data_dr, k):
+    def _execute_node(self, node, device, mode, range_start, data_dr, k, tc=-1):
         if node.is_leaf:
-            return self.leaf(node, device, mode, range_start, data_dr, k)
-        left_child = self._execute_node(node.left, device, mode, range_start, data_dr, k)
+            return self.leaf(node, device, mode, range_start, data_dr, k, tile_cols_log2=tc)
+        left_child = self._execute_node(node.left, device, mode, range_start, data_dr, k, tc)
         left = self.regrid(left_child, node.left, node, mode, k)
         del left_child
-        right_child = self._execute_node(node.right, device, mode, range_start, data_dr, k)
+        right_child = self._execute_node(node.right, device, mode, range_start, data_dr, k, tc)
         right = self.regrid(right_child, node.right, node, mode, k)
         del right_child
         self.add_inplace(left, right)
@@ -227,7 +255,8 @@ class HipBaseline:
         wall_start = time.perf_counter()
         start, end = Event(), Event()
         start.record()
-        root = self._execute_node(plan.root, device, mode, range_start, data_dr, k)
+        tc = leaf_tile_width(plan, device._host_position, data_dr) if mode == "split" else -1
+        root = self._execute_node(plan.root, device, mode, range_start, data_dr, k, tc)
         output = self.crop(root, plan)
         del root
         end.record()

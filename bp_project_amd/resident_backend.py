@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Graph, Stream, current_stream, stream_handle
-from .gpu_backend import ALL_MODES, MODES, geo_array, origin_reference  # noqa: F401
+from .gpu_backend import ALL_MODES, MODES, geo_array, leaf_tile_width, origin_reference  # noqa: F401
 
 _vp = _lib.c_void_p
 
@@ -180,10 +180,14 @@ class ResidentProgram:
                 _vp(d.phase64.ptr), node.a, node.b, geo, node.nr, node.nu, node.side,
                 self.range_start, self.data_dr, self.k, _lib.mode_code(self.mode),
                 self._interp(), 0, _vp(output.ptr))
-        oref = origin_reference(d._host_position, node.a, node.b, node.r0) \
-            if self.mode == "split" else 0
+        oref, tc = 0, -1
+        if self.mode == "split":
+            oref = origin_reference(d._host_position, node.a, node.b, node.r0)
+            if getattr(self, "_leaf_tc", None) is None:
+                self._leaf_tc = leaf_tile_width(self.plan, d._host_position, self.data_dr)
+            tc = self._leaf_tc
         self._enqueue("bpmi_bp_polar_ex", node.nr * node.nu, args, name=f"bp_polar_{self.mode}",
-                      launcher=_ExLauncher("bpmi_bp_polar_ex", oref, -1))
+                      launcher=_ExLauncher("bpmi_bp_polar_ex", oref, tc))
         return output
 
     def _regrid(self, image, source, destination, accumulate_into=None):
