@@ -424,7 +424,8 @@ def run_multipatch(args):
     ms_batch = elapsed_ms(a, b) / args.steps
     batch_out = [o.get() for o in prog.outputs]
     seq_out, isolated, completion, progs = sequential_reference(scenes, mode=args.mode,
-                                                                 merge_interp="kaiser8")
+                                                                 merge_interp="kaiser8",
+                                                                 tile_cols_log2=best)
     same = all(np.array_equal(x, y.get()) for x, y in zip(batch_out, seq_out))
     work = prog.work
     seq_total = completion[-1]

@@ -8,6 +8,7 @@ BPMI_ERR_INVALID -> ValueError, anything else -> RuntimeError.
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import c_double, c_float, c_int, c_longlong, c_size_t, c_void_p, c_char_p, POINTER
 from pathlib import Path
 
@@ -76,6 +77,8 @@ SIGNATURES = {
     "bpmi_ffbp_merges": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _i, _vp],
     "bpmi_split_tiles": [_i, _i, POINTER(c_int)],
     "bpmi_split_tiles_ex": [_i, _i, _i, POINTER(c_int)],
+    "bpmi_ffbp_merges_tiled": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _vp],
+    "bpmi_merge_tiles": [_i, _i, POINTER(c_int)],
     "bpmi_ffbp_leaves_multi": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp],
     "bpmi_eval_grid_multi": [_vp, _vp, _i, _i, _i, _vp],
     "bpmi_fill_range_phase_lut": [_i, _d, _d, _d, _vp, _vp],
@@ -97,6 +100,23 @@ class BpmiError(RuntimeError):
     pass
 
 
+def _share_hip_runtime_with_torch():
+    """Import torch (when installed) before libbpmi.so is loaded.
+
+    The torch wheel carries its own HIP runtime; loaded first, it is the one
+    runtime of the process and libbpmi.so binds to it, so torch tensors,
+    streams and libbpmi calls share one device context.  Loaded the other
+    way round, torch finds no usable device.  ``BPMI_NO_TORCH=1`` skips the
+    import for programs that never use torch.
+    """
+    if os.environ.get("BPMI_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path=None):
     """Load libbpmi.so (no GPU is touched by loading)."""
     global _lib
@@ -107,6 +127,7 @@ def load(path=None):
         raise RuntimeError(
             f"libbpmi.so not found at {p}: build it with `python -m bp_project_amd.build` "
             "(there is no CPU fallback)")
+    _share_hip_runtime_with_torch()
     lib = ctypes.CDLL(str(p))
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -7,8 +7,8 @@ Node images live in one arena addressed by per-node offsets, and the node
 geometry/offset/block tables are uploaded once at construction.
 
 In ``split`` mode the leaf launch uses the LDS-tiled kernel with each leaf's
-origin as the geometry reference (1024-pixel tiles, block table of
-(node, tile) pairs); the other modes use one thread per pixel with blocks of
+origin as the geometry reference (1024-pixel tiles of 2^tc columns, tc from
+``leaf_tile_width`` unless given; block table of (node, tile) pairs); the other modes use one thread per pixel with blocks of
 256 pixels, like the reference's 128-pixel block table.
 """
 from __future__ import annotations
@@ -34,11 +34,43 @@ def split_tiles(nr, nu, tile_cols_log2=6):
     return t.value
 
 
+def merge_tiles(nr, nu):
+    """Number of 32 x 32 parent tiles of the split-mode tiled merge."""
+    t = ctypes.c_int()
+    _lib.call("bpmi_merge_tiles", int(nr), int(nu), ctypes.byref(t))
+    return t.value
+
+
+def merge_block_table(node_int, nodes, mode):
+    """(node, tile) pairs for split mode (tiled merges), else (node, first
+    pixel) pairs of MERGE_BLOCK pixels."""
+    if mode == "split":
+        return [(int(n), t) for n in nodes
+                for t in range(merge_tiles(node_int[n, NI_NR], node_int[n, NI_NU]))]
+    return [(int(n), s) for n in nodes
+            for s in range(0, int(node_int[n, NI_NR]) * int(node_int[n, NI_NU]), MERGE_BLOCK)]
+
+
+def enqueue_merges(node_int_ptr, node_geo_ptr, offsets_ptr, arena_ptr, blocks_ptr, nblocks, k,
+                   mode, merge_interp, stream):
+    """One merge launch over a block table built by ``merge_block_table``."""
+    if mode == "split":
+        _lib.call("bpmi_ffbp_merges_tiled", _vp(node_int_ptr), _vp(node_geo_ptr),
+                  _vp(offsets_ptr), _vp(arena_ptr), _vp(blocks_ptr), int(nblocks), float(k),
+                  _lib.interp_code(merge_interp), stream)
+    else:
+        _lib.call("bpmi_ffbp_merges", _vp(node_int_ptr), _vp(node_geo_ptr), _vp(offsets_ptr),
+                  _vp(arena_ptr), _vp(blocks_ptr), int(nblocks), float(k),
+                  _lib.mode_code(mode), _lib.interp_code(merge_interp), stream)
+
+
 class BatchedResidentProgram(ResidentProgram):
     """ResidentProgram-compatible FFBP batched across each tree level."""
 
-    def __init__(self, backend, plan, device, mode, range_start, data_dr, k, chunk_pulses=64):
+    def __init__(self, backend, plan, device, mode, range_start, data_dr, k, chunk_pulses=64,
+                 leaf_tile_cols_log2=None):
         self._table_buffers = []
+        self._leaf_tc_request = leaf_tile_cols_log2
         self._batch_stages = []
         super().__init__(backend, plan, device, mode, range_start, data_dr, k,
                          algorithm="ffbp", variant="fused", chunk_pulses=chunk_pulses)
@@ -72,8 +104,10 @@ class BatchedResidentProgram(ResidentProgram):
         for height, nodes in enumerate(flat.levels):
             if height == 0:
                 if self.mode == "split":
-                    tc = self.leaf_tile_cols_log2 = leaf_tile_width(
-                        self.plan, d._host_position, self.data_dr)
+                    tc = self._leaf_tc_request
+                    if tc is None:
+                        tc = leaf_tile_width(self.plan, d._host_position, self.data_dr)
+                    tc = self.leaf_tile_cols_log2 = int(tc)
                     mapping = [(n, t) for n in nodes
                                for t in range(split_tiles(flat.node_int[n, NI_NR],
                                                           flat.node_int[n, NI_NU], tc))]
@@ -105,15 +139,20 @@ class BatchedResidentProgram(ResidentProgram):
                         name=f"batched_bp_polar_{self.mode}")
                 kernel = f"batched_bp_polar_{self.mode}"
             else:
-                mapping = [(n, s) for n in nodes
-                           for s in range(0, int(flat.node_int[n, NI_NR]) *
-                                          int(flat.node_int[n, NI_NU]), MERGE_BLOCK)]
+                mapping = merge_block_table(flat.node_int, nodes, self.mode)
                 blocks = self._table(np.asarray(mapping, dtype=np.int32))
-                self._enqueue("bpmi_ffbp_merges", len(mapping) * MERGE_BLOCK, (
-                    _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
-                    _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(mapping),
-                    self.k, mode, self._merge_interp()),
-                    name=f"batched_merge_regrid_{self.mode}")
+                if self.mode == "split":
+                    self._enqueue("bpmi_ffbp_merges_tiled", len(mapping) * MERGE_BLOCK, (
+                        _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
+                        _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr),
+                        len(mapping), self.k, self._merge_interp()),
+                        name=f"batched_merge_regrid_{self.mode}")
+                else:
+                    self._enqueue("bpmi_ffbp_merges", len(mapping) * MERGE_BLOCK, (
+                        _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
+                        _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr),
+                        len(mapping), self.k, mode, self._merge_interp()),
+                        name=f"batched_merge_regrid_{self.mode}")
                 kernel = f"batched_merge_regrid_{self.mode}"
             pixels = int(sum(int(flat.node_int[n, NI_NR]) * int(flat.node_int[n, NI_NU])
                              for n in nodes))

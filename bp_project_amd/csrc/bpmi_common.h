@@ -38,6 +38,10 @@ __device__ __forceinline__ const double* kaiser_table64(const float* t32) {
 
 __device__ __forceinline__ double fract64(double x) { return x - floor(x); }
 __device__ __forceinline__ float fract32(float x) { return x - floorf(x); }
+// Hardware fraction (v_fract_f64 / v_fract_f32): x - floor(x) in one
+// instruction for finite x.
+__device__ __forceinline__ double hw_fract64(double x) { return __builtin_amdgcn_fract(x); }
+__device__ __forceinline__ float hw_fract32(float x) { return __builtin_amdgcn_fractf(x); }
 
 // sin/cos of 2*pi*t with the hardware transcendental unit (input in turns).
 __device__ __forceinline__ void sincos_turns(float t, float* s, float* c) {

@@ -485,7 +485,7 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                 const double rm = ref.rmin - D;
                 near = rm <= D || ref.rmin <= 0.0;
                 const double mm = D / fmax(ref.rmin, 1e-30);
-                err = near ? 1e30 : D * D * D * D / (8.0 * rm * rm * rm) + mm * mm * mm * D * D / (2.0 * ref.rmin);
+                err = near ? 1e30 : D * D * D * D / (8.0 * rm * rm * rm) + mm * mm * D * D / (2.0 * ref.rmin * (1.0 - mm));
             } else {
                 const double inv = 1.0 / D;
                 q.vx = ux * inv; q.vy = uy * inv; q.vz = uz * inv;
@@ -499,8 +499,15 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                 const double extra = near ? 0.0 : rho * rho / (2.0 * (D - rho));
                 qlo = (D - B - 1e-6 - a.range_start) / a.data_dr;
                 qhi = (D + B + extra + 1e-6 - a.range_start) / a.data_dr;
-                const double dm = D - rho, mm = rho * inv;
-                err = near ? 1e30 : rho * rho * rho * rho / (8.0 * dm * dm * dm) + mm * mm * mm * rho * rho * 0.5 * inv;
+                // Truncation bound of the fourth-order range expansion used below:
+                // sqrt tail h^3 / (16 L^5), 1/(1+m) tail after m^2, and the
+                // (1+m)^-3 factor dropped from the h^2 term.
+                const double dm = D - rho, mm = rho * inv, r2 = rho * rho;
+                const double om = 1.0 - mm;
+                err = near ? 1e30
+                           : r2 * r2 * r2 / (16.0 * dm * dm * dm * dm * dm) +
+                                 r2 * 0.5 * inv * mm * mm * mm / om +
+                                 r2 * r2 * 0.125 * inv * inv * inv * 3.0 * mm / (om * om * om * om);
             }
             int flag = PF_NORMAL;
             int ilo = 0;
@@ -591,25 +598,36 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
             const float4* cb = coef + b * SP_W * F4;
 #pragma unroll
             for (int k = 0; k < SP_PPT; ++k) {
+                // linear range term in FP64 and its phase reduced modulo one turn
                 const double lin = dxk[k] * q.vx + dyk[k] * q.vy + dzk[k] * q.vz;
-                const double T = fract64(q.P + lin * kt);
+                const double T = hw_fract64(q.P + lin * kt);
                 const float l32 = (float)lin;
-                float h, m, corr, ql;
+                // Range residual d - (R + lin) = sqrt((R + lin)^2 + h) - (R + lin)
+                // with h = Q - lin^2, m = lin / R and hs = h / (2R).  Leaves
+                // (antenna within a few metres of the origin) need hs (1 - m);
+                // tiles add the fourth-order terms hs (m^2 - h / (4R^2)).
+                float ql, corr;
                 if (ORIGIN_REF) {
-                    h = q.q2 - l32 * l32;
-                    m = l32 * irk[k];
-                    corr = h * sck[k] * (1.f + m * (m - 1.f));
-                    ql = (qpk[k] + q.qb) + (l32 + corr) * invdr;
+                    const float h = q.q2 - l32 * l32;
+                    const float m = l32 * irk[k];
+                    const float hs = h * sck[k];
+                    corr = hs - hs * m;
                 } else {
-                    h = ek[k] - l32 * l32;
-                    m = l32 * q.invr;
-                    corr = h * q.scale * (1.f + m * (m - 1.f));
-                    ql = q.qb + (l32 + corr) * invdr;
+                    const float h = ek[k] - l32 * l32;
+                    const float m = l32 * q.invr;
+                    const float hs = h * q.scale;
+                    float p = fmaf(m, m - 1.f, 1.f);
+                    p = fmaf(-hs, q.scale, p);
+                    corr = hs * p;
                 }
+                if (ORIGIN_REF)
+                    ql = (qpk[k] + q.qb) + (l32 + corr) * invdr;
+                else
+                    ql = q.qb + (l32 + corr) * invdr;
                 const float turns = (float)T + corr * kt32;
                 int j = (int)ql;
+                const float tt = hw_fract32(ql);
                 j = min(max(j, 0), SP_W - 1);
-                const float tt = ql - (float)j;
                 float2 v;
                 if (INTERP == INTERP_LAGRANGE4) {
                     const float4 lo = cb[2 * j], hi = cb[2 * j + 1];
@@ -622,8 +640,10 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                 }
                 float sn, cs;
                 sincos_turns(turns, &sn, &cs);
-                acc[k].x += v.x * cs - v.y * sn;
-                acc[k].y += v.x * sn + v.y * cs;
+                acc[k].x = fmaf(v.x, cs, acc[k].x);
+                acc[k].x = fmaf(-v.y, sn, acc[k].x);
+                acc[k].y = fmaf(v.x, sn, acc[k].y);
+                acc[k].y = fmaf(v.y, cs, acc[k].y);
             }
         }
         __syncthreads();

@@ -334,18 +334,29 @@ __global__ __launch_bounds__(256) void contrib_kernel(const float2* __restrict__
 // divisions (hardware reciprocal square root / reciprocal + Newton steps),
 // the destination pixel evaluated once per parent pixel, and Kaiser-sinc
 // weights from per-tap polynomials (degree 8 in s = 2t - 1, fitted by the
-// library to the tabulated kernel at upload; max weight error ~5e-7).
+// library to the tabulated kernel at upload; max weight error ~2.5e-7).
+//
+// The 8-tap kernel is mirror-symmetric, w_{7-a}(s) = w_a(-s), so each pair
+// (a, 7 - a) shares an even part E_a(s^2) (5 coefficients) and an odd part
+// s O_a(s^2) (4 coefficients): w_a = E + s O, w_{7-a} = E - s O.
 // ---------------------------------------------------------------------------
-__constant__ float c_kaiser_poly[8][9];
+constexpr int KPOLY_EVEN = 5, KPOLY_ODD = 4, KPOLY_STRIDE = KPOLY_EVEN + KPOLY_ODD;
+__constant__ float c_kaiser_poly[4][KPOLY_STRIDE];
 
 __device__ __forceinline__ void kaiser_weights_poly(float t, float w[8]) {
     const float s = 2.f * t - 1.f;
+    const float x = s * s;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        float v = c_kaiser_poly[a][8];
+    for (int a = 0; a < 4; ++a) {
+        const float* c = c_kaiser_poly[a];
+        float e = c[KPOLY_EVEN - 1];
 #pragma unroll
-        for (int j = 7; j >= 0; --j) v = v * s + c_kaiser_poly[a][j];
-        w[a] = v;
+        for (int j = KPOLY_EVEN - 2; j >= 0; --j) e = fmaf(e, x, c[j]);
+        float o = c[KPOLY_STRIDE - 1];
+#pragma unroll
+        for (int j = KPOLY_ODD - 2; j >= 0; --j) o = fmaf(o, x, c[KPOLY_EVEN + j]);
+        w[a] = fmaf(s, o, e);
+        w[7 - a] = fmaf(-s, o, e);
     }
 }
 
@@ -374,56 +385,93 @@ __device__ __forceinline__ FastSrc make_fast_src(const float2* img, const double
     return f;
 }
 
-template <int INTERP>
-__device__ __forceinline__ float2 sample2_fast(const FastSrc& f, double qr, double qu) {
-    float2 z = make_float2(0.f, 0.f);
+template <int INTERP> struct FastShape;
+template <> struct FastShape<INTERP_KAISER8> { static constexpr int TAPS = 8, FIRST = 3; };
+template <> struct FastShape<INTERP_LAGRANGE4> { static constexpr int TAPS = 4, FIRST = 1; };
+
+// Where a destination point falls in a source polar image: stencil origin
+// (floor indices), fractional offsets and the phase rotation exp(j k (rr - r))
+// (r = 0 remodulates by exp(j k rr)).  ok = false when the point lies outside
+// the image's valid stencil range or on the origin; the sample is then 0.
+// ir/iu hold the floor indices whenever they are defined (also when !ok),
+// else INT_MAX.
+struct ChildTap {
     int ir, iu;
-    if (!floor_index(qr, &ir) || !floor_index(qu, &iu)) return z;
-    if (!InterpShape<INTERP>::valid(ir, f.nr) || !InterpShape<INTERP>::valid(iu, f.nu)) return z;
-    const float tr = (float)(qr - (double)ir), tu = (float)(qu - (double)iu);
+    float tr, tu;
+    float2 rot;
+    bool ok;
+};
+
+template <int INTERP>
+__device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double y, double r,
+                                              double kt) {
+    ChildTap c;
+    c.ok = false;
+    c.ir = 0x7fffffff;
+    c.iu = 0x7fffffff;
+    c.tr = c.tu = 0.f;
+    c.rot = make_float2(0.f, 0.f);
+    const double dx = x - f.ox, dy = y - f.oy;
+    const double s2 = dx * dx + dy * dy;
+    if (!(s2 > 0.0)) return c;
+    const double uu = dy * rsqrt64(s2);
+    const double rr2 = s2 + f.oz2;
+    const double rr = rr2 * rsqrt64(rr2);
+    const double qr = (rr - f.r0) * f.inv_dr, qu = (uu - f.u0) * f.inv_du;
+    int ir, iu;
+    if (!floor_index(qr, &ir) || !floor_index(qu, &iu)) return c;
+    c.ir = ir;
+    c.iu = iu;
+    if (!InterpShape<INTERP>::valid(ir, f.nr) || !InterpShape<INTERP>::valid(iu, f.nu)) return c;
+    c.tr = (float)(qr - (double)ir);
+    c.tu = (float)(qu - (double)iu);
+    c.rot = cis_turns64(kt * (rr - r));
+    c.ok = true;
+    return c;
+}
+
+// Separable stencil sum times the phase rotation.  row(a) points at tap
+// (a, 0) of the stencil (row ir - FIRST + a, column iu - FIRST), in global
+// memory or in an LDS copy: the arithmetic is identical either way.
+template <int INTERP, typename Row>
+__device__ __forceinline__ float2 stencil(const ChildTap& c, Row row) {
+    constexpr int TAPS = FastShape<INTERP>::TAPS;
+    float wr[TAPS], wu[TAPS];
     if (INTERP == INTERP_KAISER8) {
-        float wr[8], wu[8];
-        kaiser_weights_poly(tr, wr);
-        kaiser_weights_poly(tu, wu);
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const float2* row = f.img + (size_t)(ir + a - 3) * f.ld + (iu - 3);
-            float sx = 0.f, sy = 0.f;
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const float2 v = row[b];
-                sx += wu[b] * v.x;
-                sy += wu[b] * v.y;
-            }
-            z.x += wr[a] * sx;
-            z.y += wr[a] * sy;
-        }
-        return z;
+        kaiser_weights_poly(c.tr, wr);
+        kaiser_weights_poly(c.tu, wu);
+    } else {
+        cubic_weights(c.tr, wr);
+        cubic_weights(c.tu, wu);
     }
-    float wr[4], wu[4];
-    cubic_weights(tr, wr);
-    cubic_weights(tu, wu);
+    float2 z = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const float2* row = f.img + (size_t)(ir + a - 1) * f.ld + (iu - 1);
+    for (int a = 0; a < TAPS; ++a) {
+        const float2* rp = row(a);
         float sx = 0.f, sy = 0.f;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const float2 v = row[b];
+        for (int b = 0; b < TAPS; ++b) {
+            const float2 v = rp[b];
             sx += wu[b] * v.x;
             sy += wu[b] * v.y;
         }
         z.x += wr[a] * sx;
         z.y += wr[a] * sy;
     }
-    return z;
+    return make_float2(z.x * c.rot.x - z.y * c.rot.y, z.x * c.rot.y + z.y * c.rot.x);
+}
+
+template <int INTERP>
+__device__ __forceinline__ float2 stencil_global(const FastSrc& f, const ChildTap& c) {
+    constexpr int FIRST = FastShape<INTERP>::FIRST;
+    const float2* base = f.img + (long long)(c.ir - FIRST) * f.ld + (c.iu - FIRST);
+    const long long ld = f.ld;
+    return stencil<INTERP>(c, [&](int a) { return base + a * ld; });
 }
 
 // Destination polar pixel p of node geometry g (fast_bp/ffbp.py:62-67).
-__device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long long p, int nu,
-                                                double* x, double* y, double* r) {
-    const int ir = (int)(p / nu);
-    const int iu = (int)(p - (long long)ir * nu);
+__device__ __forceinline__ void dest_pixel_rc(const double* g, int side, int ir, int iu, double* x,
+                                              double* y, double* r) {
     const double rr = g[3] + (double)ir * g[4];
     const double u = g[5] + (double)iu * g[6];
     const double g2 = rr * rr - g[2] * g[2];
@@ -435,33 +483,24 @@ __device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long 
     *r = rr;
 }
 
+__device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long long p, int nu,
+                                                double* x, double* y, double* r) {
+    const int ir = (int)(p / nu);
+    dest_pixel_rc(g, side, ir, (int)(p - (long long)ir * nu), x, y, r);
+}
+
 // Child regrid at destination point (x, y) with destination slant range r.
 template <int INTERP>
 __device__ __forceinline__ float2 regrid_fast(const FastSrc& f, double x, double y, double r,
                                               double kt) {
-    const double dx = x - f.ox, dy = y - f.oy;
-    const double s2 = dx * dx + dy * dy;
-    if (!(s2 > 0.0)) return make_float2(0.f, 0.f);
-    const double uu = dy * rsqrt64(s2);
-    const double rr2 = s2 + f.oz2;
-    const double rr = rr2 * rsqrt64(rr2);
-    const float2 v = sample2_fast<INTERP>(f, (rr - f.r0) * f.inv_dr, (uu - f.u0) * f.inv_du);
-    const float2 rot = cis_turns64(kt * (rr - r));
-    return make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+    const ChildTap c = child_tap<INTERP>(f, x, y, r, kt);
+    return c.ok ? stencil_global<INTERP>(f, c) : make_float2(0.f, 0.f);
 }
 
 // Polar image -> point (x, y), remodulated by exp(+j k r) (ffbp.py:189-198).
 template <int INTERP>
 __device__ __forceinline__ float2 eval_fast(const FastSrc& f, double x, double y, double kt) {
-    const double dx = x - f.ox, dy = y - f.oy;
-    const double s2 = dx * dx + dy * dy;
-    if (!(s2 > 0.0)) return make_float2(0.f, 0.f);
-    const double u = dy * rsqrt64(s2);
-    const double r2 = s2 + f.oz2;
-    const double r = r2 * rsqrt64(r2);
-    const float2 v = sample2_fast<INTERP>(f, (r - f.r0) * f.inv_dr, (u - f.u0) * f.inv_du);
-    const float2 rot = cis_turns64(kt * r);
-    return make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
+    return regrid_fast<INTERP>(f, x, y, 0.0, kt);
 }
 
 template <int INTERP>
@@ -526,6 +565,162 @@ __global__ __launch_bounds__(256) void batched_merge_fast_kernel(
     arena[offsets[node] + p] = make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y));
 }
 
+// ---- LDS-tiled batched merge (split mode) -----------------------------------
+// A workgroup owns a 32 x 32 tile of one parent node (4 pixels per thread, in
+// rows t/32, t/32 + 8, ..).  The child-coordinate map is smooth over a tile,
+// so the stencil origins of the tile's four corner pixels, widened by one
+// cell, bound the tile's footprint in each child (typically ~43 x 27
+// samples); the workgroup copies that window into LDS once and the stencil
+// sums gather from LDS instead of re-fetching ~1 KB per parent pixel through
+// the vector cache.  A pixel whose stencil is not inside the window, or a
+// window above MT_CAP samples, reads global memory with identical
+// arithmetic, so results never depend on the path taken.
+constexpr int MT_ROWS = 32, MT_COLS = 32, MT_THREADS = 256;
+constexpr int MT_PPT = MT_ROWS * MT_COLS / MT_THREADS;
+constexpr int MT_CAP = 1536;
+
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+struct Window {
+    int r0, c0, h, w;  // rows [r0, r0 + h), columns [c0, c0 + w) of the child
+    bool lds;
+};
+
+// Window from the corner stencil origins ir[4], iu[4] (INT_MAX if undefined).
+template <int INTERP>
+__device__ __forceinline__ Window corner_window(const int* ir, const int* iu, int nr, int nu) {
+    constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
+    constexpr int LIM = 0x3fffffff;  // stencil origins of any real image are far below
+    Window w;
+    int rlo = ir[0], rhi = ir[0], clo = iu[0], chi = iu[0];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) {
+        rlo = min(rlo, ir[j]); rhi = max(rhi, ir[j]);
+        clo = min(clo, iu[j]); chi = max(chi, iu[j]);
+    }
+    const bool defined = rlo > -LIM && rhi < LIM && clo > -LIM && chi < LIM;
+    // widen by one cell, cover the stencils, clip to the image
+    const int r0 = max(rlo - 1 - FIRST, 0), r1 = min(rhi + 1 + (TAPS - 1 - FIRST), nr - 1);
+    const int c0 = max(clo - 1 - FIRST, 0), c1 = min(chi + 1 + (TAPS - 1 - FIRST), nu - 1);
+    const int h = r1 - r0 + 1, wd = c1 - c0 + 1;
+    w.lds = defined && h > 0 && wd > 0 && h <= MT_CAP && wd <= MT_CAP && h * wd <= MT_CAP;
+    w.r0 = r0;
+    w.c0 = c0;
+    w.h = w.lds ? h : 0;
+    w.w = w.lds ? wd : 0;
+    return w;
+}
+
+template <int INTERP>
+__device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w) {
+    constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
+    return w.lds && c.ir - FIRST >= w.r0 && c.ir - FIRST + TAPS <= w.r0 + w.h &&
+           c.iu - FIRST >= w.c0 && c.iu - FIRST + TAPS <= w.c0 + w.w;
+}
+
+// Copy the window into LDS: wave v takes rows v, v + 4, .., lane j columns
+// j, j + 64, ..; per-row steps are uniform pointer increments.
+__device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, float2* dst,
+                                             int wave, int lane) {
+    constexpr int WAVES = MT_THREADS / 64;
+    if (wave >= w.h) return;
+    const float2* src = f.img + (long long)(w.r0 + wave) * f.ld + w.c0;
+    const long long src_step = (long long)WAVES * f.ld;
+    float2* d = dst + wave * w.w;
+    const int dst_step = WAVES * w.w;
+    if (w.w <= 64) {
+        if (lane < w.w) {
+            for (int i = wave; i < w.h; i += WAVES) {
+                d[lane] = src[lane];
+                src += src_step;
+                d += dst_step;
+            }
+        }
+        return;
+    }
+    for (int i = wave; i < w.h; i += WAVES) {
+        for (int j = lane; j < w.w; j += 64) d[j] = src[j];
+        src += src_step;
+        d += dst_step;
+    }
+}
+
+template <int INTERP>
+__device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
+                                              const float2* sh) {
+    constexpr int FIRST = FastShape<INTERP>::FIRST;
+    if (!c.ok) return make_float2(0.f, 0.f);
+    if (in_window<INTERP>(c, w)) {
+        const float2* base = sh + (c.ir - FIRST - w.r0) * w.w + (c.iu - FIRST - w.c0);
+        const int pitch = w.w;
+        return stencil<INTERP>(c, [&](int a) { return base + a * pitch; });
+    }
+    return stencil_global<INTERP>(f, c);
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(MT_THREADS) void batched_merge_tiled_kernel(
+        const int* __restrict__ node_int, const double* __restrict__ node_geo,
+        const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
+        double k) {
+    __shared__ float2 sh[2][MT_CAP];
+    __shared__ int corner[4][4];  // corner j: ir_a, iu_a, ir_b, iu_b
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
+    const int* ni = node_int + 8 * node;
+    const int nr = ni[2], nu = ni[3], side = ni[4], l = ni[5], r = ni[6];
+    const int tiles_u = (nu + MT_COLS - 1) / MT_COLS;
+    const int tr = tile / tiles_u, tu = tile - tr * tiles_u;
+    const int row0 = tr * MT_ROWS + t / MT_COLS, col = tu * MT_COLS + (t % MT_COLS);
+    constexpr int ROW_STEP = MT_THREADS / MT_COLS;
+
+    const int* li = node_int + 8 * l;
+    const int* ri = node_int + 8 * r;
+    const FastSrc fa = make_fast_src(arena + offsets[l], node_geo + 7 * l, li[2], li[3], li[3]);
+    const FastSrc fb = make_fast_src(arena + offsets[r], node_geo + 7 * r, ri[2], ri[3], ri[3]);
+    const double* g = node_geo + 7 * node;
+    const double kt = k * INV_TWO_PI;
+    // the tile's corner pixels (clamped to the node) locate the windows
+    const int cj = t == 0 ? 0 : t == MT_COLS - 1 ? 1 : t == MT_THREADS - MT_COLS ? 2
+                 : t == MT_THREADS - 1 ? 3 : -1;
+    if (cj >= 0) {
+        double x, y, rr;
+        dest_pixel_rc(g, side, min(row0 + (MT_PPT - 1) * ROW_STEP * (cj >> 1), nr - 1),
+                      min(col, nu - 1), &x, &y, &rr);
+        const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
+        const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
+        corner[cj][0] = ca.ir; corner[cj][1] = ca.iu;
+        corner[cj][2] = cb.ir; corner[cj][3] = cb.iu;
+    }
+    __syncthreads();
+    int ira[4], iua[4], irb[4], iub[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ira[j] = uniform_i32(corner[j][0]); iua[j] = uniform_i32(corner[j][1]);
+        irb[j] = uniform_i32(corner[j][2]); iub[j] = uniform_i32(corner[j][3]);
+    }
+    const Window wa = corner_window<INTERP>(ira, iua, fa.nr, fa.nu);
+    const Window wb = corner_window<INTERP>(irb, iub, fb.nr, fb.nu);
+    if (wa.lds) stage_window(fa, wa, sh[0], wave, lane);
+    if (wb.lds) stage_window(fb, wb, sh[1], wave, lane);
+    __syncthreads();
+    if (col >= nu) return;
+    float2* out = arena + offsets[node];
+#pragma unroll 1
+    for (int j = 0; j < MT_PPT; ++j) {
+        const int row = row0 + j * ROW_STEP;
+        if (row >= nr) break;
+        double x, y, rr;
+        dest_pixel_rc(g, side, row, col, &x, &y, &rr);
+        const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
+        const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
+        const float2 va = tile_sample<INTERP>(fa, ca, wa, sh[0]);
+        const float2 vb = tile_sample<INTERP>(fb, cb, wb, sh[1]);
+        out[(long long)row * nu + col] = make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y));
+    }
+}
+
 template <int INTERP>
 __global__ __launch_bounds__(256) void eval_points_fast_kernel(const float2* __restrict__ img, Geo g,
                                                                int nr, int nu, int ld,
@@ -564,12 +759,14 @@ __global__ __launch_bounds__(256) void eval_grid_multi_fast_kernel(
                                             j.x0 + ix * j.dx, j.y0 + iy * j.dy, j.k * INV_TWO_PI);
 }
 
-// Chebyshev interpolation (9 nodes) of each normalised 8-tap Kaiser weight
-// w_a(t), t in [0,1), read from the FP64 table exactly as the FP64 paths do,
-// converted to a power series in s = 2t - 1.
-static void fit_kaiser_poly(const double* table, float out[8][9]) {
+// Chebyshev interpolation (9 nodes) of the normalised 8-tap Kaiser weights
+// w_a(t), a = 0..3, t in [0,1), read from the FP64 table exactly as the FP64
+// paths do, converted to a power series in s = 2t - 1 and split into even
+// coefficients (s^0, s^2, .., s^8) and odd coefficients (s^1, .., s^7).
+// Taps 4..7 are the mirror images w_{7-a}(s) = w_a(-s).
+static void fit_kaiser_poly(const double* table, float out[4][KPOLY_STRIDE]) {
     constexpr int N = 9;
-    double f[8][N];
+    double f[4][N];
     double sn[N];
     for (int j = 0; j < N; ++j) {
         sn[j] = cos(M_PI * (j + 0.5) / N);
@@ -583,7 +780,7 @@ static void fit_kaiser_poly(const double* table, float out[8][9]) {
             w[a] = table[i] + fr * (table[i + 1] - table[i]);
             sum += w[a];
         }
-        for (int a = 0; a < 8; ++a) f[a][j] = w[a] / sum;
+        for (int a = 0; a < 4; ++a) f[a][j] = w[a] / sum;
     }
     // power-basis coefficients of the Chebyshev polynomials T_0..T_8
     double T[N][N] = {};
@@ -592,7 +789,7 @@ static void fit_kaiser_poly(const double* table, float out[8][9]) {
     for (int k = 2; k < N; ++k)
         for (int m = 0; m < N; ++m)
             T[k][m] = (m > 0 ? 2.0 * T[k - 1][m - 1] : 0.0) - T[k - 2][m];
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < 4; ++a) {
         double poly[N] = {};
         for (int k = 0; k < N; ++k) {
             double c = 0.0;
@@ -600,7 +797,8 @@ static void fit_kaiser_poly(const double* table, float out[8][9]) {
             c *= (k == 0 ? 1.0 : 2.0) / N;
             for (int m = 0; m < N; ++m) poly[m] += c * T[k][m];
         }
-        for (int m = 0; m < N; ++m) out[a][m] = (float)poly[m];
+        for (int i = 0; i < KPOLY_EVEN; ++i) out[a][i] = (float)poly[2 * i];
+        for (int i = 0; i < KPOLY_ODD; ++i) out[a][KPOLY_EVEN + i] = (float)poly[2 * i + 1];
     }
 }
 
@@ -742,7 +940,7 @@ int bpmi_set_kaiser_table(const double* host_table, int len) {
         int st = check_hip(hipMalloc((void**)&g_kaiser_table, bytes), "hipMalloc(kaiser table)");
         if (st) return st;
     }
-    float poly[8][9];
+    float poly[4][KPOLY_STRIDE];
     fit_kaiser_poly(host_table, poly);
     int st = check_hip(hipMemcpyToSymbol(HIP_SYMBOL(c_kaiser_poly), poly, sizeof(poly)),
                        "hipMemcpyToSymbol(kaiser polynomials)");
@@ -842,6 +1040,33 @@ int bpmi_ffbp_merges(const int* node_int, const double* node_geo, const long lon
     };
     dispatch2(mode, interp, l);
     return check_launch("batched_merge_kernel");
+}
+
+int bpmi_merge_tiles(int nr, int nu, int* tiles) {
+    if (!tiles || nr <= 0 || nu <= 0) return set_error(BPMI_ERR_INVALID, "bad node shape");
+    const long long t = (long long)((nr + MT_ROWS - 1) / MT_ROWS) * ((nu + MT_COLS - 1) / MT_COLS);
+    if (t > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "node too large");
+    *tiles = (int)t;
+    return BPMI_OK;
+}
+
+int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo, const long long* offsets,
+                           void* arena, const int* blocks, int nblocks, double k, int interp,
+                           void* stream) {
+    int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
+    if (st) return st;
+    if (nblocks <= 0) return BPMI_OK;
+    if (!node_int || !node_geo || !offsets || !arena || !blocks)
+        return set_error(BPMI_ERR_INVALID, "null table");
+    float2* ar = (float2*)arena;
+    hipStream_t hs = (hipStream_t)stream;
+    if (interp == BPMI_INTERP_KAISER8)
+        hipLaunchKernelGGL(batched_merge_tiled_kernel<INTERP_KAISER8>, dim3(nblocks),
+                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
+    else
+        hipLaunchKernelGGL(batched_merge_tiled_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
+                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
+    return check_launch("batched_merge_tiled_kernel");
 }
 
 int bpmi_eval_points(const void* img, const double* geo, int nr, int nu, int ld, const double* xyz,

@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Event, Graph, Stream, current_stream, elapsed_ms, stream_handle, synchronize
+from .batched_resident_backend import enqueue_merges, merge_block_table
 from .gpu_backend import build_ffbp_plan, ensure_kaiser_table
 from .plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, flatten_plan
 from .scenes import CartesianGrid, polar_extent_for_grid, stripmap_scene
@@ -132,7 +133,7 @@ class MultiPatchProgram:
         self.merge_tables = []
         for h in range(1, int(heights.max()) + 1):
             nodes = np.flatnonzero(heights == h)
-            mapping = [(int(n), s) for n in nodes for s in range(0, int(sizes[n]), BLOCK)]
+            mapping = merge_block_table(self.node_int, nodes, self.mode)
             self.merge_tables.append(DeviceArray.from_host(np.asarray(mapping, np.int32)))
         # Cartesian evaluation jobs (crop as a view into each root)
         jobs = np.zeros(len(self.patches), EVAL_JOB)
@@ -195,9 +196,9 @@ class MultiPatchProgram:
                   _vp(self.leaf_blocks.ptr), self.n_leaf_blocks, mode,
                   _lib.interp_code(self.leaf_interp), self.tile_cols_log2, s)
         for table in self.merge_tables:
-            _lib.call("bpmi_ffbp_merges", _vp(self.t_node_int.ptr), _vp(self.t_node_geo.ptr),
-                      _vp(self.t_offsets.ptr), _vp(self.arena.ptr), _vp(table.ptr),
-                      table.shape[0], self.k, mode, _lib.interp_code(self.merge_interp), s)
+            enqueue_merges(self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
+                           self.arena.ptr, table.ptr, table.shape[0], self.k, self.mode,
+                           self.merge_interp, s)
         _lib.call("bpmi_eval_grid_multi", _vp(self.t_jobs.ptr), _vp(self.t_eval_blocks.ptr),
                   self.n_eval_blocks, mode, _lib.interp_code(self.merge_interp), s)
         return self.outputs
@@ -243,7 +244,8 @@ class MultiPatchProgram:
 
 
 def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
-                         angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8"):
+                         angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
+                         tile_cols_log2=6):
     """Per-patch programs (the same kernels, one patch at a time): returns the
     outputs, each patch's isolated event time and its completion time when
     the patches are launched back to back."""
@@ -254,7 +256,8 @@ def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
     for s in scenes:
         pp = PatchPlan(s, leaf_size, range_spacing, angular_oversample)
         dev = backend.upload(s.data, s.pos, s.weight, s.phase, pp.plan)
-        prog = BatchedResidentProgram(backend, pp.plan, dev, mode, s.r_start, s.data_dr, s.k)
+        prog = BatchedResidentProgram(backend, pp.plan, dev, mode, s.r_start, s.data_dr, s.k,
+                                      leaf_tile_cols_log2=tile_cols_log2)
         out = DeviceArray.empty(s.grid.shape)
         progs.append((pp, prog, out, geo_array(pp.plan)))
 

@@ -209,6 +209,16 @@ int bpmi_ffbp_merges(const int* node_int, const double* node_geo, const long lon
  * leaf node of nr x nu pixels. */
 int bpmi_split_tiles(int nr, int nu, int* tiles);
 
+/* Split-mode merges of one subtree height with 2-D tiles
+ * (batched_resident_backend.py:72-150 merge stage, LDS-tiled).  ``blocks``
+ * int32 [nblocks, 2] = (node, tile) pairs, bpmi_merge_tiles() tiles of
+ * 32 r-rows x 32 u-columns per parent node; tables as for bpmi_ffbp_merges.
+ * Bitwise equal to bpmi_ffbp_merges in BPMI_MODE_SPLIT. */
+int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo,
+                           const long long* offsets, void* arena, const int* blocks,
+                           int nblocks, double k, int interp, void* stream);
+int bpmi_merge_tiles(int nr, int nu, int* tiles);
+
 /* ---- many independent FFBP patches in one launch per level -------------- */
 /* Per-node input description for leaves of different acquisitions (no
  * reference counterpart: the reference batches the nodes of one image,

@@ -72,3 +72,16 @@ def test_gradient_autofocus_lowers_entropy(scene):
     print("entropy", res["entropy"][0], "->", res["best_entropy"],
           "ms/iter", np.median(res["ms_per_iteration"]))
     assert res["best_entropy"] < res["entropy"][0] - 0.05
+
+
+def test_torch_sees_device_after_library_load():
+    """Loading libbpmi.so before torch must leave torch's HIP device usable
+    (both bind one HIP runtime)."""
+    import subprocess
+    import sys
+    code = ("from bp_project_amd import _lib; _lib.load(); "
+            "import torch; assert torch.cuda.is_available(); "
+            "x = torch.ones(4, device='cuda'); print(float(x.sum()))")
+    proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stderr[-2000:]
+    assert proc.stdout.strip().endswith("4.0")

@@ -21,7 +21,8 @@ def test_multipatch_equals_sequential(scenes, mode):
     prog.execute()
     synchronize()
     outs = [o.get() for o in prog.outputs]
-    seq, isolated, completion, _ = sequential_reference(scenes, mode=mode, merge_interp="kaiser8")
+    seq, isolated, completion, _ = sequential_reference(scenes, mode=mode, merge_interp="kaiser8",
+                                                        tile_cols_log2=prog.tile_cols_log2)
     for a, b in zip(outs, seq):
         assert np.array_equal(a, b.get())
     assert prog.capture_graph()

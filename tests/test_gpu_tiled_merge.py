@@ -1,0 +1,80 @@
+"""Split-mode LDS-tiled merge (bpmi_ffbp_merges_tiled) on hand-built node
+tables: bitwise equal to the untiled split merge (bpmi_ffbp_merges) and
+within 1e-4 of the FP64-mode merge, both when each child's footprint fits
+the LDS tile (coarse child u sampling) and when it does not and the kernel
+gathers from global memory (child u sampling 8x finer than the parent's).
+Part of the parent lies outside the children's valid stencil range, so
+zero-filled pixels are covered too."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+_vp = ctypes.c_void_p
+
+
+def _tables(child_du):
+    from bp_project_amd.plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, NI_SIDE
+    parent = dict(o=(0.0, 0.0, 3000.0), r0=5500.0, dr=0.5, nr=45, u0=-0.02, du=1e-3, nu=45)
+    nu_c = int(np.ceil(0.07 / child_du)) + 1
+    kids = [dict(o=(0.0, s * 20.0, 3000.0), r0=5501.0, dr=0.5, nr=220, u0=-0.035, du=child_du,
+                 nu=nu_c) for s in (-1.0, 1.0)]
+    nodes = [parent] + kids
+    node_int = np.zeros((3, 8), np.int32)
+    node_geo = np.zeros((3, 7), np.float64)
+    offsets = np.zeros(3, np.int64)
+    cursor = 0
+    for i, n in enumerate(nodes):
+        node_int[i, NI_NR], node_int[i, NI_NU], node_int[i, NI_SIDE] = n["nr"], n["nu"], 1
+        node_int[i, NI_LEFT] = node_int[i, NI_RIGHT] = -1
+        node_geo[i] = (*n["o"], n["r0"], n["dr"], n["u0"], n["du"])
+        offsets[i] = cursor
+        cursor += -(-n["nr"] * n["nu"] // 64) * 64
+    node_int[0, NI_LEFT], node_int[0, NI_RIGHT], node_int[0, NI_HEIGHT] = 1, 2, 1
+    return nodes, node_int, node_geo, offsets, cursor
+
+
+@pytest.mark.parametrize("child_du", [2e-3, 1.25e-4])
+@pytest.mark.parametrize("interp", ["kaiser8", "lagrange4"])
+def test_tiled_merge_matches_untiled(child_du, interp):
+    from bp_project_amd import _lib
+    from bp_project_amd.batched_resident_backend import merge_block_table
+    from bp_project_amd.device import DeviceArray, synchronize
+    from bp_project_amd.gpu_backend import ensure_kaiser_table
+    ensure_kaiser_table()
+    nodes, node_int, node_geo, offsets, size = _tables(child_du)
+    rng = np.random.default_rng(7)
+    arena_host = np.zeros(size, np.complex64)
+    for i in (1, 2):
+        n = nodes[i]["nr"] * nodes[i]["nu"]
+        arena_host[offsets[i]:offsets[i] + n] = (rng.standard_normal(n) +
+                                                 1j * rng.standard_normal(n)).astype(np.complex64)
+    t_int, t_geo = DeviceArray.from_host(node_int), DeviceArray.from_host(node_geo)
+    t_off = DeviceArray.from_host(offsets)
+    k = 4 * np.pi * 9.6e9 / 299792458.0
+    p0 = nodes[0]["nr"] * nodes[0]["nu"]
+    results = {}
+    for name in ("tiled", "untiled", "fp64"):
+        arena = DeviceArray.from_host(arena_host)
+        mode = "fp64" if name == "fp64" else "split"
+        table = np.asarray(merge_block_table(node_int, [0], "split" if name == "tiled" else "x"),
+                           np.int32)
+        blocks = DeviceArray.from_host(table)
+        if name == "tiled":
+            _lib.call("bpmi_ffbp_merges_tiled", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
+                      _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.interp_code(interp),
+                      None)
+        else:
+            _lib.call("bpmi_ffbp_merges", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
+                      _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.mode_code(mode),
+                      _lib.interp_code(interp), None)
+        synchronize()
+        results[name] = arena.get()[:p0]
+    a, b, c = results["tiled"], results["untiled"], results["fp64"]
+    zero = np.count_nonzero(a == 0)
+    print(f"child du {child_du} {interp}: {zero} of {p0} parent pixels outside the children")
+    assert 0 < zero < p0 // 2
+    assert np.array_equal(a, b)
+    assert np.linalg.norm(a - c) <= 1e-4 * np.linalg.norm(c)
