@@ -40,6 +40,13 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Gpix·pulse/s (BP, FFBP) at NRMSE≤1e-4 vs FP64; ms per autofocus grad step"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by its reading for a
+# 1 GiB copy kernel (FETCH x 2.0, WRITE x 1.0); see
+# profiles/ffbp_c2_merge_counters.md.  Keyed by (workload, mode, stage).
+MEASURED_TRAFFIC_BYTES = {
+    ("c2_ffbp_full", "split", "batched_merge_regrid_split"): 8.40e9,
+}
 # Algorithmic operation counts (add/sub/mul = 1, FMA = 2, sqrt/sin/cos = 1).
 OPS_LEAF_LAGRANGE4 = 57      # per leaf pixel*pulse (4-point Lagrange)
 OPS_DIRECT_LINEAR = 35       # per pixel*pulse (linear interpolation)
@@ -189,14 +196,19 @@ def run_ffbp(args, scene):
         "stages": per_stage,
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": MEASURED_TRAFFIC_BYTES.get((args.workload, args.mode, dominant)),
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                            "copy-kernel calibrated)",
             "kernel": dominant,
             "ops_per_unit": (OPS_LEAF_LAGRANGE4 if "bp_polar" in dominant else
                              ops_merge if "merge" in dominant else OPS_EVAL_KAISER8),
             "unit_of_work": ("leaf pixel*pulse" if "bp_polar" in dominant else
                              "parent pixel" if "merge" in dominant else "output pixel"),
             "launches_per_step": per_stage[dominant]["launches_per_step"],
-            "note": "FP32 vector-ALU bound (no MFMA); peak = MI355X FP32 vector 157.3 TFLOP/s",
+            "note": "FP32 vector-ALU bound (no MFMA); peak = MI355X FP32 vector 157.3 "
+                    "TFLOP/s (packed); packed FP32 is disabled in this build, so FMA "
+                    "code tops out at the unpacked 78.6 TFLOP/s",
         },
         "config": {
             "workload": "c2_ffbp_full", "pulses": scene.pulses, "bins": scene.bins,

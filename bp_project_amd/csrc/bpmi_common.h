@@ -59,12 +59,12 @@ __device__ __forceinline__ float2 cis_turns64(double turns) {
 // 1/sqrt(x) in FP64 from the hardware estimate plus two Newton steps
 // (x > 0, normal): relative error at the FP64 rounding level, no IEEE
 // square-root or division sequence.
+// 1/sqrt(x): hardware estimate (v_rsq_f64, relative error <= 5.2e-8 over
+// [1e-3, 1e9] on gfx950) and one Newton step, relative error <= 4.2e-15 --
+// 2.5e-11 m on a 6 km range, a 1.6e-9-turn phase error at X band.
 __device__ __forceinline__ double rsqrt64(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    double e = fma(-h, y * y, 0.5);
-    y = fma(y, e, y);
-    e = fma(-h, y * y, 0.5);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-0.5 * x, y * y, 0.5);
     return fma(y, e, y);
 }
 

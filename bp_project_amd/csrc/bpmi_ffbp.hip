@@ -374,15 +374,20 @@ struct FastSrc {
     int nr, nu, ld;
 };
 
-__device__ __forceinline__ FastSrc make_fast_src(const float2* img, const double* g, int nr, int nu,
-                                                 int ld) {
+__device__ __forceinline__ FastSrc make_fast_src_inv(const float2* img, const double* g, int nr,
+                                                     int nu, int ld, double inv_dr, double inv_du) {
     FastSrc f;
     f.img = img;
     f.ox = g[0]; f.oy = g[1]; f.oz2 = g[2] * g[2];
-    f.r0 = g[3]; f.inv_dr = rcp64(g[4]);
-    f.u0 = g[5]; f.inv_du = rcp64(g[6]);
+    f.r0 = g[3]; f.inv_dr = inv_dr;
+    f.u0 = g[5]; f.inv_du = inv_du;
     f.nr = nr; f.nu = nu; f.ld = ld;
     return f;
+}
+
+__device__ __forceinline__ FastSrc make_fast_src(const float2* img, const double* g, int nr, int nu,
+                                                 int ld) {
+    return make_fast_src_inv(img, g, nr, nu, ld, rcp64(g[4]), rcp64(g[6]));
 }
 
 template <int INTERP> struct FastShape;
@@ -405,28 +410,26 @@ struct ChildTap {
 template <int INTERP>
 __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double y, double r,
                                               double kt) {
-    ChildTap c;
-    c.ok = false;
-    c.ir = 0x7fffffff;
-    c.iu = 0x7fffffff;
-    c.tr = c.tu = 0.f;
-    c.rot = make_float2(0.f, 0.f);
+    // Branch-free: every quantity is computed, then masked by the validity
+    // conditions (a point on the origin, an index beyond int range, a stencil
+    // leaving the image), which keeps the geometry in straight-line code.
     const double dx = x - f.ox, dy = y - f.oy;
     const double s2 = dx * dx + dy * dy;
-    if (!(s2 > 0.0)) return c;
-    const double uu = dy * rsqrt64(s2);
-    const double rr2 = s2 + f.oz2;
+    const bool off_origin = s2 > 0.0;
+    const double s2g = off_origin ? s2 : 1.0;
+    const double uu = dy * rsqrt64(s2g);
+    const double rr2 = s2g + f.oz2;
     const double rr = rr2 * rsqrt64(rr2);
     const double qr = (rr - f.r0) * f.inv_dr, qu = (uu - f.u0) * f.inv_du;
-    int ir, iu;
-    if (!floor_index(qr, &ir) || !floor_index(qu, &iu)) return c;
-    c.ir = ir;
-    c.iu = iu;
-    if (!InterpShape<INTERP>::valid(ir, f.nr) || !InterpShape<INTERP>::valid(iu, f.nu)) return c;
-    c.tr = (float)(qr - (double)ir);
-    c.tu = (float)(qu - (double)iu);
+    const double fr = floor(qr), fu = floor(qu);
+    const bool defined = off_origin && fabs(fr) < 2147483000.0 && fabs(fu) < 2147483000.0;
+    ChildTap c;
+    c.ir = defined ? (int)fr : 0x7fffffff;
+    c.iu = defined ? (int)fu : 0x7fffffff;
+    c.ok = defined && InterpShape<INTERP>::valid(c.ir, f.nr) && InterpShape<INTERP>::valid(c.iu, f.nu);
+    c.tr = (float)(qr - fr);
+    c.tu = (float)(qu - fu);
     c.rot = cis_turns64(kt * (rr - r));
-    c.ok = true;
     return c;
 }
 
@@ -470,17 +473,32 @@ __device__ __forceinline__ float2 stencil_global(const FastSrc& f, const ChildTa
 }
 
 // Destination polar pixel p of node geometry g (fast_bp/ffbp.py:62-67).
-__device__ __forceinline__ void dest_pixel_rc(const double* g, int side, int ir, int iu, double* x,
-                                              double* y, double* r) {
+// Column terms of a destination pixel: u and side * sqrt(1 - u^2).
+struct DestColumn {
+    double u, scu;
+};
+
+__device__ __forceinline__ DestColumn dest_column(const double* g, int side, int iu) {
+    DestColumn c;
+    c.u = g[5] + (double)iu * g[6];
+    const double c2 = 1.0 - c.u * c.u;
+    c.scu = (double)side * (c2 > 0.0 ? c2 * rsqrt64(c2) : 0.0);
+    return c;
+}
+
+__device__ __forceinline__ void dest_pixel_col(const double* g, const DestColumn& c, int ir,
+                                               double* x, double* y, double* r) {
     const double rr = g[3] + (double)ir * g[4];
-    const double u = g[5] + (double)iu * g[6];
     const double g2 = rr * rr - g[2] * g[2];
     const double ground = g2 > 0.0 ? g2 * rsqrt64(g2) : 0.0;
-    const double c2 = 1.0 - u * u;
-    const double cu = c2 > 0.0 ? c2 * rsqrt64(c2) : 0.0;
-    *x = g[0] + (double)side * ground * cu;
-    *y = g[1] + ground * u;
+    *x = g[0] + ground * c.scu;
+    *y = g[1] + ground * c.u;
     *r = rr;
+}
+
+__device__ __forceinline__ void dest_pixel_rc(const double* g, int side, int ir, int iu, double* x,
+                                              double* y, double* r) {
+    dest_pixel_col(g, dest_column(g, side, iu), ir, x, y, r);
 }
 
 __device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long long p, int nu,
@@ -569,15 +587,20 @@ __global__ __launch_bounds__(256) void batched_merge_fast_kernel(
 // A workgroup owns a 32 x 32 tile of one parent node (4 pixels per thread, in
 // rows t/32, t/32 + 8, ..).  The child-coordinate map is smooth over a tile,
 // so the stencil origins of the tile's four corner pixels, widened by one
-// cell, bound the tile's footprint in each child (typically ~43 x 27
-// samples); the workgroup copies that window into LDS once and the stencil
+// cell, bound the tile's footprint in each child (~1050 samples at the
+// lower levels, up to ~2050 at the root of an 8192-pulse aperture, where the
+// children's origins are far apart); the workgroup copies that window into
+// LDS once and the stencil
 // sums gather from LDS instead of re-fetching ~1 KB per parent pixel through
 // the vector cache.  A pixel whose stencil is not inside the window, or a
 // window above MT_CAP samples, reads global memory with identical
 // arithmetic, so results never depend on the path taken.
+// Launch bounds (256, 3) cap the kernel at 128 VGPRs: with ~35 KiB of LDS per
+// workgroup, 4 waves per SIMD is the occupancy either way, and tighter
+// register budgets (e.g. 67 VGPRs under (256, 4)) measured 14% slower.
 constexpr int MT_ROWS = 32, MT_COLS = 32, MT_THREADS = 256;
 constexpr int MT_PPT = MT_ROWS * MT_COLS / MT_THREADS;
-constexpr int MT_CAP = 1536;
+constexpr int MT_CAP = 2176;
 
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -659,12 +682,20 @@ __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& 
 }
 
 template <int INTERP>
-__global__ __launch_bounds__(MT_THREADS) void batched_merge_tiled_kernel(
+__global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
         const int* __restrict__ node_int, const double* __restrict__ node_geo,
         const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
         double k) {
-    __shared__ float2 sh[2][MT_CAP];
+    __shared__ float2 sh[2 * MT_CAP];
     __shared__ int corner[4][4];  // corner j: ir_a, iu_a, ir_b, iu_b
+    __shared__ double inv[4];     // 1/dr, 1/du of both children
+    // The second window's base offset is opaque to the compiler, so the
+    // stencil reads address it from a register rather than through
+    // per-instruction offset additions (ds_read2 offsets reach 2 KiB).
+    int second = MT_CAP;
+    asm volatile("" : "+s"(second));
+    float2* const sha = sh;
+    float2* const shb = sh + second;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
@@ -677,23 +708,34 @@ __global__ __launch_bounds__(MT_THREADS) void batched_merge_tiled_kernel(
 
     const int* li = node_int + 8 * l;
     const int* ri = node_int + 8 * r;
-    const FastSrc fa = make_fast_src(arena + offsets[l], node_geo + 7 * l, li[2], li[3], li[3]);
-    const FastSrc fb = make_fast_src(arena + offsets[r], node_geo + 7 * r, ri[2], ri[3], ri[3]);
+    const double* ga = node_geo + 7 * l;
+    const double* gb = node_geo + 7 * r;
     const double* g = node_geo + 7 * node;
     const double kt = k * INV_TWO_PI;
-    // the tile's corner pixels (clamped to the node) locate the windows
-    const int cj = t == 0 ? 0 : t == MT_COLS - 1 ? 1 : t == MT_THREADS - MT_COLS ? 2
-                 : t == MT_THREADS - 1 ? 3 : -1;
-    if (cj >= 0) {
-        double x, y, rr;
-        dest_pixel_rc(g, side, min(row0 + (MT_PPT - 1) * ROW_STEP * (cj >> 1), nr - 1),
-                      min(col, nu - 1), &x, &y, &rr);
-        const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
-        const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-        corner[cj][0] = ca.ir; corner[cj][1] = ca.iu;
-        corner[cj][2] = cb.ir; corner[cj][3] = cb.iu;
+    // The first wave alone computes the children's reciprocal spacings and
+    // the tile's corner pixels (clamped to the node), which locate the
+    // windows: lanes 0..3 take one corner each.
+    if (uniform_i32(wave) == 0) {
+        const FastSrc fa = make_fast_src(arena + offsets[l], ga, li[2], li[3], li[3]);
+        const FastSrc fb = make_fast_src(arena + offsets[r], gb, ri[2], ri[3], ri[3]);
+        if (t < 4) {
+            const int crow = min(tr * MT_ROWS + (t >> 1) * (MT_ROWS - 1), nr - 1);
+            const int ccol = min(tu * MT_COLS + (t & 1) * (MT_COLS - 1), nu - 1);
+            double x, y, rr;
+            dest_pixel_rc(g, side, crow, ccol, &x, &y, &rr);
+            const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
+            const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
+            corner[t][0] = ca.ir; corner[t][1] = ca.iu;
+            corner[t][2] = cb.ir; corner[t][3] = cb.iu;
+        }
+        if (t == 0) {
+            inv[0] = fa.inv_dr; inv[1] = fa.inv_du;
+            inv[2] = fb.inv_dr; inv[3] = fb.inv_du;
+        }
     }
     __syncthreads();
+    const FastSrc fa = make_fast_src_inv(arena + offsets[l], ga, li[2], li[3], li[3], inv[0], inv[1]);
+    const FastSrc fb = make_fast_src_inv(arena + offsets[r], gb, ri[2], ri[3], ri[3], inv[2], inv[3]);
     int ira[4], iua[4], irb[4], iub[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -702,21 +744,22 @@ __global__ __launch_bounds__(MT_THREADS) void batched_merge_tiled_kernel(
     }
     const Window wa = corner_window<INTERP>(ira, iua, fa.nr, fa.nu);
     const Window wb = corner_window<INTERP>(irb, iub, fb.nr, fb.nu);
-    if (wa.lds) stage_window(fa, wa, sh[0], wave, lane);
-    if (wb.lds) stage_window(fb, wb, sh[1], wave, lane);
+    if (wa.lds) stage_window(fa, wa, sha, wave, lane);
+    if (wb.lds) stage_window(fb, wb, shb, wave, lane);
     __syncthreads();
     if (col >= nu) return;
     float2* out = arena + offsets[node];
+    const DestColumn dc = dest_column(g, side, col);
 #pragma unroll 1
     for (int j = 0; j < MT_PPT; ++j) {
         const int row = row0 + j * ROW_STEP;
         if (row >= nr) break;
         double x, y, rr;
-        dest_pixel_rc(g, side, row, col, &x, &y, &rr);
+        dest_pixel_col(g, dc, row, &x, &y, &rr);
         const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
         const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-        const float2 va = tile_sample<INTERP>(fa, ca, wa, sh[0]);
-        const float2 vb = tile_sample<INTERP>(fb, cb, wb, sh[1]);
+        const float2 va = tile_sample<INTERP>(fa, ca, wa, sha);
+        const float2 vb = tile_sample<INTERP>(fb, cb, wb, shb);
         out[(long long)row * nu + col] = make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y));
     }
 }

@@ -1,0 +1,71 @@
+"""One C2-sized FFBP for hardware-counter collection.
+
+    python -m bp_project_amd.profile_merges [--pulses 8192] [--no-untiled]
+
+Runs the split-precision FFBP of the 8192-pulse x 8192-bin stripmap workload
+(leaf 16, factor 2, 8-tap Kaiser-sinc merges) once through the level-batched
+program (LDS-tiled merges), then every merge level again through the
+untiled merge kernel on the same node images, then one 1 GiB copy kernel
+(`bpmi_crop` of a 16384 x 8192 complex64 image: 1 GiB read, 1 GiB written)
+whose counters calibrate FETCH_SIZE / WRITE_SIZE.  Meant
+to run under `rocprofv3 --pmc ...`; prints one line per phase.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from .batched_resident_backend import BatchedResidentProgram, merge_block_table
+from .device import DeviceArray, synchronize
+from .gpu_backend import HipBaseline, build_ffbp_plan
+from .scenes import polar_extent_for_grid, workload_scene
+
+_vp = ctypes.c_void_p
+COPY_BYTES = 1 << 30
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pulses", type=int, default=8192)
+    ap.add_argument("--no-untiled", action="store_true")
+    a = ap.parse_args(argv)
+    t = time.perf_counter()
+    s = workload_scene("c2_ffbp_full", pulses=a.pulses, bins=a.pulses)
+    extent = polar_extent_for_grid(s.grid, s.pos.mean(axis=0), margin=2.0)
+    plan = build_ffbp_plan(s.pos, s.fc, extent, side=1, leaf_size=16, range_spacing=0.25,
+                           angular_oversample=2.5)
+    backend = HipBaseline(interp="lagrange4", merge_interp="kaiser8")
+    dev = backend.upload(s.data, s.pos, s.weight, s.phase, plan)
+    prog = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k)
+    print(f"setup {time.perf_counter() - t:.1f} s", flush=True)
+    prog.execute()
+    synchronize()
+    print("tiled FFBP done", flush=True)
+    if not a.no_untiled:
+        flat = prog.flat
+        for height, nodes in enumerate(flat.levels):
+            if height == 0:
+                continue
+            table = DeviceArray.from_host(
+                np.asarray(merge_block_table(flat.node_int, nodes, "fp64"), np.int32))
+            _lib.call("bpmi_ffbp_merges", _vp(prog._node_table.ptr),
+                      _vp(prog._geometry_table.ptr), _vp(prog._offset_table.ptr),
+                      _vp(prog.arena.ptr), _vp(table.ptr), table.shape[0], s.k,
+                      _lib.MODE_SPLIT, _lib.interp_code("kaiser8"), None)
+            synchronize()
+        print("untiled merges done", flush=True)
+    rows, cols = COPY_BYTES // (8 * 8192), 8192
+    src, dst = DeviceArray.empty((rows, cols)), DeviceArray.empty((rows, cols))
+    src.fill_zero()
+    _lib.call("bpmi_crop", _vp(src.ptr), cols, 0, 0, rows, cols, _vp(dst.ptr), None)
+    synchronize()
+    print(f"calibration copy of {COPY_BYTES} bytes done", flush=True)
+    prog.close()
+
+
+if __name__ == "__main__":
+    main()
