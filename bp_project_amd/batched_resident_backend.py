@@ -41,12 +41,32 @@ def merge_tiles(nr, nu):
     return t.value
 
 
+XCDS = 8  # MI355X: workgroup i of a launch runs on accelerator die i % 8
+
+
+def xcd_order(entries, xcds=XCDS, pad=(-1, 0)):
+    """Reorder a block table so that the hardware's round-robin placement of
+    workgroups on dies (workgroup i -> die i % xcds) gives each die one
+    contiguous run of the original table: die x executes entries
+    [x * n / xcds, (x + 1) * n / xcds) in order, so neighbouring tiles, whose
+    child windows overlap, share that die's L2.  Padding entries (node -1)
+    fill the last row; the kernel skips them."""
+    n = len(entries)
+    per = -(-n // xcds)
+    out = []
+    for i in range(per):
+        for x in range(xcds):
+            j = x * per + i
+            out.append(entries[j] if j < n else pad)
+    return out
+
+
 def merge_block_table(node_int, nodes, mode):
-    """(node, tile) pairs for split mode (tiled merges), else (node, first
-    pixel) pairs of MERGE_BLOCK pixels."""
+    """(node, tile) pairs for split mode (tiled merges, in die order), else
+    (node, first pixel) pairs of MERGE_BLOCK pixels."""
     if mode == "split":
-        return [(int(n), t) for n in nodes
-                for t in range(merge_tiles(node_int[n, NI_NR], node_int[n, NI_NU]))]
+        return xcd_order([(int(n), t) for n in nodes
+                          for t in range(merge_tiles(node_int[n, NI_NR], node_int[n, NI_NU]))])
     return [(int(n), s) for n in nodes
             for s in range(0, int(node_int[n, NI_NR]) * int(node_int[n, NI_NU]), MERGE_BLOCK)]
 

@@ -68,6 +68,19 @@ __device__ __forceinline__ double rsqrt64(double x) {
     return fma(y, e, y);
 }
 
+// 1/x: hardware estimate (v_rcp_f64, relative error <= 4.6e-8) and two Newton
+// steps (below 1 ulp in practice; not IEEE-rounded).
+__device__ __forceinline__ double rcp64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// sqrt(max(x, 0)) from the reciprocal square root (0 for x <= 0).
+__device__ __forceinline__ double sqrt64_fast(double x) { return x > 0.0 ? x * rsqrt64(x) : 0.0; }
+
 // Reference association order of cubic_weights (fast_bp/ffbp.py:17-21).
 // FP64: the reference's divisions; FP32: multiplication by the rounded
 // reciprocal (a 1-ulp weight difference, no IEEE division sequence).

@@ -334,10 +334,11 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Pixel index of slot k of thread t in a tile.
+// Pixel index of slot k of thread t in a tile (row/col of grid sources).
 template <int SRC>
 __device__ __forceinline__ long long split_pixel(const SrcArgs& s, int tile, int tc_log2, int t,
-                                                 int k, bool* valid) {
+                                                 int k, bool* valid, int* prow = nullptr,
+                                                 int* pcol = nullptr) {
     if (SRC == SRC_POINTS) {
         const long long p = (long long)tile * SP_TILE + t + k * SP_THREADS;
         *valid = p < s.count;
@@ -352,7 +353,43 @@ __device__ __forceinline__ long long split_pixel(const SrcArgs& s, int tile, int
     const int col = (tcol << tc_log2) + (t & (tc - 1));
     const int row = trow * tr + (t >> tc_log2) + k * rows_per_k;
     *valid = col < s.cols && row < s.rows;
+    if (prow) {
+        *prow = row;
+        *pcol = col;
+    }
     return (long long)row * s.cols + col;
+}
+
+// First and last row of a split tile of a grid source (clipped to the grid).
+__device__ __forceinline__ void split_tile_rows(const SrcArgs& s, int tile, int tc_log2, int* lo,
+                                                int* hi) {
+    const int tiles_c = (s.cols + (1 << tc_log2) - 1) >> tc_log2;
+    const int tr = SP_TILE >> tc_log2;
+    *lo = (tile / tiles_c) * tr;
+    *hi = min(*lo + tr - 1, s.rows - 1);
+}
+
+// FP64 coordinates of the pixel at (row, col) of a grid source, square roots
+// from rsqrt64 (relative error <= 4.2e-15); slot pixels of the split kernel.
+template <int SRC>
+__device__ __forceinline__ void split_pixel64(const SrcArgs& s, long long p, int row, int col,
+                                              double* x, double* y, double* z, double* r_out) {
+    if (SRC == SRC_POLAR) {
+        const double r = s.b0 + (double)row * s.b1;
+        const double u = s.b2 + (double)col * s.b3;
+        const double ground = sqrt64_fast(r * r - s.a2 * s.a2);
+        *x = s.a0 + (double)s.side * ground * sqrt64_fast(1.0 - u * u);
+        *y = s.a1 + ground * u;
+        *z = 0.0;
+        *r_out = r;
+    } else if (SRC == SRC_GRID) {
+        *x = s.b0 + (double)col * s.b1;
+        *y = s.b2 + (double)row * s.b3;
+        *z = s.a2;
+        *r_out = 0.0;
+    } else {
+        src_pixel64<SRC>(s, p, x, y, z, r_out);
+    }
 }
 
 template <int INTERP, int SRC, bool ORIGIN_REF>
@@ -377,6 +414,7 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
     const double kt = a.k * INV_TWO_PI;
     const float kt32 = (float)kt;
     const float invdr = (float)(1.0 / a.data_dr);
+    const double inv_ddr = 1.0 / a.data_dr;
 
     // ---- pixel setup -------------------------------------------------------
     long long pix[SP_PPT];
@@ -386,16 +424,33 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
     double rmin = 1e300, rmax = -1e300;
 #pragma unroll
     for (int k = 0; k < SP_PPT; ++k) {
-        pix[k] = split_pixel<SRC>(s, tile, tc_log2, t, k, &val[k]);
-        src_pixel64<SRC>(s, val[k] ? pix[k] : 0, &px[k], &py[k], &pz[k], &pr[k]);
-        if (val[k]) {
+        int row = 0, col = 0;
+        pix[k] = split_pixel<SRC>(s, tile, tc_log2, t, k, &val[k], &row, &col);
+        if (!val[k]) row = col = 0;
+        split_pixel64<SRC>(s, val[k] ? pix[k] : 0, row, col, &px[k], &py[k], &pz[k], &pr[k]);
+        if (!ORIGIN_REF && val[k]) {
             lo_x = fmin(lo_x, px[k]); hi_x = fmax(hi_x, px[k]);
             lo_y = fmin(lo_y, py[k]); hi_y = fmax(hi_y, py[k]);
             lo_z = fmin(lo_z, pz[k]); hi_z = fmax(hi_z, pz[k]);
             rmin = fmin(rmin, pr[k]); rmax = fmax(rmax, pr[k]);
         }
     }
-    {
+    if (ORIGIN_REF) {
+        // The node origin is the reference; the tile's slant-range span is
+        // that of its first and last rows (every row holds a valid pixel).
+        if (t == 0) {
+            int lo, hi;
+            split_tile_rows(s, tile, tc_log2, &lo, &hi);
+            const double r_lo = s.b0 + (double)lo * s.b1, r_hi = s.b0 + (double)hi * s.b1;
+            ref.cx = s.a0; ref.cy = s.a1; ref.cz = s.a2;
+            ref.hx = ref.hy = ref.hz = ref.rho = 0.0;
+            ref.rmin = fmin(r_lo, r_hi);
+            ref.rmax = fmax(r_lo, r_hi);
+            const double it = floor((ref.rmin - a.range_start) / a.data_dr);
+            ref.itile = (it > -1.0e9 && it < 1.0e9) ? (int)it : 0;
+        }
+        __syncthreads();
+    } else {
         double v[8] = {lo_x, lo_y, lo_z, -hi_x, -hi_y, -hi_z, rmin, -rmax};
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = wave_min(v[j]);
@@ -409,13 +464,9 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
 #pragma unroll
             for (int j = 0; j < 8; ++j)
                 m[j] = fmin(fmin(red[0][j], red[1][j]), fmin(red[2][j], red[3][j]));
-            if (ORIGIN_REF) {
-                ref.cx = s.a0; ref.cy = s.a1; ref.cz = s.a2;
-            } else {
-                ref.cx = 0.5 * (m[0] - m[3]);
-                ref.cy = 0.5 * (m[1] - m[4]);
-                ref.cz = 0.5 * (m[2] - m[5]);
-            }
+            ref.cx = 0.5 * (m[0] - m[3]);
+            ref.cy = 0.5 * (m[1] - m[4]);
+            ref.cz = 0.5 * (m[2] - m[5]);
             ref.hx = 0.5 * (-m[3] - m[0]);
             ref.hy = 0.5 * (-m[4] - m[1]);
             ref.hz = 0.5 * (-m[5] - m[2]);
@@ -438,12 +489,12 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
         if (ORIGIN_REF) {
             // unit vector from the node origin to the pixel; |p - o| = r
             const double r = pr[k] > 0.0 ? pr[k] : 1.0;
-            const double ir = 1.0 / r;
+            const double ir = rcp64(r);
             dxk[k] = (px[k] - cx) * ir;
             dyk[k] = (py[k] - cy) * ir;
             dzk[k] = (pz[k] - cz) * ir;
             ek[k] = 0.f;
-            qpk[k] = (float)((pr[k] - a.range_start) / a.data_dr - (double)ref.itile);
+            qpk[k] = (float)((pr[k] - a.range_start) * inv_ddr - (double)ref.itile);
             sck[k] = (float)(0.5 * ir);
             irk[k] = (float)ir;
         } else {

@@ -343,12 +343,13 @@ __global__ __launch_bounds__(256) void contrib_kernel(const float2* __restrict__
 constexpr int KPOLY_EVEN = 5, KPOLY_ODD = 4, KPOLY_STRIDE = KPOLY_EVEN + KPOLY_ODD;
 __constant__ float c_kaiser_poly[4][KPOLY_STRIDE];
 
-__device__ __forceinline__ void kaiser_weights_poly(float t, float w[8]) {
+// kp: the coefficient table (c_kaiser_poly, or a workgroup's LDS copy of it).
+__device__ __forceinline__ void kaiser_weights_poly(const float* kp, float t, float w[8]) {
     const float s = 2.f * t - 1.f;
     const float x = s * s;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        const float* c = c_kaiser_poly[a];
+        const float* c = kp + a * KPOLY_STRIDE;
         float e = c[KPOLY_EVEN - 1];
 #pragma unroll
         for (int j = KPOLY_EVEN - 2; j >= 0; --j) e = fmaf(e, x, c[j]);
@@ -360,13 +361,6 @@ __device__ __forceinline__ void kaiser_weights_poly(float t, float w[8]) {
     }
 }
 
-__device__ __forceinline__ double rcp64(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
-    return fma(y, e, y);
-}
 
 struct FastSrc {
     const float2* img;
@@ -437,12 +431,13 @@ __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double
 // (a, 0) of the stencil (row ir - FIRST + a, column iu - FIRST), in global
 // memory or in an LDS copy: the arithmetic is identical either way.
 template <int INTERP, typename Row>
-__device__ __forceinline__ float2 stencil(const ChildTap& c, Row row) {
+__device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
+                                          const float* kp = &c_kaiser_poly[0][0]) {
     constexpr int TAPS = FastShape<INTERP>::TAPS;
     float wr[TAPS], wu[TAPS];
     if (INTERP == INTERP_KAISER8) {
-        kaiser_weights_poly(c.tr, wr);
-        kaiser_weights_poly(c.tu, wu);
+        kaiser_weights_poly(kp, c.tr, wr);
+        kaiser_weights_poly(kp, c.tu, wu);
     } else {
         cubic_weights(c.tr, wr);
         cubic_weights(c.tu, wu);
@@ -465,11 +460,12 @@ __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row) {
 }
 
 template <int INTERP>
-__device__ __forceinline__ float2 stencil_global(const FastSrc& f, const ChildTap& c) {
+__device__ __forceinline__ float2 stencil_global(const FastSrc& f, const ChildTap& c,
+                                                 const float* kp = &c_kaiser_poly[0][0]) {
     constexpr int FIRST = FastShape<INTERP>::FIRST;
     const float2* base = f.img + (long long)(c.ir - FIRST) * f.ld + (c.iu - FIRST);
     const long long ld = f.ld;
-    return stencil<INTERP>(c, [&](int a) { return base + a * ld; });
+    return stencil<INTERP>(c, [&](int a) { return base + a * ld; }, kp);
 }
 
 // Destination polar pixel p of node geometry g (fast_bp/ffbp.py:62-67).
@@ -670,15 +666,15 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
 
 template <int INTERP>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
-                                              const float2* sh) {
+                                              const float2* sh, const float* kp) {
     constexpr int FIRST = FastShape<INTERP>::FIRST;
     if (!c.ok) return make_float2(0.f, 0.f);
     if (in_window<INTERP>(c, w)) {
         const float2* base = sh + (c.ir - FIRST - w.r0) * w.w + (c.iu - FIRST - w.c0);
         const int pitch = w.w;
-        return stencil<INTERP>(c, [&](int a) { return base + a * pitch; });
+        return stencil<INTERP>(c, [&](int a) { return base + a * pitch; }, kp);
     }
-    return stencil_global<INTERP>(f, c);
+    return stencil_global<INTERP>(f, c, kp);
 }
 
 template <int INTERP>
@@ -689,6 +685,10 @@ __global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
     __shared__ float2 sh[2 * MT_CAP];
     __shared__ int corner[4][4];  // corner j: ir_a, iu_a, ir_b, iu_b
     __shared__ double inv[4];     // 1/dr, 1/du of both children
+    // LDS copy of the Kaiser weight polynomials: read as vector operands, they
+    // free the 36 scalar registers whose pressure otherwise spills scalars
+    // into vector lanes inside the pixel loop.
+    __shared__ float kpoly[4 * KPOLY_STRIDE];
     // The second window's base offset is opaque to the compiler, so the
     // stencil reads address it from a register rather than through
     // per-instruction offset additions (ds_read2 offsets reach 2 KiB).
@@ -699,6 +699,7 @@ __global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
+    if (node < 0) return;  // padding entry of a die-ordered table (uniform)
     const int* ni = node_int + 8 * node;
     const int nr = ni[2], nu = ni[3], side = ni[4], l = ni[5], r = ni[6];
     const int tiles_u = (nu + MT_COLS - 1) / MT_COLS;
@@ -732,6 +733,7 @@ __global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
             inv[0] = fa.inv_dr; inv[1] = fa.inv_du;
             inv[2] = fb.inv_dr; inv[3] = fb.inv_du;
         }
+        if (t < 4 * KPOLY_STRIDE) kpoly[t] = (&c_kaiser_poly[0][0])[t];
     }
     __syncthreads();
     const FastSrc fa = make_fast_src_inv(arena + offsets[l], ga, li[2], li[3], li[3], inv[0], inv[1]);
@@ -758,8 +760,8 @@ __global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
         dest_pixel_col(g, dc, row, &x, &y, &rr);
         const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
         const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-        const float2 va = tile_sample<INTERP>(fa, ca, wa, sha);
-        const float2 vb = tile_sample<INTERP>(fb, cb, wb, shb);
+        const float2 va = tile_sample<INTERP>(fa, ca, wa, sha, kpoly);
+        const float2 vb = tile_sample<INTERP>(fb, cb, wb, shb, kpoly);
         out[(long long)row * nu + col] = make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y));
     }
 }

@@ -212,7 +212,9 @@ int bpmi_split_tiles(int nr, int nu, int* tiles);
 /* Split-mode merges of one subtree height with 2-D tiles
  * (batched_resident_backend.py:72-150 merge stage, LDS-tiled).  ``blocks``
  * int32 [nblocks, 2] = (node, tile) pairs, bpmi_merge_tiles() tiles of
- * 32 r-rows x 32 u-columns per parent node; tables as for bpmi_ffbp_merges.
+ * 32 r-rows x 32 u-columns per parent node (entries with node -1 are
+ * skipped, e.g. padding of a table ordered for the 8 dies); tables as for
+ * bpmi_ffbp_merges.
  * Bitwise equal to bpmi_ffbp_merges in BPMI_MODE_SPLIT. */
 int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo,
                            const long long* offsets, void* arena, const int* blocks,
