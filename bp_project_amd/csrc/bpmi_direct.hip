@@ -298,7 +298,10 @@ constexpr int SP_TILE = SP_THREADS * SP_PPT;
 constexpr int SP_NB = 16;
 constexpr int SP_W = 32;
 
-enum { PF_NORMAL = 0, PF_SKIP = 1, PF_SLOW = 2 };
+// Per-pulse path of a tile: staged polynomial window with the fourth- or the
+// second-order range residual, skipped (window outside the data), or exact
+// FP64 geometry.
+enum { PF_FAST4 = 0, PF_FAST2 = 1, PF_SKIP = 2, PF_SLOW = 3 };
 
 struct PulseParams {
     double vx, vy, vz;  // tile reference: unit vector (c-a)/D; origin reference: o - a
@@ -523,7 +526,7 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
             const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
             q.w = (float)pulse_weight(a.weight, n);
             q.pulse = n;
-            double qlo, qhi, err;
+            double qlo, qhi, err, err2 = 1e30;
             bool near;
             if (ORIGIN_REF) {
                 q.vx = ux; q.vy = uy; q.vz = uz;
@@ -559,8 +562,11 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                            : r2 * r2 * r2 / (16.0 * dm * dm * dm * dm * dm) +
                                  r2 * 0.5 * inv * mm * mm * mm / om +
                                  r2 * r2 * 0.125 * inv * inv * inv * 3.0 * mm / (om * om * om * om);
+                // second-order residual hs (1 - m): sqrt tail h^2 / (8 L^3) and
+                // the 1/(1+m) tail after m
+                err2 = near ? 1e30 : r2 * r2 / (8.0 * dm * dm * dm) + r2 * 0.5 * inv * mm * mm / om;
             }
-            int flag = PF_NORMAL;
+            int flag = PF_FAST4;
             int ilo = 0;
             if (!(qlo > -1.0e9 && qhi < 1.0e9)) {
                 flag = PF_SLOW;
@@ -569,8 +575,9 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                 const int ihi = (int)floor(qhi) + 1;
                 if (ihi < vlo || ilo > vhi) flag = PF_SKIP;
                 else if (ihi - ilo + 1 > SP_W || near || fabs(kt) * err > 1e-6) flag = PF_SLOW;
+                else if (ORIGIN_REF || fabs(kt) * err2 <= 1e-6) flag = PF_FAST2;
             }
-            if (flag == PF_NORMAL) {
+            if (flag <= PF_FAST2) {
                 if (ORIGIN_REF) q.qb = (float)(ref.itile - ilo);
                 else q.qb = (float)((D - a.range_start) / a.data_dr - (double)ilo);
             } else {
@@ -587,7 +594,7 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
             const PulseParams& q = prm[b];
             float4 c01 = make_float4(0.f, 0.f, 0.f, 0.f), c23 = c01;
             const int i = q.ilo + j;
-            if (q.flag != PF_NORMAL) continue;
+            if (q.flag > PF_FAST2) continue;
             if (i >= vlo && i <= vhi) {
                 const float2* row = a.data + (size_t)q.pulse * S;
                 const float w = q.w;
@@ -612,89 +619,107 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
         }
         __syncthreads();
         // ---- accumulate ------------------------------------------------------
-        for (int b = 0; b < nb; ++b) {
-            const PulseParams q = prm[b];
-            if (q.flag == PF_SKIP) continue;
-            if (q.flag == PF_SLOW) {
-                const int n = q.pulse;
-                const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
-                const float2* row = a.data + (size_t)n * S;
-                const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
+        // One pass per path (staged window with the second- or fourth-order
+        // residual, exact geometry) over that path's pulses, found from
+        // wave-wide ballots of the pulse flags: the loops run on scalar bit
+        // masks and the accumulators stay in place across pulses.
+        const int my_flag = lane < nb ? prm[lane].flag : PF_SKIP;
+        const unsigned long long pulses2 = __ballot(my_flag == PF_FAST2);
+        const unsigned long long pulses4 = __ballot(my_flag == PF_FAST4);
+        const unsigned long long pulses_slow = __ballot(my_flag == PF_SLOW);
+        auto fast_pass = [&]<bool ORDER4>() {
+            for (unsigned long long m = ORDER4 ? pulses4 : pulses2; m; m &= m - 1) {
+                const int b = __builtin_ctzll(m);
+                const PulseParams q = prm[b];
+                const float4* cb = coef + b * SP_W * F4;
 #pragma unroll
                 for (int k = 0; k < SP_PPT; ++k) {
-                    double ex, ey, ez;
+                    // linear range term in FP64 and its phase reduced modulo one turn
+                    const double lin = dxk[k] * q.vx + dyk[k] * q.vy + dzk[k] * q.vz;
+                    const double T = hw_fract64(q.P + lin * kt);
+                    const float l32 = (float)lin;
+                    // Range residual d - (R + lin) = sqrt((R + lin)^2 + h) - (R + lin)
+                    // with h = Q - lin^2, m = lin / R and hs = h / (2R): hs (1 - m),
+                    // plus the fourth-order terms hs (m^2 - h / (4R^2)) where the
+                    // tile's bound needs them.
+                    float ql, corr;
                     if (ORIGIN_REF) {
-                        // p - a = r * unit(p - o) + (o - a)
-                        ex = pr[k] * dxk[k] + q.vx;
-                        ey = pr[k] * dyk[k] + q.vy;
-                        ez = pr[k] * dzk[k] + q.vz;
+                        const float h = q.q2 - l32 * l32;
+                        const float m = l32 * irk[k];
+                        const float hs = h * sck[k];
+                        corr = hs - hs * m;
+                    } else if (ORDER4) {
+                        const float h = ek[k] - l32 * l32;
+                        const float m = l32 * q.invr;
+                        const float hs = h * q.scale;
+                        float p = fmaf(m, m - 1.f, 1.f);
+                        p = fmaf(-hs, q.scale, p);
+                        corr = hs * p;
                     } else {
-                        ex = (cx - ax) + dxk[k];
-                        ey = (cy - ay) + dyk[k];
-                        ez = (cz - az) + dzk[k];
+                        const float h = ek[k] - l32 * l32;
+                        const float m = l32 * q.invr;
+                        const float hs = h * q.scale;
+                        corr = hs - hs * m;
                     }
-                    const double d = sqrt(ex * ex + ey * ey + ez * ez);
-                    const double qq = (d - a.range_start) / a.data_dr;
-                    int i;
-                    if (!floor_index(qq, &i) || i < vlo || i > vhi) continue;
-                    const float2 v = sample1_f32w<INTERP>(row, S, i, (float)(qq - (double)i), nullptr);
-                    const double turns = kt * (ORIGIN_REF ? d - pr[k] : d) + ph;
+                    if (ORIGIN_REF)
+                        ql = (qpk[k] + q.qb) + (l32 + corr) * invdr;
+                    else
+                        ql = q.qb + (l32 + corr) * invdr;
+                    const float turns = (float)T + corr * kt32;
+                    int j = (int)ql;
+                    const float tt = hw_fract32(ql);
+                    j = min(max(j, 0), SP_W - 1);
+                    float2 v;
+                    if (INTERP == INTERP_LAGRANGE4) {
+                        const float4 lo = cb[2 * j], hi = cb[2 * j + 1];
+                        v.x = ((hi.z * tt + hi.x) * tt + lo.z) * tt + lo.x;
+                        v.y = ((hi.w * tt + hi.y) * tt + lo.w) * tt + lo.y;
+                    } else {
+                        const float4 lo = cb[j];
+                        v.x = lo.z * tt + lo.x;
+                        v.y = lo.w * tt + lo.y;
+                    }
                     float sn, cs;
-                    sincos_turns((float)fract64(turns), &sn, &cs);
-                    acc[k].x += q.w * (v.x * cs - v.y * sn);
-                    acc[k].y += q.w * (v.x * sn + v.y * cs);
+                    sincos_turns(turns, &sn, &cs);
+                    acc[k].x = fmaf(v.x, cs, acc[k].x);
+                    acc[k].x = fmaf(-v.y, sn, acc[k].x);
+                    acc[k].y = fmaf(v.x, sn, acc[k].y);
+                    acc[k].y = fmaf(v.y, cs, acc[k].y);
                 }
-                continue;
             }
-            const float4* cb = coef + b * SP_W * F4;
+        };
+        fast_pass.template operator()<false>();
+        if (!ORIGIN_REF) fast_pass.template operator()<true>();
+        for (unsigned long long m = pulses_slow; m; m &= m - 1) {
+            const int b = __builtin_ctzll(m);
+            const PulseParams q = prm[b];
+            const int n = q.pulse;
+            const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
+            const float2* row = a.data + (size_t)n * S;
+            const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
 #pragma unroll
             for (int k = 0; k < SP_PPT; ++k) {
-                // linear range term in FP64 and its phase reduced modulo one turn
-                const double lin = dxk[k] * q.vx + dyk[k] * q.vy + dzk[k] * q.vz;
-                const double T = hw_fract64(q.P + lin * kt);
-                const float l32 = (float)lin;
-                // Range residual d - (R + lin) = sqrt((R + lin)^2 + h) - (R + lin)
-                // with h = Q - lin^2, m = lin / R and hs = h / (2R).  Leaves
-                // (antenna within a few metres of the origin) need hs (1 - m);
-                // tiles add the fourth-order terms hs (m^2 - h / (4R^2)).
-                float ql, corr;
+                double ex, ey, ez;
                 if (ORIGIN_REF) {
-                    const float h = q.q2 - l32 * l32;
-                    const float m = l32 * irk[k];
-                    const float hs = h * sck[k];
-                    corr = hs - hs * m;
+                    // p - a = r * unit(p - o) + (o - a)
+                    ex = pr[k] * dxk[k] + q.vx;
+                    ey = pr[k] * dyk[k] + q.vy;
+                    ez = pr[k] * dzk[k] + q.vz;
                 } else {
-                    const float h = ek[k] - l32 * l32;
-                    const float m = l32 * q.invr;
-                    const float hs = h * q.scale;
-                    float p = fmaf(m, m - 1.f, 1.f);
-                    p = fmaf(-hs, q.scale, p);
-                    corr = hs * p;
+                    ex = (cx - ax) + dxk[k];
+                    ey = (cy - ay) + dyk[k];
+                    ez = (cz - az) + dzk[k];
                 }
-                if (ORIGIN_REF)
-                    ql = (qpk[k] + q.qb) + (l32 + corr) * invdr;
-                else
-                    ql = q.qb + (l32 + corr) * invdr;
-                const float turns = (float)T + corr * kt32;
-                int j = (int)ql;
-                const float tt = hw_fract32(ql);
-                j = min(max(j, 0), SP_W - 1);
-                float2 v;
-                if (INTERP == INTERP_LAGRANGE4) {
-                    const float4 lo = cb[2 * j], hi = cb[2 * j + 1];
-                    v.x = ((hi.z * tt + hi.x) * tt + lo.z) * tt + lo.x;
-                    v.y = ((hi.w * tt + hi.y) * tt + lo.w) * tt + lo.y;
-                } else {
-                    const float4 lo = cb[j];
-                    v.x = lo.z * tt + lo.x;
-                    v.y = lo.w * tt + lo.y;
-                }
+                const double d = sqrt(ex * ex + ey * ey + ez * ez);
+                const double qq = (d - a.range_start) / a.data_dr;
+                int i;
+                if (!floor_index(qq, &i) || i < vlo || i > vhi) continue;
+                const float2 v = sample1_f32w<INTERP>(row, S, i, (float)(qq - (double)i), nullptr);
+                const double turns = kt * (ORIGIN_REF ? d - pr[k] : d) + ph;
                 float sn, cs;
-                sincos_turns(turns, &sn, &cs);
-                acc[k].x = fmaf(v.x, cs, acc[k].x);
-                acc[k].x = fmaf(-v.y, sn, acc[k].x);
-                acc[k].y = fmaf(v.x, sn, acc[k].y);
-                acc[k].y = fmaf(v.y, cs, acc[k].y);
+                sincos_turns((float)fract64(turns), &sn, &cs);
+                acc[k].x += q.w * (v.x * cs - v.y * sn);
+                acc[k].y += q.w * (v.x * sn + v.y * cs);
             }
         }
         __syncthreads();
