@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void batched_leaf_kernel(BpArgs a, LeafTables 
 constexpr int SP_THREADS = 256;
 constexpr int SP_PPT = 4;
 constexpr int SP_TILE = SP_THREADS * SP_PPT;
-constexpr int SP_NB = 16;
+constexpr int SP_NB = 32;
 constexpr int SP_W = 32;
 
 // Per-pulse path of a tile: staged polynomial window with the fourth- or the
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
             PulseParams q;
             const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
             const double ux = cx - ax, uy = cy - ay, uz = cz - az;
-            const double D = sqrt(ux * ux + uy * uy + uz * uz);
+            const double D = sqrt64_fast(ux * ux + uy * uy + uz * uz);
             const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
             q.w = (float)pulse_weight(a.weight, n);
             q.pulse = n;
@@ -534,14 +534,14 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                 q.q2 = (float)(D * D);
                 q.scale = 0.f;
                 q.invr = 0.f;
-                qlo = (ref.rmin - D - a.range_start) / a.data_dr;
-                qhi = (ref.rmax + D - a.range_start) / a.data_dr;
+                qlo = (ref.rmin - D - a.range_start) * inv_ddr;
+                qhi = (ref.rmax + D - a.range_start) * inv_ddr;
                 const double rm = ref.rmin - D;
                 near = rm <= D || ref.rmin <= 0.0;
                 const double mm = D / fmax(ref.rmin, 1e-30);
                 err = near ? 1e30 : D * D * D * D / (8.0 * rm * rm * rm) + mm * mm * D * D / (2.0 * ref.rmin * (1.0 - mm));
             } else {
-                const double inv = 1.0 / D;
+                const double inv = rcp64(D);
                 q.vx = ux * inv; q.vy = uy * inv; q.vz = uz * inv;
                 q.P = fract64(kt * D + ph);
                 q.q2 = 0.f;
@@ -551,8 +551,8 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
                 const double rho = ref.rho;
                 near = D <= 2.0 * rho;
                 const double extra = near ? 0.0 : rho * rho / (2.0 * (D - rho));
-                qlo = (D - B - 1e-6 - a.range_start) / a.data_dr;
-                qhi = (D + B + extra + 1e-6 - a.range_start) / a.data_dr;
+                qlo = (D - B - 1e-6 - a.range_start) * inv_ddr;
+                qhi = (D + B + extra + 1e-6 - a.range_start) * inv_ddr;
                 // Truncation bound of the fourth-order range expansion used below:
                 // sqrt tail h^3 / (16 L^5), 1/(1+m) tail after m^2, and the
                 // (1+m)^-3 factor dropped from the h^2 term.
@@ -635,7 +635,10 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
 #pragma unroll
                 for (int k = 0; k < SP_PPT; ++k) {
                     // linear range term in FP64 and its phase reduced modulo one turn
-                    const double lin = dxk[k] * q.vx + dyk[k] * q.vy + dzk[k] * q.vz;
+                    // (grid tiles lie in one z plane through the tile centre:
+                    // their z offset is exactly zero)
+                    const double lin = SRC == SRC_GRID ? dxk[k] * q.vx + dyk[k] * q.vy
+                                                       : dxk[k] * q.vx + dyk[k] * q.vy + dzk[k] * q.vz;
                     const double T = hw_fract64(q.P + lin * kt);
                     const float l32 = (float)lin;
                     // Range residual d - (R + lin) = sqrt((R + lin)^2 + h) - (R + lin)
