@@ -11,7 +11,9 @@ stored per pixel and pulse).  PyTorch provides only the autograd plumbing,
 the entropy expression and the optimiser; the image and its adjoint run in
 libbpmi on the current HIP stream.
 
-``pga_autofocus`` keeps the reference's non-gradient caller for comparison.
+``pick_targets`` / ``estimate_update`` keep the reference's non-gradient
+phase-gradient caller (fast_bp/autofocus.py:20-79) for comparison, with the
+per-(target, pulse) contributions computed on the GPU.
 """
 from __future__ import annotations
 
@@ -235,17 +237,106 @@ def finite_difference_gradient(op, phase, pos, pulses, h=1e-4, mode="fp64"):
         op.mode = mode_saved
 
 
-def pga_autofocus(data, pos, r_start, dr, k, targets, weight=None):
-    """Phase-gradient estimate of per-pulse phase errors from bright targets,
-    the non-gradient caller of fast_bp/autofocus.py:37-79 reduced to its core:
-    per-target contributions (GPU, ``target_contributions``), adjacent-pulse
-    cross-product phase gradients averaged over targets, integrated.
-    Returns the estimated phase error per pulse (radians)."""
+def pick_targets(result, nr_tiles=12, nu_tiles=12):
+    """Brightest polar pixel of each of nr_tiles x nu_tiles blocks of an FFBP
+    root image, as ground points on z = 0, keeping those within 35 dB of the
+    brightest (fast_bp/autofocus.py:20-34).  ``result`` is a ``PolarImage``.
+    Returns (points [M, 3], powers [M])."""
+    img = np.asarray(result.image)
+    r_axis = np.asarray(result.r)
+    u_axis = np.asarray(result.u)
+    o = np.asarray(result.origin, np.float64)
+    points, powers = [], []
+    for rows in np.array_split(np.arange(len(r_axis)), nr_tiles):
+        for cols in np.array_split(np.arange(len(u_axis)), nu_tiles):
+            block = np.abs(img[rows[0]:rows[-1] + 1, cols[0]:cols[-1] + 1])
+            i, j = np.unravel_index(block.argmax(), block.shape)
+            r = r_axis[rows[i]]
+            u = u_axis[cols[j]]
+            ground = np.sqrt(r * r - o[2] ** 2)
+            points.append([o[0] + result.side * ground * np.sqrt(1 - u * u), o[1] + ground * u, 0.])
+            powers.append(float(block.max()) ** 2)
+    powers = np.array(powers)
+    points = np.array(points)
+    keep = powers > powers.max() * 10 ** (-35 / 10)
+    return points[keep], powers[keep]
+
+
+def phase_gradient_update(contributions, pos, points, k, prf, cutoff):
+    """Trajectory update from per-(target, pulse) contributions: the host
+    half of fast_bp/autofocus.py:37-79.
+
+    contributions: [M, P] complex (``target_contributions``); pos [P, 3];
+    points [M, 3].  Steps, in the reference's order:
+
+    1. zero-phase 4th-order Butterworth low-pass (cutoff Hz at the PRF) of
+       each target's slow-time history;
+    2. adjacent-pulse cross products -> phase gradient per target and pulse,
+       with amplitude, per-target strength (capped at 3) and coherence
+       weights, capped at 4;
+    3. the phase gradient over k is the change of range: three iteratively
+       reweighted least-squares solves of the 2 x 2 per-pulse system
+       u_los . (dx, dy) = gradient / k (ridge 0.002 trace, robust scale
+       1.4826 MAD, Huber-like reweighting at 2.5 scale);
+    4. integrate, remove the linear trend, Savitzky-Golay smooth (window
+       min(501, odd <= P/2*2-1), order 3) and damp to at most 1 m.
+
+    Returns (delta [P, 3], stats dict) like the reference.
+    """
+    from scipy import signal
+    xi = np.asarray(contributions)
+    pos = np.asarray(pos, np.float64)
+    points = np.asarray(points, np.float64).reshape(-1, 3)
+    P = len(pos)
+    sos = signal.butter(4, cutoff, fs=prf, output="sos")
+    smooth = signal.sosfiltfilt(sos, xi, axis=1)
+    cross = smooth[:, 1:] * np.conj(smooth[:, :-1])
+    gradient = np.angle(cross)
+    amp = np.abs(cross)
+    target_power = np.mean(amp, axis=1)
+    strength = np.minimum(np.sqrt(target_power / np.median(target_power)), 3.)
+    coherence = np.abs(np.sum(cross, axis=1)) / np.sum(amp, axis=1)
+    w = amp / (target_power[:, None] + 1e-12) * strength[:, None]
+    w *= np.clip(coherence[:, None], 0, 1) ** 2
+    w = np.minimum(w, 4.)
+    los = points[:, None, :] - pos[None, 1:, :]
+    dist = np.linalg.norm(los, axis=2)
+    ux = los[:, :, 0] / dist
+    uy = los[:, :, 1] / dist
+    rhs = gradient / k
+    dx = np.zeros(P - 1)
+    dy = dx.copy()
+    for _ in range(3):
+        a = np.sum(w * ux * ux, axis=0)
+        b = np.sum(w * ux * uy, axis=0)
+        d = np.sum(w * uy * uy, axis=0)
+        ex = np.sum(w * ux * rhs, axis=0)
+        ey = np.sum(w * uy * rhs, axis=0)
+        ridge = .002 * (a + d)
+        det = (a + ridge) * (d + ridge) - b * b
+        dx = (ex * (d + ridge) - ey * b) / np.maximum(det, 1e-12)
+        dy = (ey * (a + ridge) - ex * b) / np.maximum(det, 1e-12)
+        residual = rhs - ux * dx - uy * dy
+        scale = 1.4826 * np.median(np.abs(residual), axis=0) + 1e-6
+        w *= np.minimum(1, 2.5 * scale[None, :] / (np.abs(residual) + 1e-12))
+    delta = np.column_stack([np.r_[0, np.cumsum(dx)], np.r_[0, np.cumsum(dy)], np.zeros(P)])
+    delta[:, :2] = signal.detrend(delta[:, :2], axis=0, type="linear")
+    delta[:, :2] = signal.savgol_filter(delta[:, :2], min(501, P // 2 * 2 - 1), 3, axis=0)
+    maxnorm = float(np.linalg.norm(delta[:, :2], axis=1).max())
+    if maxnorm > 1.0:
+        delta *= 1. / maxnorm
+    stats = dict(targets=len(points), cutoff_hz=cutoff,
+                 median_gradient_coherence=float(np.median(coherence)),
+                 delta_rms_m=np.sqrt(np.mean(delta ** 2, axis=0)).tolist(),
+                 delta_max_m=np.max(np.abs(delta), axis=0).tolist())
+    return delta, stats
+
+
+def estimate_update(data, pos, points, r_start, dr, k, prf, cutoff, *, mode=None):
+    """fast_bp/autofocus.py:37-79 with the same signature: per-(target, pulse)
+    contributions on the GPU (``bpmi_target_contributions``, FP64 geometry
+    and phase by default, complex64 out as the reference stores them), then
+    ``phase_gradient_update`` on the host.  Returns (delta [P, 3], stats)."""
     from .ffbp import target_contributions
-    contrib = target_contributions(data, pos, targets, r_start, dr, k, mode="mixed")
-    w = np.ones(contrib.shape[1]) if weight is None else np.asarray(weight)
-    cross = (contrib[:, 1:] * np.conj(contrib[:, :-1])).sum(axis=0)
-    grad = np.angle(cross)
-    est = np.r_[0.0, np.cumsum(grad)]
-    est -= np.average(est, weights=w)
-    return est
+    xi = target_contributions(data, pos, points, r_start, dr, k, mode=mode)
+    return phase_gradient_update(xi, pos, points, k, prf, cutoff)

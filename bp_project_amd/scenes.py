@@ -105,6 +105,33 @@ def synthetic_kat_scene():
                 angular_oversample=3.)
 
 
+def autofocus_kat_scene():
+    """Deterministic phase-centre error scene of fast_bp/validate_autofocus.py:9-20
+    (no RNG): 4096 pulses along a straight 140 m track at 18 m height, four
+    targets, range-compressed with a 400 MHz sinc (900 bins from 650 m), and a
+    known detrended sinusoidal x/y position error ``err`` that the phase-gradient
+    update must recover from data recorded on ``truth`` but imaged on
+    ``truth + err``."""
+    from scipy import signal
+    n = 4096
+    t = np.linspace(0, 4, n)
+    truth = np.column_stack([np.zeros(n), np.linspace(-70, 70, n), np.full(n, 18.)])
+    targets = np.array([[710., -290., 0.], [720., 320., 0.], [810., -100., 0.], [870., 280., 0.]])
+    k = -4 * np.pi * FC_REF / C
+    err = np.column_stack([.06 * np.sin(2 * np.pi * t), .12 * np.cos(np.pi * t), np.zeros(n)])
+    err[:, :2] = signal.detrend(err[:, :2], axis=0)
+    r0 = 650.
+    dr = C / (2 * FS_REF)
+    rr = r0 + dr * np.arange(900)
+    data = np.zeros((n, len(rr)), np.complex64)
+    for xyz in targets:
+        d = np.linalg.norm(truth - xyz, axis=1)
+        data += (np.exp(-1j * k * d)[:, None]
+                 * np.sinc((rr[None, :] - d[:, None]) / (C / 400e6))).astype(np.complex64)
+    return dict(data=data, truth=truth, err=err, pos=truth + err, targets=targets,
+                r_start=r0, data_dr=dr, k=k, prf=1024., cutoff=15.)
+
+
 # ---------------------------------------------------------------------------
 # Stripmap point-target workloads
 # ---------------------------------------------------------------------------

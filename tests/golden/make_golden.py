@@ -103,7 +103,9 @@ def main():
     af = load("fast_bp/results/autofocus_validation.json")
     fixture["autofocus_validation"] = {
         k: af[k] for k in ("targets", "cutoff_hz", "initial_coordinate_rmse_m",
-                           "corrected_coordinate_rmse_m", "delta_rms_m")}
+                           "corrected_coordinate_rmse_m", "delta_rms_m", "delta_max_m",
+                           "median_gradient_coherence")}
+    fixture["autofocus_validation"]["file"] = "fast_bp/results/autofocus_validation.json"
 
     OUT.write_text(json.dumps(fixture, indent=1) + "\n", encoding="utf-8")
     print("wrote", OUT)

@@ -85,3 +85,28 @@ def test_torch_sees_device_after_library_load():
     proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert proc.returncode == 0, proc.stderr[-2000:]
     assert proc.stdout.strip().endswith("4.0")
+
+
+def test_estimate_update_kat_on_gpu(oracle, golden):
+    """The reference's phase-gradient autofocus KAT (fast_bp/validate_autofocus.py,
+    results/autofocus_validation.json) through ``estimate_update`` with the
+    per-(target, pulse) contributions computed by bpmi_target_contributions."""
+    from bp_project_amd.autofocus import estimate_update
+    from bp_project_amd.ffbp import target_contributions
+    from bp_project_amd.scenes import autofocus_kat_scene
+    s = autofocus_kat_scene()
+    args = (s["r_start"], s["data_dr"], s["k"])
+    xi = target_contributions(s["data"], s["pos"], s["targets"], *args)
+    ref_xi = oracle.target_contributions(s["data"], s["pos"], s["targets"], *args)
+    e = np.linalg.norm(xi - ref_xi) / np.linalg.norm(ref_xi)
+    print(f"contributions vs FP64 oracle: {e:.2e}")
+    assert e <= 1e-6
+    delta, stats = estimate_update(s["data"], s["pos"], s["targets"], *args, s["prf"], s["cutoff"])
+    ref = golden["autofocus_validation"]
+    np.testing.assert_allclose(stats["median_gradient_coherence"],
+                               ref["median_gradient_coherence"], rtol=1e-7)
+    np.testing.assert_allclose(stats["delta_rms_m"], ref["delta_rms_m"], rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(stats["delta_max_m"], ref["delta_max_m"], rtol=1e-5, atol=1e-12)
+    after = float(np.sqrt(np.mean((s["err"] + delta)[:, :2] ** 2)))
+    print(f"corrected rmse {after:.10f} m (reference {ref['corrected_coordinate_rmse_m']:.10f})")
+    np.testing.assert_allclose(after, ref["corrected_coordinate_rmse_m"], rtol=1e-4)
