@@ -9,6 +9,7 @@
 //   others : FP64 geometry and sample coordinates, FP32 weights and
 //            accumulation, FP64 phase reduced to turns + FP32 sincos.
 #include "bpmi_internal.h"
+#include "bpmi_split.h"
 
 #include <cstring>
 
