@@ -91,7 +91,27 @@ SIGNATURES = {
     "bpmi_bp_points_adjoint": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _ll, _d, _d, _d, _i, _vp, _vp,
                                _sz, _vp, _vp, _vp],
     "bpmi_probe_sincos_turns": [_vp, _vp, _vp, _ll, _vp],
+    "bpmi_persistent_workspace_bytes": [_i, _i, POINTER(c_size_t)],
+    "bpmi_persistent_grid": [_i, _i, POINTER(c_int)],
+    "bpmi_ffbp_persistent": [_vp, _i, _i, _i, _vp],
+    "bpmi_persistent_aborted": [_vp, _i, _i, POINTER(c_int)],
+    "bpmi_persistent_stamps": [_vp, _i, _i, _vp],
 }
+
+
+class PersistentSchedule(ctypes.Structure):
+    """bpmi_persistent_schedule (include/bpmi.h), field for field."""
+    _fields_ = [
+        ("node_int", _vp), ("node_geo", _vp), ("offsets", _vp), ("arena", _vp),
+        ("sources", _vp), ("data", _vp), ("pos", _vp), ("weight", _vp), ("phase", _vp),
+        ("samples", _i), ("tile_cols_log2", _i),
+        ("range_start", _d), ("data_dr", _d), ("k", _d),
+        ("jobs", _vp), ("njobs", _i),
+        ("items", _vp), ("nitems", _i),
+        ("node_tiles", _vp), ("nodes", _i),
+        ("job_root", _vp), ("job_blocks", _vp),
+        ("workspace", _vp), ("max_wait_s", _d),
+    ]
 
 _lib = None
 

@@ -88,10 +88,14 @@ class BatchedResidentProgram(ResidentProgram):
     """ResidentProgram-compatible FFBP batched across each tree level."""
 
     def __init__(self, backend, plan, device, mode, range_start, data_dr, k, chunk_pulses=64,
-                 leaf_tile_cols_log2=None):
+                 leaf_tile_cols_log2=None, persistent=False):
+        if persistent and mode != "split":
+            raise ValueError("the persistent schedule runs in split precision only")
         self._table_buffers = []
         self._leaf_tc_request = leaf_tile_cols_log2
         self._batch_stages = []
+        self._persistent_request = bool(persistent)
+        self.persistent = None
         super().__init__(backend, plan, device, mode, range_start, data_dr, k,
                          algorithm="ffbp", variant="fused", chunk_pulses=chunk_pulses)
         self.variant = "batched_fused"
@@ -182,10 +186,33 @@ class BatchedResidentProgram(ResidentProgram):
                 "height": height, "kernel": kernel, "node_count": len(nodes),
                 "block_count": len(mapping), "valid_pixels": pixels,
                 "block_table_bytes": int(blocks.nbytes)})
+        if self._persistent_request:
+            self._replace_with_persistent(flat, sources)
         root = flat.offsets[self.plan.root.index]
         # the root image is a view into the arena
         return DeviceArray((self.plan.root.nr * self.plan.root.nu,), np.complex64,
                            arena._alloc, arena._offset + int(root) * 8)
+
+    def _replace_with_persistent(self, flat, sources):
+        """Swap the leaf and merge launches for one persistent launch
+        (bpmi_ffbp_persistent) over the same tables and arena."""
+        from .persistent import PersistentFFBP
+        staged = len(flat.levels)
+        del self._schedule[-staged:]
+        del self.operations[-staged:]
+        self.persistent = PersistentFFBP(
+            flat.node_int, self._node_table.ptr, self._geometry_table.ptr, self._offset_table.ptr,
+            self.arena.ptr, self.k, self.leaf_tile_cols_log2, sources=sources.ptr,
+            leaf_interp=self.backend.interp, merge_interp=self.backend.merge_interp)
+        pf = self.persistent
+        count = pf.nitems * MERGE_BLOCK
+        self._enqueue("bpmi_ffbp_persistent", count, (), name=f"ffbp_persistent_{self.mode}",
+                      launcher=lambda grid, block, args: pf.launch())
+        self.operations[-1].update({"height": None, "node_count": int(flat.node_count),
+                                    "work_items": pf.nitems})
+        self._batch_stages.append({"height": "all", "kernel": "ffbp_persistent",
+                                   "node_count": int(flat.node_count),
+                                   "block_count": pf.nitems})
 
     def node_image(self, index):
         """Device view of node ``index``'s polar image (valid after execute)."""
@@ -196,6 +223,9 @@ class BatchedResidentProgram(ResidentProgram):
 
     def close(self):
         super().close()
+        if self.persistent is not None:
+            self.persistent.close()
+        self.persistent = None
         self._table_buffers.clear()
         self._node_table = None
         self._geometry_table = None

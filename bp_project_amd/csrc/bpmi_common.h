@@ -78,6 +78,61 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(y, e, y);
 }
 
+// Wave-uniform copies (v_readfirstlane) of values every lane holds equally:
+// they move table reads the compiler cannot prove uniform into scalar
+// registers; on values already in scalar registers they compile to nothing.
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uniform_i64(long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double uniform_f64(double v) {
+    return __longlong_as_double(uniform_i64(__double_as_longlong(v)));
+}
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    return (T*)(uintptr_t)uniform_i64((long long)(uintptr_t)p);
+}
+
+// Read-only views of tables a kernel never writes (node tables, work
+// lists, argument blocks): loads through the constant address space are
+// scalar loads (scalar cache) when the address is wave-uniform, where a
+// generic pointer that the compiler cannot prove unwritten gets vector loads.
+#define BPMI_CONST_AS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const BPMI_CONST_AS T* as_const(const T* p) {
+    return (const BPMI_CONST_AS T*)p;
+}
+
+// complex64 loads/stores of memory that other workgroups of the same launch
+// write or read: COH = true uses device-scope (sc1) accesses, which are
+// coherent across the accelerator dies without cache-wide write-backs or
+// invalidations; COH = false is an ordinary access.
+template <bool COH>
+__device__ __forceinline__ float2 ld_c64(const float2* p) {
+    if constexpr (COH) {
+        const unsigned long long v = __hip_atomic_load(
+            (unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+    } else {
+        return *p;
+    }
+}
+template <bool COH>
+__device__ __forceinline__ void st_c64(float2* p, float2 v) {
+    if constexpr (COH) {
+        const unsigned long long u = (unsigned long long)__float_as_uint(v.x) |
+                                     ((unsigned long long)__float_as_uint(v.y) << 32);
+        __hip_atomic_store((unsigned long long*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
+}
+// Waits until every vector-memory access this wave issued has completed
+// (s_waitcnt vmcnt(0)): device-scope stores are then visible on every die.
+__device__ __forceinline__ void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // sqrt(max(x, 0)) from the reciprocal square root (0 for x <= 0).
 __device__ __forceinline__ double sqrt64_fast(double x) { return x > 0.0 ? x * rsqrt64(x) : 0.0; }
 

@@ -431,7 +431,7 @@ __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double
 // Separable stencil sum times the phase rotation.  row(a) points at tap
 // (a, 0) of the stencil (row ir - FIRST + a, column iu - FIRST), in global
 // memory or in an LDS copy: the arithmetic is identical either way.
-template <int INTERP, typename Row>
+template <int INTERP, typename Row, bool COH = false>
 __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
                                           const float* kp = &c_kaiser_poly[0][0]) {
     constexpr int TAPS = FastShape<INTERP>::TAPS;
@@ -450,7 +450,7 @@ __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
         float sx = 0.f, sy = 0.f;
 #pragma unroll
         for (int b = 0; b < TAPS; ++b) {
-            const float2 v = rp[b];
+            const float2 v = ld_c64<COH>(rp + b);
             sx += wu[b] * v.x;
             sy += wu[b] * v.y;
         }
@@ -460,13 +460,14 @@ __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
     return make_float2(z.x * c.rot.x - z.y * c.rot.y, z.x * c.rot.y + z.y * c.rot.x);
 }
 
-template <int INTERP>
+template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 stencil_global(const FastSrc& f, const ChildTap& c,
                                                  const float* kp = &c_kaiser_poly[0][0]) {
     constexpr int FIRST = FastShape<INTERP>::FIRST;
     const float2* base = f.img + (long long)(c.ir - FIRST) * f.ld + (c.iu - FIRST);
     const long long ld = f.ld;
-    return stencil<INTERP>(c, [&](int a) { return base + a * ld; }, kp);
+    auto row = [&](int a) { return base + a * ld; };
+    return stencil<INTERP, decltype(row), COH>(c, row, kp);
 }
 
 // Destination polar pixel p of node geometry g (fast_bp/ffbp.py:62-67).
@@ -505,17 +506,17 @@ __device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long 
 }
 
 // Child regrid at destination point (x, y) with destination slant range r.
-template <int INTERP>
+template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 regrid_fast(const FastSrc& f, double x, double y, double r,
                                               double kt) {
     const ChildTap c = child_tap<INTERP>(f, x, y, r, kt);
-    return c.ok ? stencil_global<INTERP>(f, c) : make_float2(0.f, 0.f);
+    return c.ok ? stencil_global<INTERP, COH>(f, c) : make_float2(0.f, 0.f);
 }
 
 // Polar image -> point (x, y), remodulated by exp(+j k r) (ffbp.py:189-198).
-template <int INTERP>
+template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 eval_fast(const FastSrc& f, double x, double y, double kt) {
-    return regrid_fast<INTERP>(f, x, y, 0.0, kt);
+    return regrid_fast<INTERP, COH>(f, x, y, 0.0, kt);
 }
 
 template <int INTERP>
@@ -599,7 +600,6 @@ constexpr int MT_ROWS = 32, MT_COLS = 32, MT_THREADS = 256;
 constexpr int MT_PPT = MT_ROWS * MT_COLS / MT_THREADS;
 constexpr int MT_CAP = 2176;
 
-__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 struct Window {
     int r0, c0, h, w;  // rows [r0, r0 + h), columns [c0, c0 + w) of the child
@@ -640,6 +640,7 @@ __device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w) {
 
 // Copy the window into LDS: wave v takes rows v, v + 4, .., lane j columns
 // j, j + 64, ..; per-row steps are uniform pointer increments.
+template <bool COH = false>
 __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, float2* dst,
                                              int wave, int lane) {
     constexpr int WAVES = MT_THREADS / 64;
@@ -651,7 +652,7 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
     if (w.w <= 64) {
         if (lane < w.w) {
             for (int i = wave; i < w.h; i += WAVES) {
-                d[lane] = src[lane];
+                d[lane] = ld_c64<COH>(src + lane);
                 src += src_step;
                 d += dst_step;
             }
@@ -659,13 +660,13 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
         return;
     }
     for (int i = wave; i < w.h; i += WAVES) {
-        for (int j = lane; j < w.w; j += 64) d[j] = src[j];
+        for (int j = lane; j < w.w; j += 64) d[j] = ld_c64<COH>(src + j);
         src += src_step;
         d += dst_step;
     }
 }
 
-template <int INTERP>
+template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
                                               const float2* sh, const float* kp) {
     constexpr int FIRST = FastShape<INTERP>::FIRST;
@@ -675,51 +676,71 @@ __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& 
         const int pitch = w.w;
         return stencil<INTERP>(c, [&](int a) { return base + a * pitch; }, kp);
     }
-    return stencil_global<INTERP>(f, c, kp);
+    return stencil_global<INTERP, COH>(f, c, kp);
 }
 
-template <int INTERP>
-__global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
-        const int* __restrict__ node_int, const double* __restrict__ node_geo,
-        const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
-        double k) {
-    __shared__ float2 sh[2 * MT_CAP];
-    __shared__ int corner[4][4];  // corner j: ir_a, iu_a, ir_b, iu_b
-    __shared__ double inv[4];     // 1/dr, 1/du of both children
-    // LDS copy of the Kaiser weight polynomials: read as vector operands, they
-    // free the 36 scalar registers whose pressure otherwise spills scalars
-    // into vector lanes inside the pixel loop.
-    __shared__ float kpoly[4 * KPOLY_STRIDE];
+// LDS of one merge tile: both child windows, the corner stencil origins,
+// the children's reciprocal spacings and a copy of the Kaiser weight
+// polynomials (read as vector operands, they free the 36 scalar registers
+// whose pressure otherwise spills scalars into vector lanes in the pixel loop).
+struct MergeLds {
+    float2 sh[2 * MT_CAP];
+    int corner[4][4];  // corner j: ir_a, iu_a, ir_b, iu_b
+    double inv[4];     // 1/dr, 1/du of both children
+    float kpoly[4 * KPOLY_STRIDE];
+};
+
+// One 32 x 32 tile of parent node ``node`` (all MT_THREADS threads call it;
+// every thread reaches both barriers).  kp_src: the Kaiser weight
+// polynomials (c_kaiser_poly).  A caller that runs several tiles in a row
+// must place a barrier between them.
+template <int INTERP, bool COH = false>
+__device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
+                                           const double* __restrict__ node_geo,
+                                           const long long* __restrict__ offsets, float2* arena,
+                                           int node, int tile, double k, const float* kp_src,
+                                           MergeLds& lds) {
     // The second window's base offset is opaque to the compiler, so the
     // stencil reads address it from a register rather than through
     // per-instruction offset additions (ds_read2 offsets reach 2 KiB).
     int second = MT_CAP;
     asm volatile("" : "+s"(second));
-    float2* const sha = sh;
-    float2* const shb = sh + second;
+    float2* const sha = lds.sh;
+    float2* const shb = lds.sh + second;
+    const float* const kpoly = lds.kpoly;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
-    if (node < 0) return;  // padding entry of a die-ordered table (uniform)
-    const int* ni = node_int + 8 * node;
-    const int nr = ni[2], nu = ni[3], side = ni[4], l = ni[5], r = ni[6];
+    // Node table entries as wave-uniform values (scalar loads and registers).
+    const BPMI_CONST_AS int* tni = as_const(node_int);
+    const BPMI_CONST_AS double* tgeo = as_const(node_geo);
+    const BPMI_CONST_AS long long* toff = as_const(offsets);
+    const int nr = uniform_i32(tni[8 * node + 2]), nu = uniform_i32(tni[8 * node + 3]);
+    const int side = uniform_i32(tni[8 * node + 4]);
+    const int l = uniform_i32(tni[8 * node + 5]), r = uniform_i32(tni[8 * node + 6]);
+    const int lnr = uniform_i32(tni[8 * l + 2]), lnu = uniform_i32(tni[8 * l + 3]);
+    const int rnr = uniform_i32(tni[8 * r + 2]), rnu = uniform_i32(tni[8 * r + 3]);
+    const float2* const img_a = arena + uniform_i64(toff[l]);
+    const float2* const img_b = arena + uniform_i64(toff[r]);
+    float2* const out = arena + uniform_i64(toff[node]);
+    double g[7], ga[7], gb[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        g[j] = uniform_f64(tgeo[7 * node + j]);
+        ga[j] = uniform_f64(tgeo[7 * l + j]);
+        gb[j] = uniform_f64(tgeo[7 * r + j]);
+    }
     const int tiles_u = (nu + MT_COLS - 1) / MT_COLS;
     const int tr = tile / tiles_u, tu = tile - tr * tiles_u;
     const int row0 = tr * MT_ROWS + t / MT_COLS, col = tu * MT_COLS + (t % MT_COLS);
     constexpr int ROW_STEP = MT_THREADS / MT_COLS;
 
-    const int* li = node_int + 8 * l;
-    const int* ri = node_int + 8 * r;
-    const double* ga = node_geo + 7 * l;
-    const double* gb = node_geo + 7 * r;
-    const double* g = node_geo + 7 * node;
     const double kt = k * INV_TWO_PI;
     // The first wave alone computes the children's reciprocal spacings and
     // the tile's corner pixels (clamped to the node), which locate the
     // windows: lanes 0..3 take one corner each.
     if (uniform_i32(wave) == 0) {
-        const FastSrc fa = make_fast_src(arena + offsets[l], ga, li[2], li[3], li[3]);
-        const FastSrc fb = make_fast_src(arena + offsets[r], gb, ri[2], ri[3], ri[3]);
+        const FastSrc fa = make_fast_src(img_a, ga, lnr, lnu, lnu);
+        const FastSrc fb = make_fast_src(img_b, gb, rnr, rnu, rnu);
         if (t < 4) {
             const int crow = min(tr * MT_ROWS + (t >> 1) * (MT_ROWS - 1), nr - 1);
             const int ccol = min(tu * MT_COLS + (t & 1) * (MT_COLS - 1), nu - 1);
@@ -727,44 +748,59 @@ __global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
             dest_pixel_rc(g, side, crow, ccol, &x, &y, &rr);
             const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
             const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-            corner[t][0] = ca.ir; corner[t][1] = ca.iu;
-            corner[t][2] = cb.ir; corner[t][3] = cb.iu;
+            lds.corner[t][0] = ca.ir; lds.corner[t][1] = ca.iu;
+            lds.corner[t][2] = cb.ir; lds.corner[t][3] = cb.iu;
         }
         if (t == 0) {
-            inv[0] = fa.inv_dr; inv[1] = fa.inv_du;
-            inv[2] = fb.inv_dr; inv[3] = fb.inv_du;
+            lds.inv[0] = fa.inv_dr; lds.inv[1] = fa.inv_du;
+            lds.inv[2] = fb.inv_dr; lds.inv[3] = fb.inv_du;
         }
-        if (t < 4 * KPOLY_STRIDE) kpoly[t] = (&c_kaiser_poly[0][0])[t];
+        if (t < 4 * KPOLY_STRIDE) lds.kpoly[t] = kp_src[t];
     }
     __syncthreads();
-    const FastSrc fa = make_fast_src_inv(arena + offsets[l], ga, li[2], li[3], li[3], inv[0], inv[1]);
-    const FastSrc fb = make_fast_src_inv(arena + offsets[r], gb, ri[2], ri[3], ri[3], inv[2], inv[3]);
+    const FastSrc fa = make_fast_src_inv(img_a, ga, lnr, lnu, lnu, uniform_f64(lds.inv[0]),
+                                         uniform_f64(lds.inv[1]));
+    const FastSrc fb = make_fast_src_inv(img_b, gb, rnr, rnu, rnu, uniform_f64(lds.inv[2]),
+                                         uniform_f64(lds.inv[3]));
     int ira[4], iua[4], irb[4], iub[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        ira[j] = uniform_i32(corner[j][0]); iua[j] = uniform_i32(corner[j][1]);
-        irb[j] = uniform_i32(corner[j][2]); iub[j] = uniform_i32(corner[j][3]);
+        ira[j] = uniform_i32(lds.corner[j][0]); iua[j] = uniform_i32(lds.corner[j][1]);
+        irb[j] = uniform_i32(lds.corner[j][2]); iub[j] = uniform_i32(lds.corner[j][3]);
     }
     const Window wa = corner_window<INTERP>(ira, iua, fa.nr, fa.nu);
     const Window wb = corner_window<INTERP>(irb, iub, fb.nr, fb.nu);
-    if (wa.lds) stage_window(fa, wa, sha, wave, lane);
-    if (wb.lds) stage_window(fb, wb, shb, wave, lane);
+    if (wa.lds) stage_window<COH>(fa, wa, sha, wave, lane);
+    if (wb.lds) stage_window<COH>(fb, wb, shb, wave, lane);
     __syncthreads();
-    if (col >= nu) return;
-    float2* out = arena + offsets[node];
-    const DestColumn dc = dest_column(g, side, col);
+    if (col < nu) {
+        const DestColumn dc = dest_column(g, side, col);
 #pragma unroll 1
-    for (int j = 0; j < MT_PPT; ++j) {
-        const int row = row0 + j * ROW_STEP;
-        if (row >= nr) break;
-        double x, y, rr;
-        dest_pixel_col(g, dc, row, &x, &y, &rr);
-        const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
-        const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-        const float2 va = tile_sample<INTERP>(fa, ca, wa, sha, kpoly);
-        const float2 vb = tile_sample<INTERP>(fb, cb, wb, shb, kpoly);
-        out[(long long)row * nu + col] = make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y));
+        for (int j = 0; j < MT_PPT; ++j) {
+            const int row = row0 + j * ROW_STEP;
+            if (row >= nr) break;
+            double x, y, rr;
+            dest_pixel_col(g, dc, row, &x, &y, &rr);
+            const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
+            const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
+            const float2 va = tile_sample<INTERP, COH>(fa, ca, wa, sha, kpoly);
+            const float2 vb = tile_sample<INTERP, COH>(fb, cb, wb, shb, kpoly);
+            st_c64<COH>(out + (long long)row * nu + col,
+                        make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y)));
+        }
     }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
+        const int* __restrict__ node_int, const double* __restrict__ node_geo,
+        const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
+        double k) {
+    __shared__ MergeLds lds;
+    const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
+    if (node < 0) return;  // padding entry of a die-ordered table (uniform)
+    merge_tile<INTERP>(node_int, node_geo, offsets, arena, node, tile, k, &c_kaiser_poly[0][0],
+                       lds);
 }
 
 template <int INTERP>
@@ -933,6 +969,326 @@ __global__ void sincos_probe_kernel(const float* t, float* s, float* c, long lon
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     sincos_turns(t[p], &s[p], &c[p]);
+}
+
+// ---- persistent FFBP: one launch for leaves, every merge level and the
+// Cartesian evaluation ---------------------------------------------------------
+// Workgroups claim work items in list order from one ticket counter.  The
+// list is topologically ordered (every item's inputs come from items earlier
+// in the list), so a claimed item only ever waits for items already claimed
+// by running workgroups: the schedule cannot deadlock, whatever the number
+// of resident workgroups.  A merge tile waits until both children have
+// counted all their tiles complete; an evaluation block waits for its root.
+// Node images pass between items through device-scope (sc1) loads and
+// stores, which are coherent across the accelerator dies without the
+// cache-wide L2 write-back / invalidate of an agent-scope fence (measured
+// 5x slower on the 8192-pulse FFBP): every wave waits for its stores to
+// complete before the item's completion is counted, and a consumer issues
+// its device-scope loads only after it has seen the count.  Every wait is bounded by a wall-clock limit: a wait that runs out
+// sets the abort flag, after which every remaining item is skipped and all
+// workgroups drain, so the launch always ends; the host reads the flag.
+enum { PW_LEAF = 0, PW_MERGE = 1, PW_EVAL = 2 };
+enum { PS_TICKET = 0, PS_ABORT = 1, PS_NODES = 2 };
+
+struct PersistentArgs {
+    const int* node_int;
+    const double* node_geo;
+    const long long* offsets;
+    float2* arena;
+    const bpmi_node_source* sources;
+    BpArgs leaf;  // single-patch leaf inputs (when sources is null)
+    int tc_log2;
+    const bpmi_eval_job* jobs;
+    const int4* items;
+    int nitems;
+    const int* node_tiles;
+    const int* job_root;
+    const int* job_blocks;
+    int* sync;  // ticket, abort flag, done counts of the nodes, then of the jobs
+    int nodes;
+    unsigned long long* stamps;  // [0] first start, [1 + j] completion of job j
+    unsigned long long spin_ticks;
+    double k;
+};
+
+__device__ __forceinline__ int load_count(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ BpArgs uniform_bp_args(const BpArgs& v) {
+    BpArgs a;
+    a.data = uniform_ptr(v.data);
+    a.pos = uniform_ptr(v.pos);
+    a.weight = uniform_ptr(v.weight);
+    a.phase = uniform_ptr(v.phase);
+    a.samples = uniform_i32(v.samples);
+    a.pulse_begin = uniform_i32(v.pulse_begin);
+    a.pulse_end = uniform_i32(v.pulse_end);
+    a.range_start = uniform_f64(v.range_start);
+    a.data_dr = uniform_f64(v.data_dr);
+    a.k = uniform_f64(v.k);
+    a.flags = uniform_i32(v.flags);
+    a.out = uniform_ptr(v.out);
+    return a;
+}
+
+__device__ __forceinline__ SrcArgs uniform_src_args(const SrcArgs& v) {
+    SrcArgs s;
+    s.a0 = uniform_f64(v.a0); s.a1 = uniform_f64(v.a1); s.a2 = uniform_f64(v.a2);
+    s.b0 = uniform_f64(v.b0); s.b1 = uniform_f64(v.b1); s.b2 = uniform_f64(v.b2);
+    s.b3 = uniform_f64(v.b3);
+    s.rows = uniform_i32(v.rows); s.cols = uniform_i32(v.cols); s.side = uniform_i32(v.side);
+    s.xyz = uniform_ptr(v.xyz);
+    s.count = uniform_i64(v.count);
+    return s;
+}
+
+// The tables live in device memory (``tab``) and are read after each item
+// starts; the loop's own state is kept in LDS and re-read where needed, so
+// nothing stays live in scalar registers across an item body (the merge
+// body alone fills the scalar register file).
+//
+// Contention: a device-scope atomic on one address is serialised (~50 ns
+// each), so one claim and one completion count per item would cost more than
+// the items themselves.  Workgroups therefore claim PW_CHUNK consecutive
+// items per ticket and keep completion counts pending in LDS, adding them
+// to the node's counter when the next item belongs to another counter, at
+// the end of the chunk, and before the workgroup itself has to wait (so a
+// count that a waiter needs is never held back by the waiter).  A
+// dependency seen complete is remembered, so a run of tiles of one parent
+// checks its children once.
+constexpr int PW_CHUNK = 16;
+
+struct PersistentLoop {
+    const PersistentArgs* tab;
+    const int4* items;
+    int* sync;
+    unsigned long long* stamps;
+    int nitems;
+    int item, last;
+    int go;
+    int pend_idx, pend_cnt;  // pending completion count and its counter index
+    int ready0, ready1;      // last dependency pair seen complete
+};
+
+template <typename T>
+__device__ __forceinline__ T lds_reload(const T& v) {
+    return *(const volatile T*)&v;
+}
+
+
+// Thread 0: add the pending completion count to its counter (a job's
+// counter reaching its block count stamps the job's completion time).
+__device__ __forceinline__ void flush_pending(PersistentLoop& st) {
+    if (st.pend_cnt <= 0) return;
+    const BPMI_CONST_AS PersistentArgs& pa = *as_const(st.tab);
+    const int old = atomicAdd(st.sync + PS_NODES + st.pend_idx, st.pend_cnt);
+    const int job = st.pend_idx - pa.nodes;
+    if (job >= 0 && old + st.pend_cnt == as_const(pa.job_blocks)[job])
+        st.stamps[1 + job] = wall_clock64();
+    st.pend_cnt = 0;
+}
+
+// Thread 0: true when item w may run, false after an abort.
+__device__ __forceinline__ bool persistent_wait(PersistentLoop& st, int4 w) {
+    const BPMI_CONST_AS PersistentArgs& pa = *as_const(st.tab);
+    int n0 = -1, n1 = -1;
+    if (w.x == PW_MERGE) {
+        n0 = as_const(pa.node_int)[8 * w.y + 5];
+        n1 = as_const(pa.node_int)[8 * w.y + 6];
+    } else if (w.x == PW_EVAL) {
+        n0 = as_const(pa.job_root)[w.y];
+    }
+    if (n0 < 0 || (n0 == st.ready0 && n1 == st.ready1)) return true;
+    const int* done = st.sync + PS_NODES;
+    const int need0 = as_const(pa.node_tiles)[n0];
+    const int need1 = n1 >= 0 ? as_const(pa.node_tiles)[n1] : 0;
+    const unsigned long long t0 = wall_clock64();
+    for (bool first = true;; first = false) {
+        if (load_count(done + n0) >= need0 && (n1 < 0 || load_count(done + n1) >= need1)) {
+            st.ready0 = n0;
+            st.ready1 = n1;
+            return true;
+        }
+        if (first) flush_pending(st);
+        if (load_count(st.sync + PS_ABORT)) return false;
+        if (wall_clock64() - t0 > pa.spin_ticks) {
+            __hip_atomic_store(st.sync + PS_ABORT, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+}
+
+template <int LEAF_INTERP, int MERGE_INTERP>
+__global__ __launch_bounds__(MT_THREADS, 4) void ffbp_persistent_kernel(
+        const PersistentArgs* tab_arg, const int4* items_arg, int nitems_arg, int* sync_arg,
+        unsigned long long* stamps_arg) {
+    static_assert(SP_THREADS == MT_THREADS, "one workgroup shape for every item kind");
+    __shared__ union {
+        SplitLds<LEAF_INTERP> split;
+        MergeLds merge;
+    } lds;
+    __shared__ PersistentLoop st;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        st.tab = tab_arg;
+        st.items = items_arg;
+        st.sync = sync_arg;
+        st.stamps = stamps_arg;
+        st.nitems = nitems_arg;
+        st.pend_idx = -1;
+        st.pend_cnt = 0;
+        st.ready0 = st.ready1 = -2;
+        atomicMin(stamps_arg, wall_clock64());
+    }
+    for (;;) {
+        // claim the next chunk; an aborted launch claims nothing more
+        if (t == 0) {
+            const bool abort = load_count(st.sync + PS_ABORT) != 0;
+            const int first = abort ? st.nitems : atomicAdd(st.sync + PS_TICKET, PW_CHUNK);
+            st.item = first;
+            st.last = min(first + PW_CHUNK, st.nitems);
+        }
+        __syncthreads();
+        if (uniform_i32(lds_reload(st.item)) >= uniform_i32(lds_reload(st.nitems))) break;
+        for (;;) {
+            const int item = uniform_i32(lds_reload(st.item));
+            if (item >= uniform_i32(lds_reload(st.last))) break;
+            const BPMI_CONST_AS PersistentArgs* tab = as_const(uniform_ptr(lds_reload(st.tab)));
+            const BPMI_CONST_AS int* wv =
+                as_const((const int*)uniform_ptr(lds_reload(st.items))) + 4 * item;
+            const int4 w = make_int4(uniform_i32(wv[0]), uniform_i32(wv[1]), uniform_i32(wv[2]), 0);
+            if (t == 0) st.go = persistent_wait(st, w) ? 1 : 0;
+            __syncthreads();
+            const bool go = uniform_i32(lds_reload(st.go)) != 0;
+            if (go) {
+                const int* node_int = uniform_ptr(tab->node_int);
+                const double* node_geo = uniform_ptr(tab->node_geo);
+                const long long* offsets = uniform_ptr(tab->offsets);
+                float2* arena = uniform_ptr(tab->arena);
+                if (w.x == PW_LEAF) {
+                    BpArgs a;
+                    SrcArgs s{};
+                    const bpmi_node_source* sources = uniform_ptr(tab->sources);
+                    const LeafTables lt{node_int, node_geo, offsets, arena, nullptr, sources};
+                    if (!sources) {
+                        const BPMI_CONST_AS BpArgs& la = tab->leaf;
+                        a.data = la.data; a.pos = la.pos; a.weight = la.weight; a.phase = la.phase;
+                        a.samples = la.samples; a.range_start = la.range_start;
+                        a.data_dr = la.data_dr; a.k = la.k;
+                    }
+                    leaf_args(lt, w.y, &a, &s);
+                    a = uniform_bp_args(a);
+                    s = uniform_src_args(s);
+                    bp_split_tile<LEAF_INTERP, SRC_POLAR, true, true>(
+                        a, s, uniform_i32(tab->tc_log2), w.z, lds.split);
+                } else if (w.x == PW_MERGE) {
+                    merge_tile<MERGE_INTERP, true>(node_int, node_geo, offsets, arena, w.y, w.z,
+                                                   uniform_f64(tab->k), &c_kaiser_poly[0][0],
+                                                   lds.merge);
+                } else {
+                    const BPMI_CONST_AS bpmi_eval_job* jp = as_const(tab->jobs) + w.y;
+                    const int nx = uniform_i32(jp->nx), ny = uniform_i32(jp->ny);
+                    const long long p = (long long)w.z + t;
+                    if (p < (long long)nx * ny) {
+                        double geo[7];
+#pragma unroll
+                        for (int j = 0; j < 7; ++j) geo[j] = uniform_f64(jp->geo[j]);
+                        const FastSrc f = make_fast_src(uniform_ptr((const float2*)jp->img), geo,
+                                                        uniform_i32(jp->nr), uniform_i32(jp->nu),
+                                                        uniform_i32(jp->ld));
+                        const int iy = (int)(p / nx), ix = (int)(p - (long long)iy * nx);
+                        uniform_ptr((float2*)jp->out)[p] = eval_fast<MERGE_INTERP, true>(
+                            f, uniform_f64(jp->x0) + ix * uniform_f64(jp->dx),
+                            uniform_f64(jp->y0) + iy * uniform_f64(jp->dy),
+                            uniform_f64(jp->k) * INV_TWO_PI);
+                    }
+                }
+                // the item's device-scope stores have completed in every wave
+                // before its completion is counted
+                wait_vector_memory();
+            }
+            __syncthreads();
+            if (t == 0) {
+                if (lds_reload(st.go)) {
+                    // the item again, from the loop state (nothing of it was kept live)
+                    const BPMI_CONST_AS int* wi =
+                        as_const((const int*)lds_reload(st.items)) + 4 * lds_reload(st.item);
+                    const int idx =
+                        wi[0] == PW_EVAL ? as_const(lds_reload(st.tab))->nodes + wi[1] : wi[1];
+                    if (idx != st.pend_idx) {
+                        flush_pending(st);
+                        st.pend_idx = idx;
+                    }
+                    st.pend_cnt += 1;
+                }
+                st.item = lds_reload(st.item) + 1;
+            }
+            __syncthreads();
+        }
+        // every wave has read the chunk's end before thread 0 claims the next
+        __syncthreads();
+        if (t == 0) flush_pending(st);
+    }
+}
+
+// Workspace layout: the argument block, then the int32 counters (ticket,
+// abort flag, node completion counts, job completion counts), then the u64
+// time stamps (first start, completion of each job).
+constexpr size_t PW_ARGS_BYTES = 512;
+static_assert(sizeof(PersistentArgs) <= PW_ARGS_BYTES, "argument block");
+__host__ __device__ inline size_t pw_sync_ints(int nodes, int njobs) {
+    return (size_t)PS_NODES + (size_t)nodes + (size_t)njobs;
+}
+__host__ __device__ inline size_t pw_stamps_offset(int nodes, int njobs) {
+    return (PW_ARGS_BYTES + 4 * pw_sync_ints(nodes, njobs) + 7) & ~(size_t)7;
+}
+
+// Writes the argument block and clears the counters and time stamps.
+__global__ void persistent_reset_kernel(PersistentArgs args, PersistentArgs* tab, int nsync,
+                                        int nstamps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *tab = args;
+    if (i < nsync) args.sync[i] = 0;
+    if (i < nstamps) args.stamps[i] = i == 0 ? ~0ull : 0ull;
+}
+
+template <int L, int M>
+static int persistent_grid_for(int* grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st) return st;
+    st = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                   "hipDeviceGetAttribute");
+    if (st) return st;
+    st = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                       &per_cu, reinterpret_cast<const void*>(ffbp_persistent_kernel<L, M>),
+                       MT_THREADS, 0),
+                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (st) return st;
+    *grid = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    return BPMI_OK;
+}
+
+static int persistent_interps(int leaf_interp, int merge_interp) {
+    if (leaf_interp != BPMI_INTERP_LAGRANGE4 && leaf_interp != BPMI_INTERP_LINEAR)
+        return set_error(BPMI_ERR_UNSUPPORTED, "persistent leaves support lagrange4 and linear");
+    if (merge_interp != BPMI_INTERP_LAGRANGE4 && merge_interp != BPMI_INTERP_KAISER8)
+        return set_error(BPMI_ERR_UNSUPPORTED, "persistent merges support lagrange4 and kaiser8");
+    if (merge_interp == BPMI_INTERP_KAISER8 && !g_kaiser_table)
+        return set_error(BPMI_ERR_INVALID, "kaiser8 needs bpmi_set_kaiser_table() first");
+    return BPMI_OK;
+}
+
+// Calls f.template operator()<LEAF, MERGE>() for the runtime interpolators.
+template <typename F>
+static int dispatch_persistent(int leaf_interp, int merge_interp, F&& f) {
+    const bool lin = leaf_interp == BPMI_INTERP_LINEAR, kai = merge_interp == BPMI_INTERP_KAISER8;
+    if (lin) return kai ? f.template operator()<INTERP_LINEAR, INTERP_KAISER8>()
+                        : f.template operator()<INTERP_LINEAR, INTERP_LAGRANGE4>();
+    return kai ? f.template operator()<INTERP_LAGRANGE4, INTERP_KAISER8>()
+               : f.template operator()<INTERP_LAGRANGE4, INTERP_LAGRANGE4>();
 }
 
 // ---- dispatch helpers --------------------------------------------------------
@@ -1113,6 +1469,101 @@ int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo, const lo
         hipLaunchKernelGGL(batched_merge_tiled_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
                            dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
     return check_launch("batched_merge_tiled_kernel");
+}
+
+int bpmi_persistent_workspace_bytes(int nodes, int njobs, size_t* bytes) {
+    if (!bytes || nodes < 0 || njobs < 0) return set_error(BPMI_ERR_INVALID, "bad sizes");
+    *bytes = pw_stamps_offset(nodes, njobs) + 8 * (size_t)(1 + njobs);
+    return BPMI_OK;
+}
+
+int bpmi_persistent_grid(int leaf_interp, int merge_interp, int* grid) {
+    if (!grid) return set_error(BPMI_ERR_INVALID, "null grid");
+    int st = persistent_interps(leaf_interp, merge_interp);
+    if (st) return st;
+    return dispatch_persistent(leaf_interp, merge_interp,
+                               [&]<int L, int M>() { return persistent_grid_for<L, M>(grid); });
+}
+
+int bpmi_ffbp_persistent(const bpmi_persistent_schedule* sc, int leaf_interp, int merge_interp,
+                         int grid, void* stream) {
+    if (!sc) return set_error(BPMI_ERR_INVALID, "null schedule");
+    int st = persistent_interps(leaf_interp, merge_interp);
+    if (st) return st;
+    if (sc->nitems <= 0) return BPMI_OK;
+    if (!sc->node_int || !sc->node_geo || !sc->offsets || !sc->arena || !sc->items ||
+        !sc->node_tiles || !sc->workspace)
+        return set_error(BPMI_ERR_INVALID, "null table or workspace");
+    if (sc->nodes <= 0 || sc->njobs < 0) return set_error(BPMI_ERR_INVALID, "bad node/job counts");
+    if (sc->njobs > 0 && (!sc->jobs || !sc->job_root || !sc->job_blocks))
+        return set_error(BPMI_ERR_INVALID, "null evaluation table");
+    if (!sc->sources && (!sc->data || !sc->pos || sc->samples <= 0))
+        return set_error(BPMI_ERR_INVALID, "leaf inputs missing (sources or data/pos/samples)");
+    if (sc->tile_cols_log2 < 3 || sc->tile_cols_log2 > 6)
+        return set_error(BPMI_ERR_INVALID, "tile_cols_log2 must be 3..6");
+    if (!(sc->max_wait_s > 0.0) || sc->max_wait_s > 1.0e6)
+        return set_error(BPMI_ERR_INVALID, "max_wait_s must be in (0, 1e6]");
+    if (grid <= 0) {
+        st = bpmi_persistent_grid(leaf_interp, merge_interp, &grid);
+        if (st) return st;
+    }
+    char* ws = (char*)sc->workspace;
+    PersistentArgs pa;
+    pa.node_int = sc->node_int;
+    pa.node_geo = sc->node_geo;
+    pa.offsets = sc->offsets;
+    pa.arena = (float2*)sc->arena;
+    pa.sources = sc->sources;
+    pa.leaf = BpArgs{(const float2*)sc->data, sc->pos, sc->weight, sc->phase, sc->samples, 0, 0,
+                     sc->range_start, sc->data_dr, sc->k, 0, nullptr};
+    pa.tc_log2 = sc->tile_cols_log2;
+    pa.jobs = sc->jobs;
+    pa.items = (const int4*)sc->items;
+    pa.nitems = sc->nitems;
+    pa.node_tiles = sc->node_tiles;
+    pa.job_root = sc->job_root;
+    pa.job_blocks = sc->job_blocks;
+    pa.sync = (int*)(ws + PW_ARGS_BYTES);
+    pa.nodes = sc->nodes;
+    pa.stamps = (unsigned long long*)(ws + pw_stamps_offset(sc->nodes, sc->njobs));
+    // the device wall clock runs at 100 MHz
+    pa.spin_ticks = (unsigned long long)(sc->max_wait_s * 1.0e8);
+    pa.k = sc->k;
+    hipStream_t hs = (hipStream_t)stream;
+    const int nsync = (int)pw_sync_ints(sc->nodes, sc->njobs);
+    const int nstamps = 1 + sc->njobs;
+    const int nreset = nsync > nstamps ? nsync : nstamps;
+    PersistentArgs* tab = (PersistentArgs*)ws;
+    hipLaunchKernelGGL(persistent_reset_kernel, dim3(grid_1d(nreset, 256)), dim3(256), 0, hs, pa,
+                       tab, nsync, nstamps);
+    st = check_launch("persistent_reset_kernel");
+    if (st) return st;
+    return dispatch_persistent(leaf_interp, merge_interp, [&]<int L, int M>() {
+        hipLaunchKernelGGL((ffbp_persistent_kernel<L, M>), dim3(grid), dim3(MT_THREADS), 0, hs,
+                           (const PersistentArgs*)tab, pa.items, pa.nitems, pa.sync, pa.stamps);
+        return check_launch("ffbp_persistent_kernel");
+    });
+}
+
+int bpmi_persistent_aborted(const void* workspace, int nodes, int njobs, int* aborted) {
+    if (!workspace || !aborted || nodes < 0 || njobs < 0)
+        return set_error(BPMI_ERR_INVALID, "bad argument");
+    int v = 0;
+    const int* sync = (const int*)((const char*)workspace + PW_ARGS_BYTES);
+    int st = check_hip(hipMemcpy(&v, sync + PS_ABORT, sizeof(int), hipMemcpyDeviceToHost),
+                       "hipMemcpy(abort flag)");
+    if (st) return st;
+    *aborted = v;
+    return BPMI_OK;
+}
+
+int bpmi_persistent_stamps(const void* workspace, int nodes, int njobs,
+                           unsigned long long* stamps) {
+    if (!workspace || !stamps || nodes < 0 || njobs < 0)
+        return set_error(BPMI_ERR_INVALID, "bad argument");
+    const char* src = (const char*)workspace + pw_stamps_offset(nodes, njobs);
+    return check_hip(hipMemcpy(stamps, src, 8 * (size_t)(1 + njobs), hipMemcpyDeviceToHost),
+                     "hipMemcpy(stamps)");
 }
 
 int bpmi_eval_points(const void* img, const double* geo, int nr, int nu, int ld, const double* xyz,

@@ -240,13 +240,13 @@ struct LeafTables {
 };
 
 __device__ __forceinline__ void leaf_args(const LeafTables& lt, int node, BpArgs* a, SrcArgs* s) {
-    const int* ni = lt.node_int + 8 * node;
-    const double* g = lt.node_geo + 7 * node;
+    const BPMI_CONST_AS int* ni = as_const(lt.node_int) + 8 * node;
+    const BPMI_CONST_AS double* g = as_const(lt.node_geo) + 7 * node;
     a->pulse_begin = ni[0];
     a->pulse_end = ni[1];
     a->flags = 0;
     if (lt.sources) {
-        const bpmi_node_source& src = lt.sources[node];
+        const BPMI_CONST_AS bpmi_node_source& src = as_const(lt.sources)[node];
         a->data = (const float2*)src.data;
         a->pos = src.pos;
         a->weight = src.weight;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void leaf_args(const LeafTables& lt, int node, BpArgs
         a->k = src.k;
         a->samples = src.samples;
     }
-    a->out = lt.arena + lt.offsets[node];
+    a->out = lt.arena + as_const(lt.offsets)[node];
     s->a0 = g[0]; s->a1 = g[1]; s->a2 = g[2];
     s->b0 = g[3]; s->b1 = g[4]; s->b2 = g[5]; s->b3 = g[6];
     s->rows = ni[2];
@@ -395,7 +395,9 @@ struct SplitLds {
 // call it; every thread reaches every barrier, no early return).  A caller
 // that runs several tiles in a row must place a barrier between them, since
 // the first barrier here is reached after LDS writes of the tile setup.
-template <int INTERP, int SRC, bool ORIGIN_REF>
+// COH: the output is read by other workgroups of the same launch (device-
+// scope stores, see ld_c64/st_c64).
+template <int INTERP, int SRC, bool ORIGIN_REF, bool COH = false>
 __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, int tile,
                                               SplitLds<INTERP>& lds) {
     constexpr int F4 = SlotSize<INTERP>::F4;
@@ -735,11 +737,11 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
             v = make_float2(v.x * rot.x - v.y * rot.y, v.x * rot.y + v.y * rot.x);
         }
         if (a.flags & BPMI_ACCUMULATE) {
-            const float2 o = out[pix[k]];
+            const float2 o = ld_c64<COH>(out + pix[k]);
             v.x += o.x;
             v.y += o.y;
         }
-        out[pix[k]] = v;
+        st_c64<COH>(out + pix[k], v);
     }
 }
 

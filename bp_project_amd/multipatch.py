@@ -77,7 +77,7 @@ class MultiPatchProgram:
 
     def __init__(self, scenes, mode="split", leaf_size=16, range_spacing=0.5,
                  angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
-                 tile_cols_log2=6):
+                 tile_cols_log2=6, persistent=False):
         if not scenes:
             raise ValueError("no patches")
         ks = {round(s.k, 9) for s in scenes}
@@ -85,6 +85,10 @@ class MultiPatchProgram:
             raise ValueError("all patches of a batch must share the carrier wavenumber")
         if "kaiser8" in (leaf_interp, merge_interp):
             ensure_kaiser_table()
+        if persistent and mode != "split":
+            raise ValueError("the persistent schedule runs in split precision only")
+        self.persistent = bool(persistent)
+        self._persistent = None
         self.mode = mode
         self.leaf_interp = leaf_interp
         self.merge_interp = merge_interp
@@ -154,12 +158,18 @@ class MultiPatchProgram:
             jobs[j]["nr"], jobs[j]["nu"], jobs[j]["ld"] = plan.nr, plan.nu, plan.root.nu
             jobs[j]["nx"], jobs[j]["ny"] = g.nx, g.ny
             eval_blocks += [(j, s) for s in range(0, g.nx * g.ny, BLOCK)]
+        self.jobs_host = jobs
+        self.job_roots = [self.node_base[j] + pp.plan.root.index
+                          for j, pp in enumerate(self.patches)]
         self.t_jobs = DeviceArray.from_host(jobs.view(np.uint8))
         self.t_eval_blocks = DeviceArray.from_host(np.asarray(eval_blocks, np.int32))
         self.n_eval_blocks = len(eval_blocks)
         self._graph = None
         self._stream = None
         self.launch_count = 2 + len(self.merge_tables)
+        if self.persistent:
+            self._build_persistent()
+            self.launch_count = 1
 
     def set_tile_width(self, tile_cols_log2):
         """(Re)build the leaf block table for a split-mode tile width."""
@@ -182,6 +192,35 @@ class MultiPatchProgram:
         if getattr(self, "_graph", None) is not None:
             self._graph.close()
             self._graph = None
+        if getattr(self, "_persistent", None) is not None:
+            self._build_persistent()
+
+    def _build_persistent(self):
+        """One persistent launch for the whole batch.  Within each height,
+        nodes of patches with fewer levels go first, so those patches
+        complete early."""
+        from .persistent import PersistentFFBP
+        root_height = np.zeros(len(self.node_int), np.int64)
+        for j, pp in enumerate(self.patches):
+            n0 = self.node_base[j]
+            root_height[n0:n0 + len(pp.flat.node_int)] = pp.flat.node_int[pp.plan.root.index,
+                                                                          NI_HEIGHT]
+        if self._persistent is not None:
+            self._persistent.close()
+        self._persistent = PersistentFFBP(
+            self.node_int, self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
+            self.arena.ptr, self.k, self.tile_cols_log2, sources=self.t_sources.ptr,
+            jobs=self.jobs_host, job_roots=self.job_roots, leaf_interp=self.leaf_interp,
+            merge_interp=self.merge_interp, priority=root_height)
+
+    def completion_ms(self):
+        """Persistent schedule, after a completed launch: each patch's
+        completion time in ms after the launch's first workgroup started."""
+        if self._persistent is None:
+            raise RuntimeError("completion times need persistent=True")
+        if self._persistent.aborted():
+            raise RuntimeError("persistent FFBP aborted: a dependency wait timed out")
+        return self._persistent.completion_ms()
 
     @property
     def work(self):
@@ -191,6 +230,9 @@ class MultiPatchProgram:
     def execute(self):
         mode = _lib.mode_code(self.mode)
         s = stream_handle()
+        if self._persistent is not None:
+            self._persistent.launch(s)
+            return self.outputs
         _lib.call("bpmi_ffbp_leaves_multi", _vp(self.t_sources.ptr), _vp(self.t_node_int.ptr),
                   _vp(self.t_node_geo.ptr), _vp(self.t_offsets.ptr), _vp(self.arena.ptr),
                   _vp(self.leaf_blocks.ptr), self.n_leaf_blocks, mode,
@@ -225,7 +267,8 @@ class MultiPatchProgram:
         if self.mode != "split":
             return self.tile_cols_log2, {}
         work = int(sum(pp.plan.counts["leaf_pixel_pulse_products"] for pp in self.patches))
-        key = make_key("ffbp_leaves_multi", self.mode, self.leaf_interp, work)
+        kernel = "ffbp_persistent_multi" if self._persistent is not None else "ffbp_leaves_multi"
+        key = make_key(kernel, self.mode, self.leaf_interp, work)
 
         def run(tc):
             self.set_tile_width(tc)
@@ -238,6 +281,9 @@ class MultiPatchProgram:
         if self._graph is not None:
             self._graph.close()
         self._graph = None
+        if self._persistent is not None:
+            self._persistent.close()
+        self._persistent = None
         self.outputs = []
         self._inputs = []
         self.arena = None

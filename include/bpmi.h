@@ -259,6 +259,68 @@ typedef struct bpmi_eval_job {
 int bpmi_eval_grid_multi(const bpmi_eval_job* jobs, const int* blocks, int nblocks, int mode,
                          int interp, void* stream);
 
+/* ---- persistent FFBP: one launch for a whole schedule ------------------- */
+/* No single counterpart in the reference: it replaces the per-level launch
+ * sequence of BatchedResidentProgram (batched_resident_backend.py:72-150:
+ * batched leaves, one merge launch per subtree height, crop) -- and, for
+ * many patches, one such sequence per patch -- by ONE launch in split
+ * precision.  Workgroups claim work items from one ticket counter; a merge
+ * tile starts once both children have completed all their tiles, an
+ * evaluation block once its root has.  Outputs are bitwise equal to the
+ * level-by-level launches (bpmi_ffbp_leaves[_multi] + bpmi_ffbp_merges_tiled
+ * + bpmi_eval_grid_multi) on the same tables. */
+enum { BPMI_WORK_LEAF = 0, BPMI_WORK_MERGE = 1, BPMI_WORK_EVAL = 2 };
+typedef struct bpmi_persistent_schedule {
+    /* node tables and arena as for bpmi_ffbp_leaves / bpmi_ffbp_merges_tiled */
+    const int* node_int;
+    const double* node_geo;
+    const long long* offsets;
+    void* arena;
+    /* leaf inputs: per node (many patches), or NULL and the single patch below */
+    const bpmi_node_source* sources;
+    const void* data;               /* complex64 [pulses, samples] */
+    const double* pos;              /* f64 [pulses, 3] */
+    const double* weight;           /* f64 [pulses] or NULL */
+    const double* phase;            /* f64 [pulses] or NULL */
+    int samples;
+    int tile_cols_log2;             /* leaf split-tile width, 3..6 */
+    double range_start, data_dr;
+    double k;                       /* carrier wavenumber of the merges */
+    /* Cartesian evaluation jobs (may be empty) */
+    const bpmi_eval_job* jobs;
+    int njobs;
+    /* work list: int32 [nitems, 4] = (kind, node or job, tile or first
+     * pixel, 0), every item after the items it depends on */
+    const int* items;
+    int nitems;
+    const int* node_tiles;          /* int32 [nodes]: items per node */
+    int nodes;
+    const int* job_root;            /* int32 [njobs]: root node of each job */
+    const int* job_blocks;          /* int32 [njobs]: evaluation items per job */
+    /* device workspace of bpmi_persistent_workspace_bytes(): the launch
+     * arguments, the ticket / abort / completion counters and the time
+     * stamps; written by the launch itself (a reset kernel enqueued first) */
+    void* workspace;
+    double max_wait_s;              /* bound of one dependency wait, seconds */
+} bpmi_persistent_schedule;
+int bpmi_persistent_workspace_bytes(int nodes, int njobs, size_t* bytes);
+/* Workgroups that fit on the device at once for this kernel (the default
+ * grid; any positive grid is correct, this one keeps every CU busy). */
+int bpmi_persistent_grid(int leaf_interp, int merge_interp, int* grid);
+/* Launch (grid <= 0: bpmi_persistent_grid()).  leaf_interp lagrange4 or
+ * linear, merge_interp lagrange4 or kaiser8.  Enqueues the workspace reset
+ * and the launch on ``stream``; never synchronizes.  Launches that share a
+ * workspace must not overlap. */
+int bpmi_ffbp_persistent(const bpmi_persistent_schedule* schedule, int leaf_interp,
+                         int merge_interp, int grid, void* stream);
+/* After the launch has completed (synchronous device-to-host reads):
+ * *aborted = 1 when a dependency wait ran out (the outputs are then
+ * incomplete); stamps[0] = first workgroup start, stamps[1 + j] = completion
+ * of evaluation job j, in ticks of the 100 MHz device wall clock. */
+int bpmi_persistent_aborted(const void* workspace, int nodes, int njobs, int* aborted);
+int bpmi_persistent_stamps(const void* workspace, int nodes, int njobs,
+                           unsigned long long* stamps);
+
 /* ---- phase-LUT direct BP (resident_precision.py:68-175) ---------------- */
 /* lut[i] = exp(j k (range_start + i data_dr)), complex64, FP64 evaluation
  * (fill_range_phase_lut, precision_probe.cu:106-115). */
