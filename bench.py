@@ -78,8 +78,8 @@ def parse():
     p.add_argument("--precision-study", action="store_true",
                    help="direct workloads: compare fp32/mixed/dfloat/split against fp64")
     p.add_argument("--persistent", action="store_true",
-                   help="FFBP workloads: leaves and all merge levels (C5: and the evaluation) "
-                        "in one persistent launch")
+                   help="FFBP workloads: one persistent launch for leaves and all merge levels "
+                        "(C5: and the evaluation)")
     return p.parse_args()
 
 
@@ -186,7 +186,7 @@ def run_ffbp(args, scene):
     work = {
         f"batched_bp_polar_{args.mode}": leaf_pp * OPS_LEAF_LAGRANGE4,
         f"batched_merge_regrid_{args.mode}": merge_px * ops_merge,
-        f"ffbp_persistent_{args.mode}": leaf_pp * OPS_LEAF_LAGRANGE4 + merge_px * ops_merge,
+        f"ffbp_persistent_{args.mode}": merge_px * ops_merge + leaf_pp * OPS_LEAF_LAGRANGE4,
         "eval_grid": g.nx * g.ny * OPS_EVAL_KAISER8,
     }
     per_stage = {}

@@ -91,10 +91,12 @@ class BatchedResidentProgram(ResidentProgram):
                  leaf_tile_cols_log2=None, persistent=False):
         if persistent and mode != "split":
             raise ValueError("the persistent schedule runs in split precision only")
+        if persistent not in (False, True, "merges"):
+            raise ValueError("persistent must be False, True or 'merges'")
         self._table_buffers = []
         self._leaf_tc_request = leaf_tile_cols_log2
         self._batch_stages = []
-        self._persistent_request = bool(persistent)
+        self._persistent_request = persistent
         self.persistent = None
         super().__init__(backend, plan, device, mode, range_start, data_dr, k,
                          algorithm="ffbp", variant="fused", chunk_pulses=chunk_pulses)
@@ -194,16 +196,19 @@ class BatchedResidentProgram(ResidentProgram):
                            arena._alloc, arena._offset + int(root) * 8)
 
     def _replace_with_persistent(self, flat, sources):
-        """Swap the leaf and merge launches for one persistent launch
-        (bpmi_ffbp_persistent) over the same tables and arena."""
+        """Swap the merge launches (persistent=True: and the leaf launch) for
+        one persistent launch (bpmi_ffbp_persistent) over the same tables and
+        arena."""
         from .persistent import PersistentFFBP
-        staged = len(flat.levels)
+        leaves = self._persistent_request is True
+        staged = len(flat.levels) - (0 if leaves else 1)
         del self._schedule[-staged:]
         del self.operations[-staged:]
         self.persistent = PersistentFFBP(
             flat.node_int, self._node_table.ptr, self._geometry_table.ptr, self._offset_table.ptr,
             self.arena.ptr, self.k, self.leaf_tile_cols_log2, sources=sources.ptr,
-            leaf_interp=self.backend.interp, merge_interp=self.backend.merge_interp)
+            leaf_interp=self.backend.interp, merge_interp=self.backend.merge_interp,
+            leaves=leaves)
         pf = self.persistent
         count = pf.nitems * MERGE_BLOCK
         self._enqueue("bpmi_ffbp_persistent", count, (), name=f"ffbp_persistent_{self.mode}",

@@ -1272,6 +1272,10 @@ static int persistent_grid_for(int* grid) {
 }
 
 static int persistent_interps(int leaf_interp, int merge_interp) {
+    if (leaf_interp == BPMI_INTERP_NONE)
+        return set_error(BPMI_ERR_UNSUPPORTED,
+                         "a persistent work list without leaves is switched off (its one GPU run "
+                         "did not finish; cause not found)");
     if (leaf_interp != BPMI_INTERP_LAGRANGE4 && leaf_interp != BPMI_INTERP_LINEAR)
         return set_error(BPMI_ERR_UNSUPPORTED, "persistent leaves support lagrange4 and linear");
     if (merge_interp != BPMI_INTERP_LAGRANGE4 && merge_interp != BPMI_INTERP_KAISER8)

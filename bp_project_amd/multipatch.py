@@ -87,7 +87,9 @@ class MultiPatchProgram:
             ensure_kaiser_table()
         if persistent and mode != "split":
             raise ValueError("the persistent schedule runs in split precision only")
-        self.persistent = bool(persistent)
+        if persistent not in (False, True, "merges"):
+            raise ValueError("persistent must be False, True or 'merges'")
+        self.persistent = persistent
         self._persistent = None
         self.mode = mode
         self.leaf_interp = leaf_interp
@@ -169,7 +171,7 @@ class MultiPatchProgram:
         self.launch_count = 2 + len(self.merge_tables)
         if self.persistent:
             self._build_persistent()
-            self.launch_count = 1
+            self.launch_count = 1 if self.persistent is True else 2
 
     def set_tile_width(self, tile_cols_log2):
         """(Re)build the leaf block table for a split-mode tile width."""
@@ -211,7 +213,8 @@ class MultiPatchProgram:
             self.node_int, self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
             self.arena.ptr, self.k, self.tile_cols_log2, sources=self.t_sources.ptr,
             jobs=self.jobs_host, job_roots=self.job_roots, leaf_interp=self.leaf_interp,
-            merge_interp=self.merge_interp, priority=root_height)
+            merge_interp=self.merge_interp, priority=root_height,
+            leaves=self.persistent is True)
 
     def completion_ms(self):
         """Persistent schedule, after a completed launch: each patch's
@@ -231,12 +234,11 @@ class MultiPatchProgram:
         mode = _lib.mode_code(self.mode)
         s = stream_handle()
         if self._persistent is not None:
+            if not self._persistent.leaves:
+                self._launch_leaves(mode, s)
             self._persistent.launch(s)
             return self.outputs
-        _lib.call("bpmi_ffbp_leaves_multi", _vp(self.t_sources.ptr), _vp(self.t_node_int.ptr),
-                  _vp(self.t_node_geo.ptr), _vp(self.t_offsets.ptr), _vp(self.arena.ptr),
-                  _vp(self.leaf_blocks.ptr), self.n_leaf_blocks, mode,
-                  _lib.interp_code(self.leaf_interp), self.tile_cols_log2, s)
+        self._launch_leaves(mode, s)
         for table in self.merge_tables:
             enqueue_merges(self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
                            self.arena.ptr, table.ptr, table.shape[0], self.k, self.mode,
@@ -244,6 +246,12 @@ class MultiPatchProgram:
         _lib.call("bpmi_eval_grid_multi", _vp(self.t_jobs.ptr), _vp(self.t_eval_blocks.ptr),
                   self.n_eval_blocks, mode, _lib.interp_code(self.merge_interp), s)
         return self.outputs
+
+    def _launch_leaves(self, mode, s):
+        _lib.call("bpmi_ffbp_leaves_multi", _vp(self.t_sources.ptr), _vp(self.t_node_int.ptr),
+                  _vp(self.t_node_geo.ptr), _vp(self.t_offsets.ptr), _vp(self.arena.ptr),
+                  _vp(self.leaf_blocks.ptr), self.n_leaf_blocks, mode,
+                  _lib.interp_code(self.leaf_interp), self.tile_cols_log2, s)
 
     def capture_graph(self):
         if self._graph is None:

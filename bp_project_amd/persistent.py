@@ -29,6 +29,7 @@ from .plan import NI_HEIGHT, NI_NR, NI_NU
 
 WORK_LEAF, WORK_MERGE, WORK_EVAL = 0, 1, 2
 EVAL_BLOCK = 256
+INTERP_NONE = -1  # BPMI_INTERP_NONE: a work list without leaf items
 TICKS_PER_MS = 1.0e5  # the device wall clock runs at 100 MHz
 
 
@@ -44,11 +45,13 @@ def _merge_tiles(nr, nu):
     return t.value
 
 
-def work_items(node_int, tile_cols_log2, jobs=(), priority=None):
+def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True):
     """Work list of a forest of FFBP trees.
 
     node_int: int32 [nodes, 8] node table (a, b, nr, nu, side, left, right,
-    height); jobs: sequence of (root node, output pixels) per evaluation job.
+    height); jobs: sequence of (root node, output pixels) per evaluation job;
+    leaves=False leaves the leaf tiles out (they are computed before the
+    launch; their completion targets are then 0).
     Returns (items int32 [n, 4], node_tiles int32 [nodes], job_blocks int32
     [jobs]).  Every item comes after the items it depends on.
     """
@@ -59,8 +62,10 @@ def work_items(node_int, tile_cols_log2, jobs=(), priority=None):
     node_tiles = np.zeros(nodes, np.int32)
     for n in range(nodes):
         nr, nu = int(node_int[n, NI_NR]), int(node_int[n, NI_NU])
-        node_tiles[n] = (_split_tiles(nr, nu, tile_cols_log2) if heights[n] == 0
-                         else _merge_tiles(nr, nu))
+        if heights[n] == 0:
+            node_tiles[n] = _split_tiles(nr, nu, tile_cols_log2) if leaves else 0
+        else:
+            node_tiles[n] = _merge_tiles(nr, nu)
     job_blocks = np.array([-(-int(px) // EVAL_BLOCK) for _, px in jobs], np.int32)
     job_height = [int(heights[root]) for root, _ in jobs]
     items = []
@@ -75,11 +80,15 @@ def work_items(node_int, tile_cols_log2, jobs=(), priority=None):
     return np.asarray(items, np.int32).reshape(-1, 4), node_tiles, job_blocks
 
 
-def check_order(items, node_int, job_root):
+def check_order(items, node_int, job_root, leaves=True):
     """True when every item follows all items it depends on (host check of a
-    work list; the kernel's deadlock freedom rests on it)."""
+    work list; the kernel's deadlock freedom rests on it).  leaves=False:
+    leaf nodes are complete before the launch."""
     node_int = np.asarray(node_int)
     last = {}
+    if not leaves:
+        for n in np.flatnonzero(node_int[:, NI_HEIGHT] == 0):
+            last[int(n)] = -1
     for i, (kind, idx, _, _) in enumerate(items):
         if kind != WORK_EVAL:
             last[int(idx)] = i
@@ -107,17 +116,23 @@ class PersistentFFBP:
     def __init__(self, node_int_host, node_int, node_geo, offsets, arena, k, tile_cols_log2, *,
                  sources=None, data=None, pos=None, weight=None, phase=None, samples=0,
                  range_start=0.0, data_dr=1.0, jobs=None, job_roots=(), leaf_interp="lagrange4",
-                 merge_interp="kaiser8", priority=None, max_wait_s=10.0, grid=0):
+                 merge_interp="kaiser8", priority=None, max_wait_s=10.0, grid=0, leaves=True):
+        if not leaves:
+            raise NotImplementedError(
+                "a persistent work list without leaves is switched off: its one GPU run did not "
+                "finish and the cause has not been found")
         node_int_host = np.asarray(node_int_host)
         self.nodes = len(node_int_host)
         self.njobs = 0 if jobs is None else len(jobs)
+        self.leaves = bool(leaves)
         job_px = [] if jobs is None else [int(j["nx"]) * int(j["ny"]) for j in jobs]
         items, node_tiles, job_blocks = work_items(node_int_host, tile_cols_log2,
-                                                   list(zip(job_roots, job_px)), priority)
-        if not check_order(items, node_int_host, job_roots):
+                                                   list(zip(job_roots, job_px)), priority,
+                                                   leaves=self.leaves)
+        if not check_order(items, node_int_host, job_roots, leaves=self.leaves):
             raise RuntimeError("persistent work list is not topologically ordered")
         self.items_host = items
-        self.leaf_interp = _lib.interp_code(leaf_interp)
+        self.leaf_interp = _lib.interp_code(leaf_interp) if self.leaves else INTERP_NONE
         self.merge_interp = _lib.interp_code(merge_interp)
         self.grid = int(grid)
         keep = []

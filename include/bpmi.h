@@ -45,6 +45,7 @@ extern "C" {
 #define BPMI_INTERP_LAGRANGE4 0 /* fast_bp/ffbp.py:17-40 */
 #define BPMI_INTERP_LINEAR 1
 #define BPMI_INTERP_KAISER8 2   /* 8-tap Kaiser(beta=6)-windowed sinc */
+#define BPMI_INTERP_NONE (-1)   /* persistent FFBP without leaves: rejected (switched off) */
 
 /* flags for BP outputs */
 #define BPMI_ACCUMULATE 1 /* out += result (else out = result) */
@@ -308,7 +309,9 @@ int bpmi_persistent_workspace_bytes(int nodes, int njobs, size_t* bytes);
  * grid; any positive grid is correct, this one keeps every CU busy). */
 int bpmi_persistent_grid(int leaf_interp, int merge_interp, int* grid);
 /* Launch (grid <= 0: bpmi_persistent_grid()).  leaf_interp lagrange4 or
- * linear, merge_interp lagrange4 or kaiser8.  Enqueues the workspace reset
+ * linear (BPMI_INTERP_NONE, a list without leaf items, returns
+ * BPMI_ERR_UNSUPPORTED: that variant is switched off); merge_interp
+ * lagrange4 or kaiser8.  Enqueues the workspace reset
  * and the launch on ``stream``; never synchronizes.  Launches that share a
  * workspace must not overlap. */
 int bpmi_ffbp_persistent(const bpmi_persistent_schedule* schedule, int leaf_interp,

@@ -22,13 +22,21 @@ def _single(merge_interp, persistent):
     return s, plan, prog
 
 
-@pytest.mark.parametrize("merge_interp", ["kaiser8", "lagrange4"])
-def test_single_plan_persistent_equals_level_launches(oracle, merge_interp):
+MERGES_ONLY_OFF = pytest.mark.skip(
+    reason="persistent list without leaves switched off: its one GPU run did not finish "
+           "(test timeout at its first launch), cause not found; the path raises")
+
+
+@pytest.mark.parametrize("merge_interp,persistent", [
+    ("kaiser8", True), ("lagrange4", True),
+    pytest.param("kaiser8", "merges", marks=MERGES_ONLY_OFF)])
+def test_single_plan_persistent_equals_level_launches(oracle, merge_interp, persistent):
     from bp_project_amd.device import synchronize
     s, plan, ref_prog = _single(merge_interp, False)
-    _, _, prog = _single(merge_interp, True)
+    _, _, prog = _single(merge_interp, persistent)
     print(f"{merge_interp}: {prog.persistent.nitems} work items, launches {prog.launch_count}")
-    assert prog.launch_count == ref_prog.launch_count - len(ref_prog.flat.levels) + 1
+    kept = 0 if persistent is True else 1
+    assert prog.launch_count == ref_prog.launch_count - len(ref_prog.flat.levels) + 1 + kept
     ref_prog.execute()
     synchronize()
     ref = ref_prog.output.get()
@@ -52,13 +60,14 @@ def _patches():
     return make_patches(patch_mix(count=10, scale=0.25))
 
 
-def test_multipatch_persistent_equals_level_launches_and_graph():
+@pytest.mark.parametrize("persistent", [True, pytest.param("merges", marks=MERGES_ONLY_OFF)])
+def test_multipatch_persistent_equals_level_launches_and_graph(persistent):
     from bp_project_amd.device import synchronize
     from bp_project_amd.multipatch import MultiPatchProgram
     scenes = _patches()
     ref_prog = MultiPatchProgram(scenes, mode="split", tile_cols_log2=5)
-    prog = MultiPatchProgram(scenes, mode="split", tile_cols_log2=5, persistent=True)
-    assert prog.launch_count == 1
+    prog = MultiPatchProgram(scenes, mode="split", tile_cols_log2=5, persistent=persistent)
+    assert prog.launch_count == (1 if persistent is True else 2)
     ref_prog.execute()
     synchronize()
     ref = [o.get() for o in ref_prog.outputs]

@@ -91,3 +91,23 @@ def test_persistent_entry_point_validation():
     sc.nitems = 4
     with pytest.raises(ValueError, match="null table"):
         _lib.call("bpmi_ffbp_persistent", ctypes.byref(sc), 0, 0, 0, None)
+
+
+def test_work_list_without_leaves():
+    plan, flat = _plan()
+    ni = flat.node_int
+    items, node_tiles, _ = work_items(ni, 6, leaves=False)
+    assert not (items[:, 0] == WORK_LEAF).any()
+    assert (node_tiles[ni[:, NI_HEIGHT] == 0] == 0).all()
+    assert check_order(items, ni, [], leaves=False)
+    assert not check_order(items, ni, [], leaves=True)
+
+
+def test_persistent_without_leaves_is_switched_off():
+    from bp_project_amd.persistent import PersistentFFBP
+    plan, flat = _plan()
+    with pytest.raises(NotImplementedError, match="switched off"):
+        PersistentFFBP(flat.node_int, 0, 0, 0, 0, 1.0, 6, leaves=False)
+    sc = _lib.PersistentSchedule()
+    with pytest.raises(_lib.BpmiError, match="switched off"):
+        _lib.call("bpmi_ffbp_persistent", ctypes.byref(sc), -1, 2, 0, None)
