@@ -46,7 +46,7 @@ PEAK_HBM_GBS = 8000.0
 # 1 GiB copy kernel (FETCH x 2.0, WRITE x 1.0); see
 # profiles/ffbp_c2_merge_counters.md.  Keyed by (workload, mode, stage).
 MEASURED_TRAFFIC_BYTES = {
-    ("c2_ffbp_full", "split", "batched_merge_regrid_split"): 8.40e9,
+    ("c2_ffbp_full", "split", "batched_merge_regrid_split"): 4.22e9,
 }
 # Algorithmic operation counts (add/sub/mul = 1, FMA = 2, sqrt/sin/cos = 1).
 OPS_LEAF_LAGRANGE4 = 57      # per leaf pixel*pulse (4-point Lagrange)
