@@ -67,7 +67,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="c2_ffbp_full",
                    choices=["c2_ffbp_full", "c1_direct_small", "c3_direct_precision",
-                            "c4_autofocus", "c5_multipatch"])
+                            "c4_autofocus", "c5_multipatch", "range_compress"])
     p.add_argument("--mode", default="split")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-quality", action="store_true", help="skip the FP64 quality gate")
@@ -492,6 +492,60 @@ def run_multipatch(args):
     }
 
 
+def run_range_compress(args):
+    """Range compression of an 8192-pulse x 8192-sample raw block (the C2 cube
+    before compression): 1 us, 200 MHz down-chirp echoes in noise at 400 MHz,
+    no oversampling.  Roofline: HBM, compulsory 16 B per raw sample (complex64
+    read and written once)."""
+    from bp_project_amd.device import DeviceArray, Event, elapsed_ms, synchronize
+    from bp_project_amd.sar_io import fft_length, range_compress_device
+    pulses, samples, tp = 8192, 8192, 1e-6
+    rng = np.random.default_rng(20260919)
+    raw = (rng.standard_normal((pulses, samples), np.float32)
+           + 1j * rng.standard_normal((pulses, samples), np.float32)).astype(np.complex64)
+    d = DeviceArray.from_host(raw)
+    out = DeviceArray.empty((pulses, samples))
+    nfft = fft_length(samples, tp)
+    for _ in range(args.warmup):
+        range_compress_device(d, tp, out=out)
+    synchronize()
+    a, b = Event(), Event()
+    a.record()
+    for _ in range(args.steps):
+        range_compress_device(d, tp, out=out)
+    b.record()
+    synchronize()
+    ms = elapsed_ms(a, b) / args.steps
+    bytes_ = 16.0 * pulses * samples
+    achieved = bytes_ / (ms * 1e-3) / 1e9
+    return {"ms_per_step": ms, "value": pulses * samples / (ms * 1e-3) / 1e6,
+            "unit": "Msample/s",
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                         "traffic": None, "kernel": "range compression pipeline",
+                         "unit_of_work": "raw sample", "bytes_per_unit": 16,
+                         "note": "compulsory bytes of the whole pipeline (mean/pad, forward "
+                                 "FFT, spectrum product, inverse FFT, crop) over its time"},
+            "config": {"workload": "range_compress", "pulses": pulses, "samples": samples,
+                       "tp_s": tp, "fs_hz": 400e6, "bandwidth_hz": 200e6, "nfft": nfft,
+                       "oversample": 1, "fft": "hipFFT (rocFFT) complex64"}}
+
+
+def cpu_baseline_range_compress():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    rng = np.random.default_rng(1)
+    rows, samples = 256, 8192
+    raw = (rng.standard_normal((rows, samples)) + 1j * rng.standard_normal((rows, samples)))
+    t = time.perf_counter()
+    oracle.range_compress(raw.astype(np.complex64))
+    dt = time.perf_counter() - t
+    return {"value": round(rows * samples / dt / 1e6, 3), "unit": "Msample/s", "cores": 1,
+            "kind": "port", "seconds": round(dt, 2),
+            "sample": f"FP64 oracle (scipy.fft, one worker) range compression of {rows} pulses "
+                      f"x {samples} samples"}
+
+
 def cpu_baseline_direct(scene, name):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -522,7 +576,8 @@ def main():
         sys.exit(1)
     from bp_project_amd.scenes import workload_scene
     t = time.perf_counter()
-    scene = None if args.workload == "c5_multipatch" else workload_scene(args.workload)
+    scene = (None if args.workload in ("c5_multipatch", "range_compress")
+             else workload_scene(args.workload))
     if scene is not None:
         log(f"scene {args.workload}: {scene.data.shape}, {len(scene.targets)} targets, "
             f"{time.perf_counter() - t:.1f} s")
@@ -536,13 +591,16 @@ def main():
     elif args.workload == "c5_multipatch":
         res = run_multipatch(args)
         cpu = None
+    elif args.workload == "range_compress":
+        res = run_range_compress(args)
+        cpu = None if args.no_cpu else cpu_baseline_range_compress()
     else:
         res = run_direct(args, scene, args.workload)
         cpu = None if args.no_cpu else cpu_baseline_direct(scene, args.workload)
     if cpu:
         log("cpu baseline:", json.dumps(cpu))
     line = {
-        "metric": METRIC, "value": round(res["value"], 3), "unit": "Gpix·pulse/s",
+        "metric": METRIC, "value": round(res["value"], 3), "unit": res.get("unit", "Gpix·pulse/s"),
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(res["ms_per_step"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",

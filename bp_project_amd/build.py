@@ -43,10 +43,16 @@ def flags():
             "-Wno-unused-command-line-argument", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 
+def libraries():
+    """hipFFT (rocFFT) for the range-compression FFTs."""
+    rocm = Path(hipcc()).resolve().parent.parent
+    return [f"-L{rocm / 'lib'}", "-lhipfft", f"-Wl,-rpath,{rocm / 'lib'}"]
+
+
 def build(verbose=False):
     """Compile every HIP source into libbpmi.so (always rebuilds)."""
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *flags(), *map(str, sources()), "-o", str(tmp)]
+    cmd = [hipcc(), *flags(), *map(str, sources()), *libraries(), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,0 +1,334 @@
+// Range compression and raw IQ decode: the step before backprojection
+// (fast_bp/sar_io.py:18-81 frame decode, :116-139 range_compress).
+//
+// Range compression is a linear correlation of each pulse with a centred,
+// Hamming-windowed LFM reference, computed with zero-padded complex64 FFTs
+// (hipFFT / rocFFT, batched over pulses) exactly as the reference's scipy
+// code: subtract the row mean, FFT to nfft = next_fast_len(samples + L - 1),
+// multiply by the FFT of conj(reversed reference), inverse FFT, crop
+// [L - 1 - L/2, + samples).  With oversample > 1 the length-nfft correlation
+// is Fourier-interpolated to nfft * oversample first (scipy.signal.resample:
+// spectrum copied to both ends, the +N/2 bin halved and mirrored to -N/2),
+// which here is done on the product spectrum directly (resample's forward
+// FFT of the inverse FFT is the identity).  The hand-written kernels do the
+// row mean and padding, the reference spectrum's samples, the spectrum
+// product / upsampling layout and the crop; the FFTs are hipFFT plans cached
+// per (size, batch) whose work area is the caller's workspace.
+//
+// The IQ decode gathers the int16 I/Q pairs of selected frames of a raw
+// capture (already in device memory) into complex64 rows.
+#include "bpmi_internal.h"
+
+#include <hipfft/hipfft.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+namespace bpmi {
+
+constexpr int RC_THREADS = 256;
+
+// One workgroup per pulse: FP64 row mean, then raw - mean into a row of
+// nfft samples padded with zeros.
+__global__ __launch_bounds__(RC_THREADS) void rc_center_pad_kernel(const float2* __restrict__ raw,
+                                                                    long long ld_raw, int samples,
+                                                                    int nfft,
+                                                                    float2* __restrict__ out) {
+    __shared__ double part[2][RC_THREADS / 64];
+    const long long p = blockIdx.x;
+    const float2* row = raw + p * ld_raw;
+    double sx = 0.0, sy = 0.0;
+    for (int i = threadIdx.x; i < samples; i += RC_THREADS) {
+        const float2 v = row[i];
+        sx += v.x;
+        sy += v.y;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][wave] = sx;
+        part[1][wave] = sy;
+    }
+    __syncthreads();
+    double mx = 0.0, my = 0.0;
+#pragma unroll
+    for (int w = 0; w < RC_THREADS / 64; ++w) {
+        mx += part[0][w];
+        my += part[1][w];
+    }
+    const float fx = (float)(mx / samples), fy = (float)(my / samples);
+    float2* dst = out + p * nfft;
+    for (int i = threadIdx.x; i < nfft; i += RC_THREADS) {
+        if (i < samples) {
+            const float2 v = row[i];
+            dst[i] = make_float2(v.x - fx, v.y - fy);
+        } else {
+            dst[i] = make_float2(0.f, 0.f);
+        }
+    }
+}
+
+// h[n] = conj(ref[L - 1 - n]) for n < L, zero up to nfft, where
+// ref[m] = exp(j pi s B / tp tr^2) w[m], tr = (m - floor(L/2)) / fs and w the
+// periodic Hamming window 0.54 - 0.46 cos(2 pi m / L) (sar_io.py:124-128);
+// phase and window in FP64, rounded to complex64 like the reference's arrays.
+__global__ void rc_filter_kernel(int L, double chirp_rate, double fs, int nfft,
+                                 float2* __restrict__ h) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nfft) return;
+    if (n >= L) {
+        h[n] = make_float2(0.f, 0.f);
+        return;
+    }
+    const int m = L - 1 - n;
+    const double tr = (double)(m - L / 2) / fs;
+    double s, c;
+    sincos(M_PI * chirp_rate * tr * tr, &s, &c);
+    const float w = (float)(0.54 - 0.46 * cos(2.0 * M_PI * (double)m / (double)L));
+    const float2 ref = make_float2((float)c * w, (float)s * w);
+    h[n] = make_float2(ref.x, -ref.y);
+}
+
+// Product spectrum X[k] H[k] laid out for an inverse FFT of length
+// num = N * os (os = 1: in place).  For os > 1: bins [0, N/2] at the start,
+// bins (N/2, N) at the end, the +N/2 bin (N even) halved and mirrored to
+// num - N/2, zeros between (scipy.signal.resample upsampling).
+__global__ void rc_multiply_kernel(const float2* __restrict__ X, const float2* __restrict__ H,
+                                   int N, int os, long long pulses, float2* __restrict__ Y) {
+    const long long num = (long long)N * os;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= pulses * num) return;
+    const long long p = idx / num;
+    const long long j = idx - p * num;
+    const float2* x = X + p * N;
+    long long src = -1;
+    float scale = 1.f;
+    if (os == 1) {
+        src = j;
+    } else {
+        const int nyq = N / 2 + 1;
+        if (j < nyq) {
+            src = j;
+            if ((N & 1) == 0 && j == N / 2) scale = 0.5f;
+        } else if (j >= num - (N - nyq)) {
+            src = j - num + N;
+        } else if ((N & 1) == 0 && j == num - N / 2) {
+            src = N / 2;
+            scale = 0.5f;
+        }
+    }
+    float2 v = make_float2(0.f, 0.f);
+    if (src >= 0) {
+        const float2 a = x[src], b = H[src];
+        v = make_float2((a.x * b.x - a.y * b.y) * scale, (a.x * b.y + a.y * b.x) * scale);
+    }
+    Y[idx] = v;
+}
+
+// out[p, i] = y[p, start + i] * scale, i < count.
+__global__ void rc_crop_kernel(const float2* __restrict__ y, long long ldy, int start, int count,
+                               float scale, long long pulses, float2* __restrict__ out) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= pulses * count) return;
+    const long long p = idx / count;
+    const long long i = idx - p * count;
+    const float2 v = y[p * ldy + start + i];
+    out[idx] = make_float2(v.x * scale, v.y * scale);
+}
+
+// Frame decode: out[i, s] = (I, Q) int16 pair s of frame index[i]; an index
+// outside [0, nframes) gives a zero row.
+__global__ void decode_iq16_kernel(const unsigned char* __restrict__ frames, long long nframes,
+                                   long long frame_bytes, long long iq_offset, int samples,
+                                   const long long* __restrict__ index, long long count,
+                                   float2* __restrict__ out) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count * samples) return;
+    const long long i = idx / samples;
+    const long long s = idx - i * samples;
+    const long long f = index[i];
+    float2 v = make_float2(0.f, 0.f);
+    if (f >= 0 && f < nframes) {
+        const short2 q = *(const short2*)(frames + f * frame_bytes + iq_offset + 4 * s);
+        v = make_float2((float)q.x, (float)q.y);
+    }
+    out[idx] = v;
+}
+
+// ---- hipFFT plans, cached per (length, batch) --------------------------------
+struct FftPlan {
+    hipfftHandle handle = nullptr;
+    size_t work = 0;
+};
+
+static std::mutex g_fft_mutex;
+static std::map<std::pair<int, int>, FftPlan> g_fft_plans;
+
+static int check_fft(hipfftResult r, const char* what) {
+    if (r == HIPFFT_SUCCESS) return BPMI_OK;
+    return set_error(BPMI_ERR_HIP, "%s failed (hipfft status %d)", what, (int)r);
+}
+
+// Batched in-place C2C plan of length n; the work area is set per call.
+static int fft_plan(int n, int batch, FftPlan** out) {
+    std::lock_guard<std::mutex> lock(g_fft_mutex);
+    auto key = std::make_pair(n, batch);
+    auto it = g_fft_plans.find(key);
+    if (it != g_fft_plans.end()) {
+        *out = &it->second;
+        return BPMI_OK;
+    }
+    FftPlan plan;
+    int st = check_fft(hipfftCreate(&plan.handle), "hipfftCreate");
+    if (st) return st;
+    st = check_fft(hipfftSetAutoAllocation(plan.handle, 0), "hipfftSetAutoAllocation");
+    int dims[1] = {n};
+    if (!st)
+        st = check_fft(hipfftMakePlanMany(plan.handle, 1, dims, nullptr, 1, n, nullptr, 1, n,
+                                          HIPFFT_C2C, batch, &plan.work),
+                       "hipfftMakePlanMany");
+    if (st) {
+        hipfftDestroy(plan.handle);
+        return st;
+    }
+    auto res = g_fft_plans.emplace(key, plan);
+    *out = &res.first->second;
+    return BPMI_OK;
+}
+
+static int fft_exec(FftPlan* plan, void* data, void* work, int direction, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_fft_mutex);
+    int st = check_fft(hipfftSetStream(plan->handle, stream), "hipfftSetStream");
+    if (!st) st = check_fft(hipfftSetWorkArea(plan->handle, work), "hipfftSetWorkArea");
+    if (!st)
+        st = check_fft(hipfftExecC2C(plan->handle, (hipfftComplex*)data, (hipfftComplex*)data,
+                                     direction),
+                       "hipfftExecC2C");
+    return st;
+}
+
+struct RcLayout {
+    size_t h, x, y, work, total;  // byte offsets in the workspace
+    FftPlan *filt, *fwd, *inv;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int rc_layout(int pulses, int samples, int L, int nfft, int os, RcLayout* lay) {
+    if (pulses <= 0 || samples <= 0) return set_error(BPMI_ERR_INVALID, "empty raw data");
+    if (L <= 0) return set_error(BPMI_ERR_INVALID, "reference length must be positive");
+    if (os < 1 || os > 64) return set_error(BPMI_ERR_INVALID, "oversample must be 1..64");
+    if (nfft < samples + L - 1)
+        return set_error(BPMI_ERR_INVALID, "nfft %d below samples + L - 1 = %d (circular wrap)",
+                         nfft, samples + L - 1);
+    if ((long long)nfft * os > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "nfft too large");
+    int st = fft_plan(nfft, 1, &lay->filt);
+    if (!st) st = fft_plan(nfft, pulses, &lay->fwd);
+    if (!st) st = fft_plan(nfft * os, pulses, &lay->inv);
+    if (st) return st;
+    const size_t c64 = 8;
+    lay->h = 0;
+    lay->x = align256((size_t)nfft * c64);
+    lay->y = lay->x + align256((size_t)pulses * nfft * c64);
+    lay->work = lay->y + (os > 1 ? align256((size_t)pulses * nfft * os * c64) : 0);
+    size_t w = lay->filt->work;
+    if (lay->fwd->work > w) w = lay->fwd->work;
+    if (lay->inv->work > w) w = lay->inv->work;
+    lay->total = lay->work + align256(w);
+    return BPMI_OK;
+}
+
+static int reference_length(double tp, double fs, int* L) {
+    const double l = tp * fs;
+    if (!(l >= 0.5) || l > 1.0e8) return set_error(BPMI_ERR_INVALID, "bad pulse width x rate");
+    *L = (int)nearbyint(l);
+    return BPMI_OK;
+}
+
+}  // namespace bpmi
+
+using namespace bpmi;
+
+extern "C" {
+
+int bpmi_range_compress_workspace_bytes(int pulses, int samples, double tp, double fs, int nfft,
+                                        int oversample, size_t* bytes) {
+    if (!bytes) return set_error(BPMI_ERR_INVALID, "null bytes");
+    int L = 0;
+    int st = reference_length(tp, fs, &L);
+    if (st) return st;
+    RcLayout lay;
+    st = rc_layout(pulses, samples, L, nfft, oversample, &lay);
+    if (st) return st;
+    *bytes = lay.total;
+    return BPMI_OK;
+}
+
+int bpmi_range_compress(const void* raw, int pulses, int samples, long long ld_raw, double tp,
+                        double fs, double bandwidth, int chirp_sign, int nfft, int oversample,
+                        void* out, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!raw || !out || !workspace) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (ld_raw < samples) return set_error(BPMI_ERR_INVALID, "ld_raw below samples");
+    if (chirp_sign != 1 && chirp_sign != -1)
+        return set_error(BPMI_ERR_INVALID, "chirp_sign must be +1 or -1");
+    if (!(bandwidth > 0.0) || !(fs > 0.0) || !(tp > 0.0))
+        return set_error(BPMI_ERR_INVALID, "tp, fs and bandwidth must be positive");
+    int L = 0;
+    int st = reference_length(tp, fs, &L);
+    if (st) return st;
+    RcLayout lay;
+    st = rc_layout(pulses, samples, L, nfft, oversample, &lay);
+    if (st) return st;
+    if (workspace_bytes < lay.total)
+        return set_error(BPMI_ERR_INVALID, "workspace of %zu bytes below the %zu needed",
+                         workspace_bytes, lay.total);
+    char* ws = (char*)workspace;
+    float2* H = (float2*)(ws + lay.h);
+    float2* X = (float2*)(ws + lay.x);
+    float2* Y = oversample > 1 ? (float2*)(ws + lay.y) : X;
+    void* work = ws + lay.work;
+    hipStream_t hs = (hipStream_t)stream;
+    const long long num = (long long)nfft * oversample;
+    hipLaunchKernelGGL(rc_filter_kernel, dim3(grid_1d(nfft, 256)), dim3(256), 0, hs, L,
+                       (double)chirp_sign * bandwidth / tp, fs, nfft, H);
+    st = check_launch("rc_filter_kernel");
+    if (!st) st = fft_exec(lay.filt, H, work, HIPFFT_FORWARD, hs);
+    if (st) return st;
+    hipLaunchKernelGGL(rc_center_pad_kernel, dim3(pulses), dim3(RC_THREADS), 0, hs,
+                       (const float2*)raw, ld_raw, samples, nfft, X);
+    st = check_launch("rc_center_pad_kernel");
+    if (!st) st = fft_exec(lay.fwd, X, work, HIPFFT_FORWARD, hs);
+    if (st) return st;
+    hipLaunchKernelGGL(rc_multiply_kernel, dim3(grid_1d((long long)pulses * num, 256)), dim3(256),
+                       0, hs, X, H, nfft, oversample, (long long)pulses, Y);
+    st = check_launch("rc_multiply_kernel");
+    if (!st) st = fft_exec(lay.inv, Y, work, HIPFFT_BACKWARD, hs);
+    if (st) return st;
+    const int offset = (L - 1 - L / 2) * oversample;
+    const int count = samples * oversample;
+    hipLaunchKernelGGL(rc_crop_kernel, dim3(grid_1d((long long)pulses * count, 256)), dim3(256),
+                       0, hs, Y, num, offset, count, (float)(1.0 / nfft), (long long)pulses,
+                       (float2*)out);
+    return check_launch("rc_crop_kernel");
+}
+
+int bpmi_decode_iq16(const void* frames, long long nframes, long long frame_bytes,
+                     long long iq_offset, int samples, const long long* frame_index,
+                     long long count, void* out, void* stream) {
+    if (!frames || !frame_index || !out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (samples <= 0 || count < 0 || nframes < 0) return set_error(BPMI_ERR_INVALID, "bad sizes");
+    if (frame_bytes % 4 || iq_offset % 4 || iq_offset < 0 ||
+        iq_offset + 4LL * samples > frame_bytes)
+        return set_error(BPMI_ERR_INVALID, "IQ block must be 4-byte aligned inside the frame");
+    if (count == 0) return BPMI_OK;
+    hipLaunchKernelGGL(decode_iq16_kernel, dim3(grid_1d(count * samples, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const unsigned char*)frames, nframes, frame_bytes,
+                       iq_offset, samples, frame_index, count, (float2*)out);
+    return check_launch("decode_iq16_kernel");
+}
+
+}  // extern "C"

@@ -336,6 +336,33 @@ int bpmi_bp_polar_lut(const void* data, int samples, const double* pos, const do
                       int nr, int nu, int side, double range_start, double data_dr, double k,
                       const void* lut, void* out, void* stream);
 
+/* ---- range compression and raw IQ decode (the step before BP) ---------- */
+/* Matched filter of fast_bp/sar_io.py:116-139 (range_compress): each pulse
+ * of raw complex64 [pulses, ld_raw] (first ``samples`` valid) minus its mean,
+ * correlated with the centred LFM reference exp(j pi chirp_sign B/tp tr^2)
+ * times a periodic Hamming window (L = round(tp fs) samples), through
+ * zero-padded complex64 FFTs of length nfft (>= samples + L - 1; the
+ * reference uses scipy.fft.next_fast_len(samples + L - 1)); oversample > 1
+ * Fourier-interpolates the correlation (scipy.signal.resample) before the
+ * crop.  out complex64 [pulses, samples * oversample]: sample k is the pulse
+ * centred at raw sample k / oversample.  FFTs run on hipFFT plans cached per
+ * size (their twiddle tables are allocated when a plan is first made); the
+ * FFT work area and every intermediate are in the caller's workspace of
+ * bpmi_range_compress_workspace_bytes() bytes (which makes the plans). */
+int bpmi_range_compress_workspace_bytes(int pulses, int samples, double tp, double fs, int nfft,
+                                        int oversample, size_t* bytes);
+int bpmi_range_compress(const void* raw, int pulses, int samples, long long ld_raw, double tp,
+                        double fs, double bandwidth, int chirp_sign, int nfft, int oversample,
+                        void* out, void* workspace, size_t workspace_bytes, void* stream);
+/* Raw capture frames (fast_bp/sar_io.py:18-81 Capture.read): ``frames`` holds
+ * nframes frames of frame_bytes bytes in device memory; row i of out
+ * (complex64 [count, samples]) is the int16 I/Q block at byte iq_offset of
+ * frame frame_index[i] (device int64 array; an index outside [0, nframes)
+ * gives zeros). */
+int bpmi_decode_iq16(const void* frames, long long nframes, long long frame_bytes,
+                     long long iq_offset, int samples, const long long* frame_index,
+                     long long count, void* out, void* stream);
+
 /* ---- Kaiser table and probes ------------------------------------------- */
 /* Upload the Kaiser-sinc kernel table used by BPMI_INTERP_KAISER8: host
  * array of BPMI_KAISER_TABLE_LEN doubles, the kernel at x = i/2048.  FP64

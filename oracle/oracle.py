@@ -470,3 +470,71 @@ def np_regrid(img, src_origin, sr0, sdr, su0, sdu, dst_origin, r0, dr, nr, u0, d
     uu = dy / g
     v = np_sample2(img, (rr - sr0) / sdr, (uu - su0) / sdu)
     return v * np.exp(1j * k * (rr - rr_))
+
+
+# ---------------------------------------------------------------------------
+# Range compression (the step before BP), FP64 restatement of
+# fast_bp/sar_io.py:116-139.  Pinned by the reference's fractional-delay
+# known answer (fast_bp/validate.py:55-59: a 1 us, 200 MHz chirp centred at
+# sample 1512.4 compresses, at 4x Fourier oversampling, to a peak at 1512.5 --
+# results/synthetic_validation.json ``lfm_peak_sample``) and by closed forms
+# (a chirp centred on an integer sample peaks there at the compressed energy).
+# ---------------------------------------------------------------------------
+
+FS_REF = 400e6
+
+
+def lfm_reference(tp, bandwidth, chirp_sign, fs=FS_REF):
+    """Centred LFM reference with a periodic Hamming window (sar_io.py:124-128),
+    complex128: samples at (n - floor(L/2)) / fs, L = round(tp fs)."""
+    from scipy.signal.windows import hamming
+    length = int(round(tp * fs))
+    tr = (np.arange(length) - length // 2) / fs
+    ref = np.exp(1j * np.pi * chirp_sign * bandwidth / tp * tr ** 2)
+    return ref * hamming(length, sym=False)
+
+
+def range_compress(raw, tp=1e-6, bandwidth=200e6, chirp_sign=-1, oversample=1, fs=FS_REF,
+                   nfft=None):
+    """Linear correlation with the centred reference via zero-padded FFTs
+    (sar_io.py:129-139), complex128.  Sample k of the output is the received
+    pulse centred at raw sample k; ``oversample`` > 1 Fourier-interpolates the
+    length-nfft correlation (scipy.signal.resample semantics, the +N/2 bin
+    split between +N/2 and -N/2) before the same crop."""
+    from scipy import fft, signal
+    raw = np.asarray(raw, np.complex128)
+    ref = lfm_reference(tp, bandwidth, chirp_sign, fs)
+    length = len(ref)
+    if nfft is None:
+        nfft = fft.next_fast_len(raw.shape[1] + length - 1)
+    h = np.conj(ref[::-1])
+    spectrum = fft.fft(raw - raw.mean(axis=1, keepdims=True), nfft, axis=1)
+    spectrum *= fft.fft(h, nfft)[None, :]
+    offset = length - 1 - length // 2
+    corr = fft.ifft(spectrum, axis=1)
+    if oversample == 1:
+        return corr[:, offset:offset + raw.shape[1]].copy()
+    corr = signal.resample(corr, nfft * oversample, axis=1)
+    return corr[:, offset * oversample:(offset + raw.shape[1]) * oversample].copy()
+
+
+def lfm_echo(samples, centre, tp=1e-6, bandwidth=200e6, fs=FS_REF):
+    """Baseband LFM echo of the validate.py KAT (validate.py:55-56): the
+    down-chirp exp(-j pi B/tp t^2), |t| < tp/2, centred at fractional sample
+    ``centre``; complex64 [1, samples]."""
+    tr = (np.arange(samples) - centre) / fs
+    x = np.exp(-1j * np.pi * bandwidth / tp * tr ** 2) * (np.abs(tr) < tp / 2)
+    return x.astype(np.complex64)[None, :]
+
+
+def decode_frames(path, offset, frame_size, nr, frame_index):
+    """I/Q of capture frames (fast_bp/sar_io.py:79-81 Capture.read) read with
+    plain byte arithmetic: frame f starts at offset + f * frame_size, its
+    int16 I/Q pairs at +160; complex64 [len(frame_index), nr]."""
+    data = np.fromfile(path, dtype=np.uint8)
+    out = np.zeros((len(frame_index), nr), np.complex64)
+    for row, f in enumerate(frame_index):
+        start = offset + int(f) * frame_size + 160
+        iq = data[start:start + 4 * nr].view("<i2").reshape(nr, 2).astype(np.float32)
+        out[row] = iq[:, 0] + 1j * iq[:, 1]
+    return out
