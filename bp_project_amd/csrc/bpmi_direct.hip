@@ -376,19 +376,33 @@ __device__ __forceinline__ void sample1_with_derivative(const float2* __restrict
     }
 }
 
+// Per pixel and pulse: FP64 range from the hardware reciprocal square root
+// with one Newton step (which also gives 1/d), the sample coordinate by a
+// multiplication with 1/data_dr, the phase reduced with v_fract_f64.  Per
+// pulse the workgroup needs five sums over its 1024 pixels; rather than five
+// cross-lane reductions per pulse, each thread keeps the sums of its four
+// pixels for ADJ_NB pulses in registers and the workgroup reduces them
+// through LDS once per ADJ_NB pulses (each of the ADJ_NB * 5 rows summed by
+// four threads in a fixed order, then across those four lanes).
+constexpr int ADJ_NB = 4;
+constexpr int ADJ_VALS = 5;
+constexpr int ADJ_ROWS = ADJ_NB * ADJ_VALS;
+
 template <int INTERP, int SRC>
 __global__ __launch_bounds__(ADJ_THREADS) void bp_adjoint_kernel(BpArgs a, SrcArgs s,
                                                                  const float2* __restrict__ grad,
                                                                  float* __restrict__ partials) {
-    __shared__ float red[4][8];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    static_assert(ADJ_ROWS * 4 <= ADJ_THREADS, "four reducing threads per row");
+    __shared__ float red[ADJ_ROWS][ADJ_THREADS + 1];
+    const int t = threadIdx.x;
     const long long base = (long long)blockIdx.x * ADJ_TILE;
     const long long count = src_count(s, SRC);
     const int S = a.samples;
     const int P = a.pulse_end - a.pulse_begin;
     const double kt = a.k * INV_TWO_PI;
+    const double inv_ddr = 1.0 / a.data_dr;
     const float k32 = (float)a.k;
-    const float invdr = (float)(1.0 / a.data_dr);
+    const float invdr = (float)inv_ddr;
     double px[ADJ_PPT], py[ADJ_PPT], pz[ADJ_PPT];
     float2 g[ADJ_PPT];
     bool val[ADJ_PPT];
@@ -400,55 +414,64 @@ __global__ __launch_bounds__(ADJ_THREADS) void bp_adjoint_kernel(BpArgs a, SrcAr
         src_pixel64<SRC>(s, val[k] ? p : 0, &px[k], &py[k], &pz[k], &r);
         g[k] = val[k] ? grad[p] : make_float2(0.f, 0.f);
     }
-    for (int n = a.pulse_begin; n < a.pulse_end; ++n) {
-        const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
-        const float2* row = a.data + (size_t)n * S;
-        const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
-        const float w = (float)pulse_weight(a.weight, n);
-        float are = 0.f, aim = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int n0 = a.pulse_begin; n0 < a.pulse_end; n0 += ADJ_NB) {
+        const int nb = min(ADJ_NB, a.pulse_end - n0);
+        float acc[ADJ_NB][ADJ_VALS];
 #pragma unroll
-        for (int k = 0; k < ADJ_PPT; ++k) {
-            if (!val[k]) continue;
-            const double ex = px[k] - ax, ey = py[k] - ay, ez = pz[k] - az;
-            const double d = sqrt(ex * ex + ey * ey + ez * ez);
-            const double q = (d - a.range_start) / a.data_dr;
-            int i;
-            if (!floor_index(q, &i)) continue;
-            float2 v, dv;
-            sample1_with_derivative<INTERP>(row, S, i, (float)(q - (double)i), &v, &dv);
-            float sn, cs;
-            sincos_turns((float)fract64(kt * d + ph), &sn, &cs);
-            // conj(g) * w e^{j theta}
-            const float hr = w * (g[k].x * cs + g[k].y * sn);
-            const float hi = w * (g[k].x * sn - g[k].y * cs);
-            // A += conj(g) w e^{j theta} v
-            are += hr * v.x - hi * v.y;
-            aim += hr * v.y + hi * v.x;
-            // b = conj(g) w e^{j theta} (dv/dr + j k v) / d ; only Re(b) is needed
-            const float ur = dv.x * invdr - k32 * v.y;
-            const float ui = dv.y * invdr + k32 * v.x;
-            const float invd = (float)(1.0 / d);
-            const float br = (hr * ur - hi * ui) * invd;
-            // (p_n - x) = -(ex, ey, ez)
-            gx -= br * (float)ex;
-            gy -= br * (float)ey;
-            gz -= br * (float)ez;
-        }
-        float vals[5] = {are, aim, gx, gy, gz};
+        for (int j = 0; j < ADJ_NB; ++j) {
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            float v = vals[j];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            vals[j] = v;
-        }
-        if (lane == 0) {
+            for (int v = 0; v < ADJ_VALS; ++v) acc[j][v] = 0.f;
+            if (j >= nb) continue;
+            const int n = n0 + j;
+            const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
+            const float2* row = a.data + (size_t)n * S;
+            const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
+            const float w = (float)pulse_weight(a.weight, n);
 #pragma unroll
-            for (int j = 0; j < 5; ++j) red[wave][j] = vals[j];
+            for (int k = 0; k < ADJ_PPT; ++k) {
+                if (!val[k]) continue;
+                const double ex = px[k] - ax, ey = py[k] - ay, ez = pz[k] - az;
+                const double s2 = ex * ex + ey * ey + ez * ez;
+                const double rinv = rsqrt64(s2);
+                const double d = s2 * rinv;
+                const double q = (d - a.range_start) * inv_ddr;
+                int i;
+                if (!floor_index(q, &i)) continue;
+                float2 v, dv;
+                sample1_with_derivative<INTERP>(row, S, i, (float)(q - (double)i), &v, &dv);
+                float sn, cs;
+                sincos_turns((float)hw_fract64(kt * d + ph), &sn, &cs);
+                // conj(g) * w e^{j theta}
+                const float hr = w * (g[k].x * cs + g[k].y * sn);
+                const float hi = w * (g[k].x * sn - g[k].y * cs);
+                // A += conj(g) w e^{j theta} v
+                acc[j][0] += hr * v.x - hi * v.y;
+                acc[j][1] += hr * v.y + hi * v.x;
+                // b = conj(g) w e^{j theta} (dv/dr + j k v) / d ; only Re(b) is needed
+                const float ur = dv.x * invdr - k32 * v.y;
+                const float ui = dv.y * invdr + k32 * v.x;
+                const float br = (hr * ur - hi * ui) * (float)rinv;
+                // (p_n - x) = -(ex, ey, ez)
+                acc[j][2] -= br * (float)ex;
+                acc[j][3] -= br * (float)ey;
+                acc[j][4] -= br * (float)ez;
+            }
         }
+#pragma unroll
+        for (int j = 0; j < ADJ_NB; ++j)
+#pragma unroll
+            for (int v = 0; v < ADJ_VALS; ++v) red[j * ADJ_VALS + v][t] = acc[j][v];
         __syncthreads();
-        if (t < 5) {
-            const float sum = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-            partials[((size_t)blockIdx.x * P + (n - a.pulse_begin)) * 5 + t] = sum;
+        if (t < 4 * ADJ_ROWS) {
+            const int r = t >> 2, part = t & 3;
+            float sum = 0.f;
+            for (int c = part; c < ADJ_THREADS; c += 4) sum += red[r][c];
+            sum += __shfl_xor(sum, 1, 4);
+            sum += __shfl_xor(sum, 2, 4);
+            const int j = r / ADJ_VALS;
+            if (part == 0 && j < nb)
+                partials[((size_t)blockIdx.x * P + (n0 + j - a.pulse_begin)) * ADJ_VALS +
+                         (r - j * ADJ_VALS)] = sum;
         }
         __syncthreads();
     }
