@@ -45,8 +45,9 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by its reading for a
 # 1 GiB copy kernel (FETCH x 2.0, WRITE x 1.0); see
 # profiles/ffbp_c2_merge_counters.md.  Keyed by (workload, mode, stage).
-MEASURED_TRAFFIC_BYTES = {
-    ("c2_ffbp_full", "split", "batched_merge_regrid_split"): 4.22e9,
+MEASURED_TRAFFIC_BYTES = {  # (workload, mode, kernel, cull)
+    ("c2_ffbp_full", "split", "batched_merge_regrid_split", "none"): 4.22e9,
+    ("c2_ffbp_full", "split", "batched_merge_regrid_split", "grid"): 3.63e9,
 }
 # Algorithmic operation counts (add/sub/mul = 1, FMA = 2, sqrt/sin/cos = 1).
 OPS_LEAF_LAGRANGE4 = 57      # per leaf pixel*pulse (4-point Lagrange)
@@ -77,6 +78,11 @@ def parse():
     p.add_argument("--merge-interp", default="kaiser8")
     p.add_argument("--precision-study", action="store_true",
                    help="direct workloads: compare fp32/mixed/dfloat/split against fp64")
+    p.add_argument("--cull", default="grid", choices=["none", "crop", "grid"],
+                   help="C2: launch only the tiles the root crop ('crop') or the Cartesian "
+                        "grid's footprint in it ('grid') depends on (output bitwise unchanged)")
+    p.add_argument("--no-unculled", action="store_true",
+                   help="C2 with culling: skip timing the unculled step beside it")
     p.add_argument("--persistent", action="store_true",
                    help="FFBP workloads: one persistent launch for leaves and all merge levels "
                         "(C5: and the evaluation)")
@@ -124,9 +130,12 @@ def ffbp_workload(args, scene):
     dev = backend.upload(s.data, s.pos, s.weight, s.phase, plan)
     vp = _lib.c_void_p
 
-    def make(mode):
+    def make(mode, cull_region=None):
+        cull = None
+        if mode == "split" and cull_region not in (None, "none"):
+            cull = "crop" if cull_region == "crop" else g
         prog = BatchedResidentProgram(backend, plan, dev, mode, s.r_start, s.data_dr, s.k,
-                                      persistent=args.persistent and mode == "split")
+                                      persistent=args.persistent and mode == "split", cull=cull)
         cart = DeviceArray.empty(g.shape)
         geo = geo_array(plan)
         interp = _lib.interp_code(args.merge_interp)
@@ -161,7 +170,7 @@ def run_ffbp(args, scene):
 
     plan, backend, dev, make = ffbp_workload(args, scene)
     g = scene.grid
-    prog, cart, step, _ = make(args.mode)
+    prog, cart, step, _ = make(args.mode, args.cull)
     for _ in range(args.warmup):
         step()
     synchronize()
@@ -180,7 +189,10 @@ def run_ffbp(args, scene):
     # per-stage algorithmic work
     flat = prog.flat
     nodes_px = flat.node_int[:, 2].astype(np.int64) * flat.node_int[:, 3]
-    leaf_pp = int(plan.counts["leaf_pixel_pulse_products"])
+    if prog.computed_pixels is not None:
+        nodes_px = np.array([prog.computed_pixels[i] for i in range(flat.node_count)], np.int64)
+    spans = (flat.node_int[:, 1] - flat.node_int[:, 0]).astype(np.int64)
+    leaf_pp = int(sum(nodes_px[i] * spans[i] for i in flat.levels[0]))
     merge_px = int(sum(nodes_px[i] for lvl in flat.levels[1:] for i in lvl))
     ops_merge = OPS_MERGE_KAISER8 if args.merge_interp == "kaiser8" else OPS_MERGE_LAGRANGE4
     work = {
@@ -206,7 +218,8 @@ def run_ffbp(args, scene):
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": MEASURED_TRAFFIC_BYTES.get((args.workload, args.mode, dominant)),
+            "traffic": MEASURED_TRAFFIC_BYTES.get((args.workload, args.mode, dominant,
+                                                   args.cull)),
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                             "copy-kernel calibrated)",
             "kernel": dominant,
@@ -231,8 +244,33 @@ def run_ffbp(args, scene):
             "pixel_m": g.dx, "mode": args.mode, "root_polar": [plan.root.nr, plan.root.nu],
             "persistent": bool(args.persistent),
             "arena_gb": round(flat.arena_elems * 8 / 1e9, 2),
+            "cull": args.cull,
         },
     }
+    if prog.cull is not None:
+        result["cull"] = dict(prog.cull, computed_fraction=round(
+            prog.cull["computed_pixels"] / prog.cull["node_pixels"], 4))
+    if prog.cull is not None and not args.no_unculled:
+        # the same step without culling: its time, and the outputs compared bit for bit
+        full_prog, full_cart, full_step, _ = make(args.mode)
+        for _ in range(args.warmup):
+            full_step()
+        synchronize()
+        start.record()
+        for _ in range(args.steps):
+            full_step()
+        end.record()
+        synchronize()
+        full_ms = elapsed_ms(start, end) / args.steps
+        step()
+        synchronize()
+        same = bool(np.array_equal(cart.get(), full_cart.get()))
+        full_prog.close()
+        result["cull"].update(unculled_ms_per_step=round(full_ms, 4),
+                              output_bitwise_equal_unculled=same)
+        log("cull:", json.dumps(result["cull"]))
+        if not same:
+            raise SystemExit("culled output differs from the unculled output")
     if not args.no_quality:
         img = cart.get()
         ref_prog, ref_cart, ref_step, _ = make("fp64")
@@ -608,7 +646,7 @@ def main():
     }
     if "quality" in res:
         line["quality"] = res["quality"]
-    for extra in ("autofocus", "multipatch"):
+    for extra in ("autofocus", "multipatch", "cull"):
         if extra in res:
             line[extra] = res[extra]
     if "precision_study" in res:

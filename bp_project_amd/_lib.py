@@ -79,6 +79,9 @@ SIGNATURES = {
     "bpmi_split_tiles_ex": [_i, _i, _i, POINTER(c_int)],
     "bpmi_ffbp_merges_tiled": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _vp],
     "bpmi_merge_tiles": [_i, _i, POINTER(c_int)],
+    "bpmi_ffbp_required_tiles": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "bpmi_eval_required_tiles": [_vp, _i, _i, _i, _i, _i, _i, _d, _d, _i, _d, _d, _i, _i, _vp,
+                                 _vp],
     "bpmi_ffbp_leaves_multi": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp],
     "bpmi_eval_grid_multi": [_vp, _vp, _i, _i, _i, _vp],
     "bpmi_fill_range_phase_lut": [_i, _d, _d, _d, _vp, _vp],

@@ -10,6 +10,11 @@ In ``split`` mode the leaf launch uses the LDS-tiled kernel with each leaf's
 origin as the geometry reference (1024-pixel tiles of 2^tc columns, tc from
 ``leaf_tile_width`` unless given; block table of (node, tile) pairs); the other modes use one thread per pixel with blocks of
 256 pixels, like the reference's 128-pixel block table.
+
+``cull="crop"`` (or a Cartesian grid) launches only the tiles that the root
+crop (or the grid evaluation's footprint in it) depends on, found on the
+device by ``culling.required_masks``; the output region is bitwise
+unchanged and the pixels of other tiles stay zero.
 """
 from __future__ import annotations
 
@@ -18,7 +23,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .device import DeviceArray
+from .device import DeviceArray, synchronize
 from .gpu_backend import leaf_tile_width
 from .plan import NI_NR, NI_NU, flatten_plan
 from .resident_backend import ResidentProgram
@@ -61,12 +66,14 @@ def xcd_order(entries, xcds=XCDS, pad=(-1, 0)):
     return out
 
 
-def merge_block_table(node_int, nodes, mode):
-    """(node, tile) pairs for split mode (tiled merges, in die order), else
-    (node, first pixel) pairs of MERGE_BLOCK pixels."""
+def merge_block_table(node_int, nodes, mode, tiles=None):
+    """(node, tile) pairs for split mode (tiled merges, in die order; only
+    ``tiles[node]`` when given), else (node, first pixel) pairs of
+    MERGE_BLOCK pixels."""
     if mode == "split":
-        return xcd_order([(int(n), t) for n in nodes
-                          for t in range(merge_tiles(node_int[n, NI_NR], node_int[n, NI_NU]))])
+        return xcd_order([(int(n), int(t)) for n in nodes
+                          for t in (tiles[n] if tiles is not None else
+                                    range(merge_tiles(node_int[n, NI_NR], node_int[n, NI_NU])))])
     return [(int(n), s) for n in nodes
             for s in range(0, int(node_int[n, NI_NR]) * int(node_int[n, NI_NU]), MERGE_BLOCK)]
 
@@ -88,11 +95,16 @@ class BatchedResidentProgram(ResidentProgram):
     """ResidentProgram-compatible FFBP batched across each tree level."""
 
     def __init__(self, backend, plan, device, mode, range_start, data_dr, k, chunk_pulses=64,
-                 leaf_tile_cols_log2=None, persistent=False):
+                 leaf_tile_cols_log2=None, persistent=False, cull=None):
         if persistent and mode != "split":
             raise ValueError("the persistent schedule runs in split precision only")
         if persistent not in (False, True, "merges"):
             raise ValueError("persistent must be False, True or 'merges'")
+        if cull is not None and (mode != "split" or persistent):
+            raise ValueError("culling applies to the split-mode level launches only")
+        self._cull_request = cull
+        self.cull = None
+        self.computed_pixels = None  # {node: pixels computed} when culled
         self._table_buffers = []
         self._leaf_tc_request = leaf_tile_cols_log2
         self._batch_stages = []
@@ -110,6 +122,7 @@ class BatchedResidentProgram(ResidentProgram):
             "total_owned_device_bytes": self.metadata["total_owned_device_bytes"] + table_bytes,
             "batch_mapping": "one workgroup per node pixel tile; nodes grouped by subtree height",
             "fusion": "each parent's two child regrids rounded to complex64 then FP32 add",
+            "cull": self.cull,
         })
 
     def _table(self, host):
@@ -127,16 +140,23 @@ class BatchedResidentProgram(ResidentProgram):
         self._offset_table = self._table(flat.offsets.astype(np.int64))
         d = self.device
         mode = _lib.mode_code(self.mode)
+        tiles = computed = None
+        if self.mode == "split":
+            tc = self._leaf_tc_request
+            if tc is None:
+                tc = leaf_tile_width(self.plan, d._host_position, self.data_dr)
+            tc = self.leaf_tile_cols_log2 = int(tc)
+            if self._cull_request is not None:
+                tiles, computed = self._cull(flat, arena, tc)
         for height, nodes in enumerate(flat.levels):
             if height == 0:
                 if self.mode == "split":
-                    tc = self._leaf_tc_request
-                    if tc is None:
-                        tc = leaf_tile_width(self.plan, d._host_position, self.data_dr)
-                    tc = self.leaf_tile_cols_log2 = int(tc)
-                    mapping = [(n, t) for n in nodes
-                               for t in range(split_tiles(flat.node_int[n, NI_NR],
-                                                          flat.node_int[n, NI_NU], tc))]
+                    mapping = [(n, int(t)) for n in nodes
+                               for t in (tiles[n] if tiles is not None else
+                                         range(split_tiles(flat.node_int[n, NI_NR],
+                                                           flat.node_int[n, NI_NU], tc)))]
+                    if not mapping:
+                        mapping = [(nodes[0], 0)]
                     from .multipatch import NODE_SOURCE
                     src = np.zeros(flat.node_count, NODE_SOURCE)
                     src["data"], src["pos"] = d.data.ptr, d.pos64.ptr
@@ -165,7 +185,9 @@ class BatchedResidentProgram(ResidentProgram):
                         name=f"batched_bp_polar_{self.mode}")
                 kernel = f"batched_bp_polar_{self.mode}"
             else:
-                mapping = merge_block_table(flat.node_int, nodes, self.mode)
+                mapping = merge_block_table(flat.node_int, nodes, self.mode, tiles)
+                if not mapping:
+                    mapping = [(-1, 0)]
                 blocks = self._table(np.asarray(mapping, dtype=np.int32))
                 if self.mode == "split":
                     self._enqueue("bpmi_ffbp_merges_tiled", len(mapping) * MERGE_BLOCK, (
@@ -180,7 +202,8 @@ class BatchedResidentProgram(ResidentProgram):
                         len(mapping), self.k, mode, self._merge_interp()),
                         name=f"batched_merge_regrid_{self.mode}")
                 kernel = f"batched_merge_regrid_{self.mode}"
-            pixels = int(sum(int(flat.node_int[n, NI_NR]) * int(flat.node_int[n, NI_NU])
+            pixels = int(sum(computed[n] if computed is not None else
+                             int(flat.node_int[n, NI_NR]) * int(flat.node_int[n, NI_NU])
                              for n in nodes))
             self.operations[-1].update({"height": height, "node_count": len(nodes),
                                         "valid_pixels": pixels})
@@ -194,6 +217,37 @@ class BatchedResidentProgram(ResidentProgram):
         # the root image is a view into the arena
         return DeviceArray((self.plan.root.nr * self.plan.root.nu,), np.complex64,
                            arena._alloc, arena._offset + int(root) * 8)
+
+    def _cull(self, flat, arena, tile_cols_log2):
+        """Required tiles of every node for the requested output region
+        (culling.required_masks); the arena is zeroed once so that pixels
+        never computed read as zero."""
+        from .culling import required_masks, tile_selection
+        masks = required_masks(flat, self.plan, self._node_table, self._geometry_table,
+                               region=self._cull_request,
+                               merge_interp=self.backend.merge_interp,
+                               eval_interp=self.backend.merge_interp)
+        tiles, computed = tile_selection(masks, flat, tile_cols_log2)
+        self.computed_pixels = computed
+        _lib.call("bpmi_zero", _vp(arena.ptr), arena.nbytes, None)
+        synchronize()
+        total = int(np.sum(flat.node_int[:, NI_NR].astype(np.int64) * flat.node_int[:, NI_NU]))
+        self.cull = {
+            "region": ("crop" if isinstance(self._cull_request, str) else
+                       "cartesian grid %dx%d" % (self._cull_request.ny, self._cull_request.nx)),
+            "node_pixels": total,
+            "computed_pixels": int(sum(computed.values())),
+            "merge_tiles": int(sum(len(tiles[n]) for h in flat.levels[1:] for n in h)),
+            "merge_tiles_total": int(sum(merge_tiles(flat.node_int[n, NI_NR],
+                                                     flat.node_int[n, NI_NU])
+                                         for h in flat.levels[1:] for n in h)),
+            "computed_pixels_by_height": [int(sum(computed[n] for n in h)) for h in flat.levels],
+            "leaf_tiles": int(sum(len(tiles[n]) for n in flat.levels[0])),
+            "leaf_tiles_total": int(sum(split_tiles(flat.node_int[n, NI_NR],
+                                                    flat.node_int[n, NI_NU], tile_cols_log2)
+                                        for n in flat.levels[0])),
+        }
+        return tiles, computed
 
     def _replace_with_persistent(self, flat, sources):
         """Swap the merge launches (persistent=True: and the leaf launch) for

@@ -803,6 +803,141 @@ __global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
                        lds);
 }
 
+// ---- required-tile marking ----------------------------------------------------
+// A parent pixel reads a child image only through its stencil, so the child
+// pixels that a set of parent tiles can reach lie in the union of those
+// tiles' stencil boxes.  The marking kernels locate the stencils with the
+// same geometry (dest_pixel_col + child_tap, or the evaluation's child_tap
+// at r = 0) as the kernels that read them, widen each stencil by one cell on
+// every side and set the mask byte of every 32 x 32 tile the box touches.
+// Every pixel with a defined stencil origin counts, also those whose stencil
+// leaves the image (they read nothing; counting them only over-marks).
+// Mask bytes are only ever set to 1, so concurrent writers agree.
+constexpr int RT_FAR = 1 << 29;  // stencil origins of any real image are far below
+
+// Stencil box of one defined origin (ir, iu), widened by one cell and clipped
+// to an nr x nu image, merged into [rlo, rhi] x [clo, chi].
+template <int INTERP>
+__device__ __forceinline__ void box_add(int ir, int iu, int nr, int nu, int& rlo, int& rhi,
+                                        int& clo, int& chi) {
+    constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
+    if (ir < -RT_FAR || ir > RT_FAR || iu < -RT_FAR || iu > RT_FAR) return;
+    const int r0 = max(ir - FIRST - 1, 0), r1 = min(ir - FIRST + TAPS, nr - 1);
+    const int c0 = max(iu - FIRST - 1, 0), c1 = min(iu - FIRST + TAPS, nu - 1);
+    if (r0 > r1 || c0 > c1) return;
+    rlo = min(rlo, r0); rhi = max(rhi, r1);
+    clo = min(clo, c0); chi = max(chi, c1);
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sets the mask bytes of the tiles of an image with tiles_u tile columns
+// that cover rows [rlo, rhi] x columns [clo, chi] (threads t0, t0 + step, ..).
+__device__ __forceinline__ void mark_box(unsigned char* mask, int tiles_u, int rlo, int rhi,
+                                         int clo, int chi, int t0, int step) {
+    if (rlo > rhi || clo > chi) return;
+    const int tr0 = rlo / MT_ROWS, tr1 = rhi / MT_ROWS;
+    const int tc0 = clo / MT_COLS, tc1 = chi / MT_COLS;
+    const int ntc = tc1 - tc0 + 1, n = (tr1 - tr0 + 1) * ntc;
+    for (int i = t0; i < n; i += step) {
+        const int a = i / ntc;
+        mask[(long long)(tr0 + a) * tiles_u + tc0 + (i - a * ntc)] = 1;
+    }
+}
+
+// One workgroup per (parent node, tile) entry; entries whose own mask byte is
+// clear return at once.  Pixel placement as in merge_tile.
+template <int INTERP>
+__global__ __launch_bounds__(MT_THREADS) void required_merge_tiles_kernel(
+        const int* __restrict__ node_int, const double* __restrict__ node_geo,
+        const long long* __restrict__ mask_offsets, unsigned char* masks,
+        const int* __restrict__ blocks) {
+    __shared__ int box[2][4];
+    const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
+    if (node < 0 || masks[mask_offsets[node] + tile] == 0) return;
+    const int t = threadIdx.x, lane = t & 63;
+    if (t < 8) box[t >> 2][t & 3] = (t & 1) ? -1 : 0x7fffffff;
+    __syncthreads();
+    const int* ni = node_int + 8 * node;
+    const int nr = ni[2], nu = ni[3], side = ni[4];
+    const int child[2] = {ni[5], ni[6]};
+    double g[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) g[j] = node_geo[7 * node + j];
+    int rlo[2], rhi[2], clo[2], chi[2];
+    FastSrc f[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int* ci = node_int + 8 * child[c];
+        f[c] = make_fast_src(nullptr, node_geo + 7 * child[c], ci[2], ci[3], ci[3]);
+        rlo[c] = clo[c] = 0x7fffffff;
+        rhi[c] = chi[c] = -1;
+    }
+    const int tiles_u = (nu + MT_COLS - 1) / MT_COLS;
+    const int tr = tile / tiles_u, tu = tile - tr * tiles_u;
+    const int row0 = tr * MT_ROWS + t / MT_COLS, col = tu * MT_COLS + (t % MT_COLS);
+    constexpr int ROW_STEP = MT_THREADS / MT_COLS;
+    if (col < nu) {
+        const DestColumn dc = dest_column(g, side, col);
+        for (int j = 0; j < MT_PPT; ++j) {
+            const int row = row0 + j * ROW_STEP;
+            if (row >= nr) break;
+            double x, y, rr;
+            dest_pixel_col(g, dc, row, &x, &y, &rr);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const ChildTap ct = child_tap<INTERP>(f[c], x, y, rr, 0.0);
+                box_add<INTERP>(ct.ir, ct.iu, f[c].nr, f[c].nu, rlo[c], rhi[c], clo[c], chi[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int a = wave_min_i(rlo[c]), b = wave_max_i(rhi[c]);
+        const int d = wave_min_i(clo[c]), e = wave_max_i(chi[c]);
+        if (lane == 0) {
+            atomicMin(&box[c][0], a); atomicMax(&box[c][1], b);
+            atomicMin(&box[c][2], d); atomicMax(&box[c][3], e);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+        mark_box(masks + mask_offsets[child[c]], (f[c].nu + MT_COLS - 1) / MT_COLS, box[c][0],
+                 box[c][1], box[c][2], box[c][3], t, MT_THREADS);
+}
+
+// One thread per Cartesian output pixel of the split-mode grid evaluation of
+// an nr x nu view of a root image (row_offset, col_offset into the root, whose
+// mask has root_tiles_u tile columns); each wave marks its pixels' box.
+template <int INTERP>
+__global__ __launch_bounds__(256) void required_eval_tiles_kernel(
+        Geo g, int nr, int nu, int row_offset, int col_offset, int root_tiles_u, double x0,
+        double dx, int nx, double y0, double dy, int ny, unsigned char* mask) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int rlo = 0x7fffffff, rhi = -1, clo = 0x7fffffff, chi = -1;
+    if (p < (long long)nx * ny) {
+        const int iy = (int)(p / nx), ix = (int)(p - (long long)iy * nx);
+        const double gv[7] = {g.ox, g.oy, g.oz, g.r0, g.dr, g.u0, g.du};
+        const FastSrc f = make_fast_src(nullptr, gv, nr, nu, nu);
+        const ChildTap ct = child_tap<INTERP>(f, x0 + ix * dx, y0 + iy * dy, 0.0, 0.0);
+        box_add<INTERP>(ct.ir, ct.iu, nr, nu, rlo, rhi, clo, chi);
+    }
+    rlo = wave_min_i(rlo); rhi = wave_max_i(rhi);
+    clo = wave_min_i(clo); chi = wave_max_i(chi);
+    if (rlo <= rhi)
+        mark_box(mask, root_tiles_u, rlo + row_offset, rhi + row_offset, clo + col_offset,
+                 chi + col_offset, lane, 64);
+}
+
 template <int INTERP>
 __global__ __launch_bounds__(256) void eval_points_fast_kernel(const float2* __restrict__ img, Geo g,
                                                                int nr, int nu, int ld,
@@ -1473,6 +1608,48 @@ int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo, const lo
         hipLaunchKernelGGL(batched_merge_tiled_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
                            dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
     return check_launch("batched_merge_tiled_kernel");
+}
+
+int bpmi_ffbp_required_tiles(const int* node_int, const double* node_geo,
+                             const long long* mask_offsets, void* masks, const int* blocks,
+                             int nblocks, int interp, void* stream) {
+    int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
+    if (st) return st;
+    if (nblocks <= 0) return BPMI_OK;
+    if (!node_int || !node_geo || !mask_offsets || !masks || !blocks)
+        return set_error(BPMI_ERR_INVALID, "null table");
+    unsigned char* m = (unsigned char*)masks;
+    hipStream_t hs = (hipStream_t)stream;
+    if (interp == BPMI_INTERP_KAISER8)
+        hipLaunchKernelGGL(required_merge_tiles_kernel<INTERP_KAISER8>, dim3(nblocks),
+                           dim3(MT_THREADS), 0, hs, node_int, node_geo, mask_offsets, m, blocks);
+    else
+        hipLaunchKernelGGL(required_merge_tiles_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
+                           dim3(MT_THREADS), 0, hs, node_int, node_geo, mask_offsets, m, blocks);
+    return check_launch("required_merge_tiles_kernel");
+}
+
+int bpmi_eval_required_tiles(const double* geo, int nr, int nu, int row_offset, int col_offset,
+                             int root_nr, int root_nu, double x0, double dx, int nx, double y0,
+                             double dy, int ny, int interp, void* mask, void* stream) {
+    int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
+    if (st) return st;
+    if (!geo || !mask) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (nr <= 0 || nu <= 0 || nx <= 0 || ny <= 0 || row_offset < 0 || col_offset < 0 ||
+        row_offset + nr > root_nr || col_offset + nu > root_nu)
+        return set_error(BPMI_ERR_INVALID, "bad image shape");
+    const Geo g = make_geo(geo);
+    const int tiles_u = (root_nu + MT_COLS - 1) / MT_COLS;
+    unsigned char* m = (unsigned char*)mask;
+    const dim3 gr(grid_1d((long long)nx * ny, 256));
+    hipStream_t hs = (hipStream_t)stream;
+    if (interp == BPMI_INTERP_KAISER8)
+        hipLaunchKernelGGL(required_eval_tiles_kernel<INTERP_KAISER8>, gr, dim3(256), 0, hs, g, nr,
+                           nu, row_offset, col_offset, tiles_u, x0, dx, nx, y0, dy, ny, m);
+    else
+        hipLaunchKernelGGL(required_eval_tiles_kernel<INTERP_LAGRANGE4>, gr, dim3(256), 0, hs, g,
+                           nr, nu, row_offset, col_offset, tiles_u, x0, dx, nx, y0, dy, ny, m);
+    return check_launch("required_eval_tiles_kernel");
 }
 
 int bpmi_persistent_workspace_bytes(int nodes, int njobs, size_t* bytes) {

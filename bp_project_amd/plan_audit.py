@@ -89,10 +89,11 @@ def source_coordinates(destination, source, flat_index):
     return qr, qu, defined
 
 
-def audit_support(plan, interp="lagrange4", batch_size=65536):
+def audit_support(plan, interp="lagrange4", batch_size=65536, sites=None):
     """Propagate required stencil support from the root crop to the leaves.
     Returns the reference's report (resident_plan_audit.py:60-138) with the
-    interpolator named."""
+    interpolator named.  ``sites``: optional dict that receives every node's
+    required-site mask (bool [nr, nu]) by node index."""
     if interp not in STENCILS:
         raise ValueError(f"interp must be one of {tuple(STENCILS)}")
     first, taps = STENCILS[interp]
@@ -140,6 +141,8 @@ def audit_support(plan, interp="lagrange4", batch_size=65536):
 
     def walk(node, required):
         # edges in the reference's order: an edge, then the whole subtree below it
+        if sites is not None:
+            sites[node.index] = required
         if node.is_leaf:
             return
         for child in (node.left, node.right):

@@ -1,10 +1,12 @@
 """One C2-sized FFBP for hardware-counter collection.
 
-    python -m bp_project_amd.profile_merges [--pulses 8192] [--no-untiled]
+    python -m bp_project_amd.profile_merges [--pulses 8192] [--no-untiled] [--cull grid]
 
 Runs the split-precision FFBP of the 8192-pulse x 8192-bin stripmap workload
 (leaf 16, factor 2, 8-tap Kaiser-sinc merges) once through the level-batched
-program (LDS-tiled merges), then every merge level again through the
+program (LDS-tiled merges; ``--cull`` none, crop or grid selects the tiles as
+the C2 step does, default grid: those the 4096 x 4096 Cartesian output needs),
+then every merge level again through the
 untiled merge kernel on the same node images, then one 1 GiB copy kernel
 (`bpmi_crop` of a 16384 x 8192 complex64 image: 1 GiB read, 1 GiB written)
 whose counters calibrate FETCH_SIZE / WRITE_SIZE.  Meant
@@ -32,6 +34,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--pulses", type=int, default=8192)
     ap.add_argument("--no-untiled", action="store_true")
+    ap.add_argument("--cull", default="grid", choices=["none", "crop", "grid"])
     a = ap.parse_args(argv)
     t = time.perf_counter()
     s = workload_scene("c2_ffbp_full", pulses=a.pulses, bins=a.pulses)
@@ -40,7 +43,11 @@ def main(argv=None):
                            angular_oversample=2.5)
     backend = HipBaseline(interp="lagrange4", merge_interp="kaiser8")
     dev = backend.upload(s.data, s.pos, s.weight, s.phase, plan)
-    prog = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k)
+    cull = {"none": None, "crop": "crop", "grid": s.grid}[a.cull]
+    prog = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
+                                  cull=cull)
+    if prog.cull:
+        print(f"cull: {prog.cull}", flush=True)
     print(f"setup {time.perf_counter() - t:.1f} s", flush=True)
     prog.execute()
     synchronize()

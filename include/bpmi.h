@@ -222,6 +222,27 @@ int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo,
                            int nblocks, double k, int interp, void* stream);
 int bpmi_merge_tiles(int nr, int nu, int* tiles);
 
+/* Required-tile marking for culled schedules (no reference counterpart: the
+ * reference computes every pixel of every node, gpu_backend.py:121-224 sizes
+ * each node to the whole root sector).  ``masks`` uint8, one byte per
+ * bpmi_merge_tiles() tile of each node, node i's bytes at masks +
+ * mask_offsets[i], tile = tile_row * ceil(nu / 32) + tile_col.
+ * bpmi_ffbp_required_tiles: for every (node, tile) entry of ``blocks`` whose
+ * byte is set, sets the bytes of both children's tiles that the split-mode
+ * tiled merge of that tile reads (stencils widened by one cell).  Run it per
+ * subtree height from the root down: the set bytes of every node then cover
+ * all pixels that the set root tiles depend on.
+ * bpmi_eval_required_tiles: sets the root bytes that bpmi_eval_grid in
+ * BPMI_MODE_SPLIT reads for the Cartesian grid, for the nr x nu view at
+ * (row_offset, col_offset) of an root_nr x root_nu root (geo: the view's
+ * geometry, as passed to bpmi_eval_grid).  Bytes are only ever set. */
+int bpmi_ffbp_required_tiles(const int* node_int, const double* node_geo,
+                             const long long* mask_offsets, void* masks, const int* blocks,
+                             int nblocks, int interp, void* stream);
+int bpmi_eval_required_tiles(const double* geo, int nr, int nu, int row_offset, int col_offset,
+                             int root_nr, int root_nu, double x0, double dx, int nx, double y0,
+                             double dy, int ny, int interp, void* mask, void* stream);
+
 /* ---- many independent FFBP patches in one launch per level -------------- */
 /* Per-node input description for leaves of different acquisitions (no
  * reference counterpart: the reference batches the nodes of one image,

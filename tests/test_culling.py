@@ -1,0 +1,103 @@
+"""Host side of required-tile culling: tile bookkeeping against brute force,
+and the numpy restatement of the marking (tests/culling_reference.py)
+against the plan audit's independent per-pixel support propagation."""
+import numpy as np
+import pytest
+
+from bp_project_amd.culling import (TILE, crop_tiles, mask_layout, merge_tiles_of, split_tiles_of,
+                                    tile_selection)
+
+
+def _plan(pulses=1024, grid_n=128):
+    from bp_project_amd.plan import build_ffbp_plan
+    from bp_project_amd.scenes import polar_extent_for_grid, stripmap_scene
+    s = stripmap_scene(pulses=pulses, bins=16, n_targets=0, grid_n=grid_n, grid_spacing=0.5,
+                       seed=5, snr_db=None)
+    extent = polar_extent_for_grid(s.grid, s.pos.mean(axis=0), margin=2.0)
+    return s, build_ffbp_plan(s.pos, s.fc, extent, side=1, leaf_size=16, range_spacing=0.5,
+                              angular_oversample=2.5)
+
+
+@pytest.mark.parametrize("tc", [3, 4, 5, 6])
+def test_split_tiles_of_matches_brute_force(tc):
+    rng = np.random.default_rng(tc)
+    for nr, nu in [(37, 70), (100, 9), (64, 64), (5, 200)]:
+        mask = rng.random((-(-nr // TILE), -(-nu // TILE))) < 0.4
+        got = set(split_tiles_of(mask, nr, nu, tc).tolist())
+        w, h = 1 << tc, 1024 >> tc
+        cols = -(-nu // w)
+        want = set()
+        for r in range(nr):
+            for c in range(nu):
+                if mask[r // TILE, c // TILE]:
+                    want.add((r // h) * cols + c // w)
+        assert got == want
+
+
+def test_mask_layout_and_merge_tiles():
+    node_int = np.zeros((3, 8), np.int32)
+    node_int[:, 2] = [32, 33, 1]
+    node_int[:, 3] = [64, 1, 65]
+    offsets, tr, tu, total = mask_layout(node_int)
+    assert tr.tolist() == [1, 2, 1] and tu.tolist() == [2, 1, 3]
+    assert offsets.tolist() == [0, 2, 4] and total == 7
+    m = np.array([[True, False, True], [False, True, False]])
+    assert merge_tiles_of(m).tolist() == [0, 2, 4]
+
+
+def test_crop_tiles_cover_exactly_the_crop():
+    _, plan = _plan()
+    m = crop_tiles(plan)
+    pix = np.repeat(np.repeat(m, TILE, 0), TILE, 1)[:plan.root.nr, :plan.root.nu]
+    (r0, r1), (u0, u1) = plan.crop_r, plan.crop_u
+    assert pix[r0:r1, u0:u1].all()
+    rows = np.flatnonzero(pix.any(axis=1))
+    cols = np.flatnonzero(pix.any(axis=0))
+    assert rows[0] == r0 // TILE * TILE and rows[-1] == min((r1 - 1) // TILE * TILE + TILE,
+                                                            plan.root.nr) - 1
+    assert cols[0] == u0 // TILE * TILE
+
+
+def test_tile_selection_pixel_counts():
+    from bp_project_amd.plan import flatten_plan
+    _, plan = _plan()
+    flat = flatten_plan(plan)
+    _, tr, tu, _ = mask_layout(flat.node_int)
+    masks = [np.ones((tr[i], tu[i]), bool) for i in range(flat.node_count)]
+    tiles, pixels = tile_selection(masks, flat, 5)
+    for n in range(flat.node_count):
+        assert pixels[n] == flat.node_int[n, 2] * flat.node_int[n, 3]
+
+
+@pytest.mark.parametrize("interp", ["kaiser8", "lagrange4"])
+def test_reference_marking_equals_audit_support(interp):
+    """Pixel-level marking (tile 1, pixels that read, no widening) equals the
+    required sites that the plan audit propagates independently; the 32 x 32
+    tile marking contains them and still leaves tiles unmarked."""
+    from culling_reference import reference_masks
+    from bp_project_amd.plan_audit import audit_support
+    _, plan = _plan()
+    sites = {}
+    audit_support(plan, interp, sites=sites)
+    exact = reference_masks(plan, interp, widen=0, ok_only=True, tile=1)
+    masks = reference_masks(plan, interp, widen=0, ok_only=True)
+    culled = 0
+    for node, req in sites.items():
+        assert np.array_equal(exact[node], req), f"node {node}"
+        pix = np.repeat(np.repeat(masks[node], TILE, 0), TILE, 1)[:req.shape[0], :req.shape[1]]
+        assert not np.any(req & ~pix), f"node {node}: required site outside the marked tiles"
+        culled += int((~masks[node]).sum())
+    assert culled > 0  # the geometry leaves tiles nobody reads
+
+
+@pytest.mark.parametrize("stencil", [(-2, 8), (-3, 7), (-4, 8)])
+def test_planted_stencil_defects_are_caught(monkeypatch, stencil):
+    """A restatement with a stencil one cell off disagrees with the audit."""
+    import culling_reference
+    from bp_project_amd.plan_audit import audit_support
+    _, plan = _plan()
+    sites = {}
+    audit_support(plan, "kaiser8", sites=sites)
+    monkeypatch.setitem(culling_reference.STENCILS, "kaiser8", stencil)
+    exact = culling_reference.reference_masks(plan, "kaiser8", widen=0, ok_only=True, tile=1)
+    assert any(not np.array_equal(exact[n], req) for n, req in sites.items())

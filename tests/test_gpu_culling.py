@@ -1,0 +1,109 @@
+"""Required-tile culling on the GPU: the device masks contain the exact
+necessary tiles of the numpy restatement (and stay within its generous
+bound), and culled programs reproduce the unculled output region bit for
+bit -- the root crop, or the Cartesian evaluation of it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(merge_interp):
+    from bp_project_amd.gpu_backend import HipBaseline, build_ffbp_plan
+    from bp_project_amd.scenes import polar_extent_for_grid, stripmap_scene
+    s = stripmap_scene(pulses=1024, bins=2048, n_targets=8, grid_n=128, grid_spacing=0.5, seed=5)
+    extent = polar_extent_for_grid(s.grid, s.pos.mean(axis=0), margin=2.0)
+    plan = build_ffbp_plan(s.pos, s.fc, extent, side=1, leaf_size=16, range_spacing=0.5,
+                           angular_oversample=2.5)
+    backend = HipBaseline(interp="lagrange4", merge_interp=merge_interp)
+    dev = backend.upload(s.data, s.pos, s.weight, s.phase, plan)
+    return s, plan, backend, dev
+
+
+def _program(s, plan, backend, dev, cull):
+    from bp_project_amd.batched_resident_backend import BatchedResidentProgram
+    return BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
+                                  leaf_tile_cols_log2=5, cull=cull)
+
+
+def _evaluate(prog, plan, s, interp):
+    from bp_project_amd import _lib
+    from bp_project_amd.device import DeviceArray, stream_handle, synchronize
+    from bp_project_amd.gpu_backend import geo_array
+    g = s.grid
+    out = DeviceArray.empty(g.shape)
+    geo = geo_array(plan)
+    _lib.call("bpmi_eval_grid", _lib.c_void_p(prog.output.ptr), geo.ctypes.data_as(_lib.c_void_p),
+              plan.nr, plan.nu, plan.nu, g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, s.k,
+              _lib.mode_code("split"), _lib.interp_code(interp), _lib.c_void_p(out.ptr),
+              stream_handle())
+    synchronize()
+    return out.get()
+
+
+@pytest.mark.parametrize("merge_interp", ["kaiser8", "lagrange4"])
+def test_device_masks_bracket_the_restatement(merge_interp):
+    from culling_reference import grid_root_tiles, reference_masks
+    from bp_project_amd.culling import required_masks
+    from bp_project_amd.plan import flatten_plan
+    from bp_project_amd.device import DeviceArray
+    s, plan, backend, dev = _setup(merge_interp)
+    flat = flatten_plan(plan)
+    nt = DeviceArray.from_host(flat.node_int)
+    gt = DeviceArray.from_host(flat.node_geo)
+    for region in ("crop", s.grid):
+        print(f"{merge_interp}, region {'crop' if region == 'crop' else 'grid'}", flush=True)
+        got = required_masks(flat, plan, nt, gt, region=region, merge_interp=merge_interp,
+                             eval_interp=merge_interp)
+        root_exact = root_loose = None
+        if region != "crop":
+            root_exact = grid_root_tiles(plan, s.grid, merge_interp, widen=0, ok_only=True)
+            root_loose = grid_root_tiles(plan, s.grid, merge_interp, widen=2)
+            assert not np.any(root_exact & ~got[plan.root.index])
+            assert not np.any(got[plan.root.index] & ~root_loose)
+        exact = reference_masks(plan, merge_interp, widen=0, ok_only=True, root_mask=root_exact)
+        loose = reference_masks(plan, merge_interp, widen=2, root_mask=root_loose)
+        kept = total = 0
+        for n in range(flat.node_count):
+            assert not np.any(exact[n] & ~got[n]), f"node {n}: necessary tile not marked"
+            assert not np.any(got[n] & ~loose[n]), f"node {n}: tile beyond the generous bound"
+            kept += int(got[n].sum())
+            total += got[n].size
+        print(f"kept {kept} of {total} tiles")
+        assert kept < total
+
+
+@pytest.mark.parametrize("merge_interp", ["kaiser8", "lagrange4"])
+def test_culled_programs_reproduce_the_output_bitwise(merge_interp):
+    from bp_project_amd.device import synchronize
+    s, plan, backend, dev = _setup(merge_interp)
+    full = _program(s, plan, backend, dev, None)
+    crop = _program(s, plan, backend, dev, "crop")
+    grid = _program(s, plan, backend, dev, s.grid)
+    for p in (crop, grid):
+        print(p.cull, flush=True)
+        assert p.cull["computed_pixels"] < p.cull["node_pixels"]
+        assert p.cull["merge_tiles"] < p.cull["merge_tiles_total"]
+    full.execute()
+    synchronize()
+    ref = full.output.get()
+    ref_cart = _evaluate(full, plan, s, merge_interp)
+    for rep in range(2):
+        crop.execute()
+        grid.execute()
+        synchronize()
+        assert np.array_equal(crop.output.get(), ref), f"crop, run {rep}"
+        assert np.array_equal(_evaluate(grid, plan, s, merge_interp), ref_cart), f"grid, run {rep}"
+    assert grid.cull["computed_pixels"] <= crop.cull["computed_pixels"]
+    for p in (full, crop, grid):
+        p.close()
+
+
+def test_cull_rejects_other_modes():
+    s, plan, backend, dev = _setup("kaiser8")
+    from bp_project_amd.batched_resident_backend import BatchedResidentProgram
+    with pytest.raises(ValueError):
+        BatchedResidentProgram(backend, plan, dev, "fp64", s.r_start, s.data_dr, s.k, cull="crop")
+    with pytest.raises(ValueError):
+        BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
+                               persistent=True, cull="crop")
