@@ -471,8 +471,9 @@ def run_multipatch(args):
     t = time.perf_counter()
     scenes = make_patches(patch_mix(count=64))
     log(f"64 patches generated in {time.perf_counter() - t:.1f} s")
+    cull = args.cull != "none" and args.mode == "split" and not args.persistent
     prog = MultiPatchProgram(scenes, mode=args.mode, merge_interp="kaiser8",
-                             persistent=args.persistent)
+                             persistent=args.persistent, cull=cull)
     best, times = prog.autotune()
     log(f"autotuned leaf tile width 2^{best}: {times}")
     prog.capture_graph()
@@ -501,7 +502,7 @@ def run_multipatch(args):
         lat_p99 = float(np.percentile(lat, 99))
     seq_out, isolated, completion, progs = sequential_reference(scenes, mode=args.mode,
                                                                  merge_interp="kaiser8",
-                                                                 tile_cols_log2=best)
+                                                                 tile_cols_log2=best, cull=cull)
     same = all(np.array_equal(x, y.get()) for x, y in zip(batch_out, seq_out))
     work = prog.work
     seq_total = completion[-1]
@@ -526,7 +527,8 @@ def run_multipatch(args):
             "launches_per_batch": prog.launch_count},
         "roofline": None,
         "config": {"workload": "c5_multipatch", "patches": len(scenes), "mode": args.mode,
-                   "merge_interp": "kaiser8", "graph": True, "persistent": bool(args.persistent)},
+                   "merge_interp": "kaiser8", "graph": True, "persistent": bool(args.persistent),
+                   "cull": "grid" if cull else "none"},
     }
 
 

@@ -4,7 +4,9 @@ All leaves of all patches run in one launch (per-node input table), the
 merges of every patch run one launch per subtree height, and every patch's
 Cartesian output is evaluated in one launch; the whole batch is captured
 in one HIP graph, so a batch is one graph launch with no host round trips.
-The split-mode leaf tile width is chosen by the autotuner.
+The split-mode leaf tile width is chosen by the autotuner.  With
+``cull=True`` only the node tiles that each patch's Cartesian output depends
+on are launched (culling.py); the outputs are bitwise unchanged.
 
 Counterpart of nothing in the reference (it images one ROI at a time,
 fast_bp/batched_resident_backend.py); it composes the reference-parity
@@ -18,6 +20,7 @@ import numpy as np
 from . import _lib
 from .device import DeviceArray, Event, Graph, Stream, current_stream, elapsed_ms, stream_handle, synchronize
 from .batched_resident_backend import enqueue_merges, merge_block_table
+from .culling import merge_tiles_of, split_tiles_of
 from .gpu_backend import build_ffbp_plan, ensure_kaiser_table
 from .plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, flatten_plan
 from .scenes import CartesianGrid, polar_extent_for_grid, stripmap_scene
@@ -77,9 +80,11 @@ class MultiPatchProgram:
 
     def __init__(self, scenes, mode="split", leaf_size=16, range_spacing=0.5,
                  angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
-                 tile_cols_log2=6, persistent=False):
+                 tile_cols_log2=6, persistent=False, cull=False):
         if not scenes:
             raise ValueError("no patches")
+        if cull and (mode != "split" or persistent):
+            raise ValueError("culling applies to the split-mode level launches only")
         ks = {round(s.k, 9) for s in scenes}
         if len(ks) != 1:
             raise ValueError("all patches of a batch must share the carrier wavenumber")
@@ -134,12 +139,15 @@ class MultiPatchProgram:
         self.t_node_geo = DeviceArray.from_host(self.node_geo)
         self.t_offsets = DeviceArray.from_host(offsets)
         self.t_sources = DeviceArray.from_host(np.concatenate(sources).view(np.uint8))
+        self._masks = self._required_masks() if cull else None
         self.set_tile_width(tile_cols_log2)
         heights = self.node_int[:, NI_HEIGHT]
         self.merge_tables = []
+        tiles = (None if self._masks is None else
+                 {n: merge_tiles_of(m) for n, m in enumerate(self._masks)})
         for h in range(1, int(heights.max()) + 1):
             nodes = np.flatnonzero(heights == h)
-            mapping = merge_block_table(self.node_int, nodes, self.mode)
+            mapping = merge_block_table(self.node_int, nodes, self.mode, tiles) or [(-1, 0)]
             self.merge_tables.append(DeviceArray.from_host(np.asarray(mapping, np.int32)))
         # Cartesian evaluation jobs (crop as a view into each root)
         jobs = np.zeros(len(self.patches), EVAL_JOB)
@@ -173,12 +181,33 @@ class MultiPatchProgram:
             self._build_persistent()
             self.launch_count = 1 if self.persistent is True else 2
 
+    def _required_masks(self):
+        """Tile masks of every node (global index) that each patch's
+        Cartesian output depends on (culling.required_masks per patch); the
+        arena is zeroed once so that tiles never computed read as zero."""
+        from .culling import required_masks
+        masks = []
+        for pp in self.patches:
+            nt = DeviceArray.from_host(pp.flat.node_int)
+            gt = DeviceArray.from_host(pp.flat.node_geo)
+            masks += required_masks(pp.flat, pp.plan, nt, gt, region=pp.scene.grid,
+                                    merge_interp=self.merge_interp,
+                                    eval_interp=self.merge_interp)
+        _lib.call("bpmi_zero", _vp(self.arena.ptr), self.arena.nbytes, None)
+        synchronize()
+        return masks
+
     def set_tile_width(self, tile_cols_log2):
         """(Re)build the leaf block table for a split-mode tile width."""
         self.tile_cols_log2 = int(tile_cols_log2)
         leaves = np.flatnonzero(self.node_int[:, NI_HEIGHT] == 0)
         mapping = []
-        if self.mode == "split":
+        if self.mode == "split" and self._masks is not None:
+            for n in leaves:
+                sel = split_tiles_of(self._masks[n], int(self.node_int[n, NI_NR]),
+                                     int(self.node_int[n, NI_NU]), self.tile_cols_log2)
+                mapping += [(int(n), int(i)) for i in sel]
+        elif self.mode == "split":
             import ctypes
             t = ctypes.c_int()
             for n in leaves:
@@ -299,7 +328,7 @@ class MultiPatchProgram:
 
 def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
                          angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
-                         tile_cols_log2=6):
+                         tile_cols_log2=6, cull=False):
     """Per-patch programs (the same kernels, one patch at a time): returns the
     outputs, each patch's isolated event time and its completion time when
     the patches are launched back to back."""
@@ -311,7 +340,8 @@ def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
         pp = PatchPlan(s, leaf_size, range_spacing, angular_oversample)
         dev = backend.upload(s.data, s.pos, s.weight, s.phase, pp.plan)
         prog = BatchedResidentProgram(backend, pp.plan, dev, mode, s.r_start, s.data_dr, s.k,
-                                      leaf_tile_cols_log2=tile_cols_log2)
+                                      leaf_tile_cols_log2=tile_cols_log2,
+                                      cull=s.grid if cull else None)
         out = DeviceArray.empty(s.grid.shape)
         progs.append((pp, prog, out, geo_array(pp.plan)))
 

@@ -42,6 +42,43 @@ def test_multipatch_equals_sequential(scenes, mode):
     prog.close()
 
 
+def test_culled_multipatch_equals_unculled(scenes):
+    """Tile culling to each patch's grid: outputs bitwise equal to the
+    unculled batch and to the culled per-patch programs, also after an
+    autotuned tile-width change and under graph replay."""
+    from bp_project_amd.device import synchronize
+    from bp_project_amd.multipatch import MultiPatchProgram, sequential_reference
+    full = MultiPatchProgram(scenes, mode="split", merge_interp="kaiser8", tile_cols_log2=5)
+    culled = MultiPatchProgram(scenes, mode="split", merge_interp="kaiser8", tile_cols_log2=5,
+                               cull=True)
+    full.execute()
+    culled.execute()
+    synchronize()
+    ref = [o.get() for o in full.outputs]
+    assert all(np.array_equal(a, o.get()) for a, o in zip(ref, culled.outputs))
+    seq, _, _, _ = sequential_reference(scenes, mode="split", merge_interp="kaiser8",
+                                        tile_cols_log2=5, cull=True)
+    assert all(np.array_equal(a, b.get()) for a, b in zip(ref, seq))
+    for tc in (4, 6):
+        full.set_tile_width(tc)
+        culled.set_tile_width(tc)
+        full.execute()
+        culled.execute()
+        synchronize()
+        assert all(np.array_equal(a.get(), b.get())
+                   for a, b in zip(full.outputs, culled.outputs)), f"tile width 2^{tc}"
+    assert culled.capture_graph()
+    for _ in range(2):
+        culled.replay()
+        synchronize()
+        assert all(np.array_equal(a.get(), b.get())
+                   for a, b in zip(full.outputs, culled.outputs))
+    with pytest.raises(ValueError):
+        MultiPatchProgram(scenes, mode="mixed", cull=True)
+    full.close()
+    culled.close()
+
+
 def test_multipatch_patch_vs_fp64_oracle(oracle, scenes):
     """One patch through the batch against the oracle's FP64 FFBP + evaluation."""
     from bp_project_amd.device import synchronize
