@@ -89,6 +89,7 @@ SIGNATURES = {
                           _vp],
     "bpmi_set_kaiser_table": [_vp, _i],
     "bpmi_adjoint_workspace_bytes": [_ll, _i, POINTER(c_size_t)],
+    "bpmi_grid_adjoint_workspace_bytes": [_i, _i, _i, POINTER(c_size_t)],
     "bpmi_bp_grid_adjoint": [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _i, _d, _d, _i, _d, _d, _d,
                              _d, _i, _vp, _vp, _sz, _vp, _vp, _vp],
     "bpmi_bp_points_adjoint": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _ll, _d, _d, _d, _i, _vp, _vp,

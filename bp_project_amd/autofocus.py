@@ -106,7 +106,7 @@ class GridOperator:
         pos = pos.detach().to(torch.float64).contiguous()
         grad = grad_image.detach().to(torch.complex64).contiguous()
         nbytes = ctypes.c_size_t()
-        _lib.call("bpmi_adjoint_workspace_bytes", self.nx * self.ny, P, ctypes.byref(nbytes))
+        _lib.call("bpmi_grid_adjoint_workspace_bytes", self.nx, self.ny, P, ctypes.byref(nbytes))
         work = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.data.device)
         gphi = torch.empty(P, dtype=torch.float64, device=self.data.device)
         gpos = torch.empty((P, 3), dtype=torch.float64, device=self.data.device)

@@ -9,7 +9,7 @@
 //    backproject_points (fast_bp/ffbp.py:43-55).
 //
 // 2. The SPLIT kernel: a workgroup owns a 1024-pixel tile; pulses are
-//    processed in blocks of 16 whose per-pulse geometry is reduced once per
+//    processed in blocks of 32 whose per-pulse geometry is reduced once per
 //    tile in FP64 and whose range-window slices are staged in LDS as cubic
 //    (or linear) polynomial coefficients pre-multiplied by the pulse weight.
 //    Per pixel and pulse the residual geometry runs in FP32, the large linear
@@ -477,6 +477,225 @@ __global__ __launch_bounds__(ADJ_THREADS) void bp_adjoint_kernel(BpArgs a, SrcAr
     }
 }
 
+// ---- split-precision adjoint of the grid BP ---------------------------------
+// The SPLIT forward's decomposition applied to the adjoint: a workgroup owns
+// a 2-D tile of 1024 grid pixels; per block of ADJ_SNB pulses the FP64 tile
+// reference and per-pulse parameters (split_pulse_params) are computed once
+// and the range windows staged in LDS as weighted cubic (or line)
+// coefficients (stage_pulse_block).  Per pixel and pulse: the FP64 linear
+// range term and its phase modulo one turn, the FP32 range residual, the
+// staged polynomial's value v and derivative dv at the sample coordinate,
+// and the five products of bp_adjoint_kernel, with 1/d from the tile's 1/D
+// by a second-order expansion in (d - D)/D (|d - D| <= tile radius, so the
+// relative error is below (rho/D)^3).  Pulses whose bound fails (PF_SLOW)
+// take bp_adjoint_kernel's exact FP64 per-pixel path.  Sums are reduced per
+// ADJ_NB pulses through LDS into partials[tile][pulse][5] as there.
+constexpr int ADJ_SNB = 16;
+
+template <int INTERP>
+struct AdjSplitLds {
+    float4 coef[ADJ_SNB * SP_W * SlotSize<INTERP>::F4];
+    PulseParams prm[ADJ_SNB];
+    float4 off[ADJ_SNB];  // (c - a) of each pulse in FP32
+    TileRef ref;
+    double red8[4][8];
+    float red[ADJ_ROWS][ADJ_THREADS + 1];
+};
+
+template <int INTERP>
+__global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
+        BpArgs a, SrcArgs s, int tc_log2, const float2* __restrict__ grad,
+        float* __restrict__ partials) {
+    static_assert(ADJ_THREADS == SP_THREADS && ADJ_PPT == SP_PPT, "split tile shape");
+    static_assert(ADJ_SNB % ADJ_NB == 0, "whole pulse groups per block");
+    constexpr int F4 = SlotSize<INTERP>::F4;
+    __shared__ AdjSplitLds<INTERP> lds;
+    const int t = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int S = a.samples;
+    const int P = a.pulse_end - a.pulse_begin;
+    const double kt = a.k * INV_TWO_PI;
+    const float kt32 = (float)kt;
+    const float k32 = (float)a.k;
+    const double inv_ddr = 1.0 / a.data_dr;
+    const float invdr = (float)inv_ddr;
+
+    // ---- pixels and the tile reference ---------------------------------------
+    bool val[SP_PPT];
+    double px[SP_PPT], py[SP_PPT];  // dead after the tile reference
+    float2 g[SP_PPT];
+    double lo_x = 1e300, lo_y = 1e300, hi_x = -1e300, hi_y = -1e300;
+#pragma unroll
+    for (int k = 0; k < SP_PPT; ++k) {
+        int row = 0, col = 0;
+        const long long p = split_pixel<SRC_GRID>(s, tile, tc_log2, t, k, &val[k], &row, &col);
+        if (!val[k]) row = col = 0;
+        double z, r;
+        split_pixel64<SRC_GRID>(s, val[k] ? p : 0, row, col, &px[k], &py[k], &z, &r);
+        g[k] = val[k] ? grad[p] : make_float2(0.f, 0.f);
+        if (val[k]) {
+            lo_x = fmin(lo_x, px[k]); hi_x = fmax(hi_x, px[k]);
+            lo_y = fmin(lo_y, py[k]); hi_y = fmax(hi_y, py[k]);
+        }
+    }
+    // grid pixels lie in the plane z = z0: zero z extent, slant span unused
+    tile_reference_bbox(a, lo_x, lo_y, s.a2, hi_x, hi_y, s.a2, 0.0, 0.0, lds.red8, lds.ref, t);
+    const double cx = lds.ref.cx, cy = lds.ref.cy;
+    double dxk[SP_PPT], dyk[SP_PPT];
+    float ek[SP_PPT], ex32[SP_PPT], ey32[SP_PPT];
+#pragma unroll
+    for (int k = 0; k < SP_PPT; ++k) {
+        dxk[k] = px[k] - cx;
+        dyk[k] = py[k] - cy;
+        ek[k] = (float)(dxk[k] * dxk[k] + dyk[k] * dyk[k]);
+        ex32[k] = (float)dxk[k];
+        ey32[k] = (float)dyk[k];
+    }
+    const int vlo = InterpShape<INTERP>::FIRST < 0 ? -InterpShape<INTERP>::FIRST : 0;
+    const int vhi = S - (InterpShape<INTERP>::TAPS + InterpShape<INTERP>::FIRST);
+
+    for (int nb0 = a.pulse_begin; nb0 < a.pulse_end; nb0 += ADJ_SNB) {
+        const int nb = min(ADJ_SNB, a.pulse_end - nb0);
+        if (t < nb) {
+            const int n = nb0 + t;
+            lds.prm[t] = split_pulse_params<false>(a, lds.ref, n, kt, inv_ddr, vlo, vhi);
+            lds.off[t] = make_float4((float)(lds.ref.cx - a.pos[3 * n]),
+                                     (float)(lds.ref.cy - a.pos[3 * n + 1]),
+                                     (float)(lds.ref.cz - a.pos[3 * n + 2]), 0.f);
+        }
+        __syncthreads();
+        stage_pulse_block<INTERP>(a, lds.prm, lds.coef, nb, vlo, vhi, t);
+        __syncthreads();
+        for (int j0 = 0; j0 < nb; j0 += ADJ_NB) {
+            float acc[ADJ_NB][ADJ_VALS];
+#pragma unroll
+            for (int jj = 0; jj < ADJ_NB; ++jj) {
+#pragma unroll
+                for (int v = 0; v < ADJ_VALS; ++v) acc[jj][v] = 0.f;
+                const int b = j0 + jj;
+                if (b >= nb) continue;
+                const int flag = uniform_i32(lds.prm[b].flag);
+                if (flag == PF_SKIP) continue;
+                const PulseParams q = lds.prm[b];
+                const float4 off = lds.off[b];
+                if (flag <= PF_FAST2) {
+                    const bool order4 = flag == PF_FAST4;
+                    const float4* cb = lds.coef + b * SP_W * F4;
+#pragma unroll
+                    for (int k = 0; k < SP_PPT; ++k) {
+                        const double lin = dxk[k] * q.vx + dyk[k] * q.vy;
+                        const double T = hw_fract64(q.P + lin * kt);
+                        const float l32 = (float)lin;
+                        const float h = ek[k] - l32 * l32;
+                        const float m = l32 * q.invr;
+                        const float hs = h * q.scale;
+                        float corr;
+                        if (order4) {
+                            float pz = fmaf(m, m - 1.f, 1.f);
+                            pz = fmaf(-hs, q.scale, pz);
+                            corr = hs * pz;
+                        } else {
+                            corr = hs - hs * m;
+                        }
+                        const float dl = l32 + corr;  // d - D
+                        const float ql = q.qb + dl * invdr;
+                        const float turns = (float)T + corr * kt32;
+                        int j = (int)ql;
+                        const float tt = hw_fract32(ql);
+                        j = min(max(j, 0), SP_W - 1);
+                        float2 v, dv;
+                        if (INTERP == INTERP_LAGRANGE4) {
+                            // p = ((c3 t + c2) t + c1) t + c0, p' = B + t (A + t c3)
+                            const float4 lo = cb[2 * j], hi = cb[2 * j + 1];
+                            const float ax_ = fmaf(hi.z, tt, hi.x), ay_ = fmaf(hi.w, tt, hi.y);
+                            const float bx = fmaf(ax_, tt, lo.z), by = fmaf(ay_, tt, lo.w);
+                            v.x = fmaf(bx, tt, lo.x);
+                            v.y = fmaf(by, tt, lo.y);
+                            dv.x = fmaf(tt, fmaf(tt, hi.z, ax_), bx);
+                            dv.y = fmaf(tt, fmaf(tt, hi.w, ay_), by);
+                        } else {
+                            const float4 lo = cb[j];
+                            v.x = fmaf(lo.z, tt, lo.x);
+                            v.y = fmaf(lo.w, tt, lo.y);
+                            dv.x = lo.z;
+                            dv.y = lo.w;
+                        }
+                        float sn, cs;
+                        sincos_turns(turns, &sn, &cs);
+                        // z = w v e^{j theta}, dz = w dv e^{j theta} (w is in the coefficients)
+                        const float zx = v.x * cs - v.y * sn, zy = v.x * sn + v.y * cs;
+                        const float dzx = dv.x * cs - dv.y * sn, dzy = dv.x * sn + dv.y * cs;
+                        // A += conj(g) z
+                        acc[jj][0] += g[k].x * zx + g[k].y * zy;
+                        acc[jj][1] += g[k].x * zy - g[k].y * zx;
+                        // Re(conj(g) (dz / data_dr + j k z)) / d
+                        const float ur = dzx * invdr - k32 * zy;
+                        const float ui = dzy * invdr + k32 * zx;
+                        const float e = dl * q.invr;
+                        const float rinv = q.invr * fmaf(e, e - 1.f, 1.f);
+                        const float br = (g[k].x * ur + g[k].y * ui) * rinv;
+                        // (p_n - x) = -((x - c) + (c - a))
+                        acc[jj][2] -= br * (ex32[k] + off.x);
+                        acc[jj][3] -= br * (ey32[k] + off.y);
+                        acc[jj][4] -= br * off.z;
+                    }
+                } else {
+                    // exact FP64 geometry, samples from global memory
+                    const int n = q.pulse;
+                    const float2* row = a.data + (size_t)n * S;
+                    const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
+                    const double ux = cx - a.pos[3 * n], uy = cy - a.pos[3 * n + 1];
+                    const double ez = s.a2 - a.pos[3 * n + 2];
+                    const float w = q.w;
+#pragma unroll
+                    for (int k = 0; k < SP_PPT; ++k) {
+                        if (!val[k]) continue;
+                        const double ex = dxk[k] + ux, ey = dyk[k] + uy;
+                        const double s2 = ex * ex + ey * ey + ez * ez;
+                        const double rinv = rsqrt64(s2);
+                        const double d = s2 * rinv;
+                        const double qq = (d - a.range_start) * inv_ddr;
+                        int i;
+                        if (!floor_index(qq, &i)) continue;
+                        float2 v, dv;
+                        sample1_with_derivative<INTERP>(row, S, i, (float)(qq - (double)i), &v,
+                                                        &dv);
+                        float sn, cs;
+                        sincos_turns((float)hw_fract64(kt * d + ph), &sn, &cs);
+                        const float hr = w * (g[k].x * cs + g[k].y * sn);
+                        const float hi = w * (g[k].x * sn - g[k].y * cs);
+                        acc[jj][0] += hr * v.x - hi * v.y;
+                        acc[jj][1] += hr * v.y + hi * v.x;
+                        const float ur = dv.x * invdr - k32 * v.y;
+                        const float ui = dv.y * invdr + k32 * v.x;
+                        const float br = (hr * ur - hi * ui) * (float)rinv;
+                        acc[jj][2] -= br * (float)ex;
+                        acc[jj][3] -= br * (float)ey;
+                        acc[jj][4] -= br * (float)ez;
+                    }
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < ADJ_NB; ++jj)
+#pragma unroll
+                for (int v = 0; v < ADJ_VALS; ++v) lds.red[jj * ADJ_VALS + v][t] = acc[jj][v];
+            __syncthreads();
+            if (t < 4 * ADJ_ROWS) {
+                const int r = t >> 2, part = t & 3;
+                float sum = 0.f;
+                for (int c = part; c < ADJ_THREADS; c += 4) sum += lds.red[r][c];
+                sum += __shfl_xor(sum, 1, 4);
+                sum += __shfl_xor(sum, 2, 4);
+                const int jj = r / ADJ_VALS;
+                if (part == 0 && j0 + jj < nb)
+                    partials[((size_t)tile * P + (nb0 + j0 + jj - a.pulse_begin)) * ADJ_VALS +
+                             (r - jj * ADJ_VALS)] = sum;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Fixed-order FP64 reduction over tiles: out_phase[n] = -Im A, out_pos[n] = G.
 __global__ void adjoint_reduce_kernel(const float* __restrict__ partials, int tiles, int P,
                                       double* __restrict__ grad_phase,
@@ -498,42 +717,88 @@ __global__ void adjoint_reduce_kernel(const float* __restrict__ partials, int ti
     }
 }
 
-static int adjoint_launch(int interp, int src, const BpArgs& a, const SrcArgs& s, long long count,
-                          const void* grad, void* partials, size_t partials_bytes,
+static int adjoint_reduce(const BpArgs& a, long long tiles, const void* partials,
                           double* grad_phase, double* grad_pos, hipStream_t hs) {
-    if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
-        return set_error(BPMI_ERR_UNSUPPORTED, "adjoint supports lagrange4 and linear");
-    if (!grad || !partials) return set_error(BPMI_ERR_INVALID, "grad and partials must be non-null");
-    const long long tiles = (count + ADJ_TILE - 1) / ADJ_TILE;
     const int P = a.pulse_end - a.pulse_begin;
-    if (P <= 0) return BPMI_OK;
-    const size_t need = (size_t)tiles * (size_t)P * 5 * sizeof(float);
-    if (partials_bytes < need)
-        return set_error(BPMI_ERR_INVALID, "partials workspace too small: need %zu bytes", need);
-    if (tiles > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "image too large");
-    const dim3 g((unsigned)tiles);
-    if (src == SRC_GRID) {
-        if (interp == BPMI_INTERP_LAGRANGE4)
-            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LAGRANGE4, SRC_GRID>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
-        else
-            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LINEAR, SRC_GRID>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
-    } else {
-        if (interp == BPMI_INTERP_LAGRANGE4)
-            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LAGRANGE4, SRC_POINTS>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
-        else
-            hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LINEAR, SRC_POINTS>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
-    }
-    int st = check_launch("bp_adjoint_kernel");
-    if (st) return st;
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(grid_1d(P, 256)), dim3(256), 0, hs,
                        (const float*)partials, (int)tiles, P, grad_phase, grad_pos);
     return check_launch("adjoint_reduce_kernel");
 }
 
+static int adjoint_check(int interp, const void* grad, const void* partials) {
+    if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
+        return set_error(BPMI_ERR_UNSUPPORTED, "adjoint supports lagrange4 and linear");
+    if (!grad || !partials) return set_error(BPMI_ERR_INVALID, "grad and partials must be non-null");
+    return BPMI_OK;
+}
+
+static size_t adjoint_partials_bytes(long long tiles, int pulses) {
+    return (size_t)tiles * (size_t)pulses * ADJ_VALS * sizeof(float);
+}
+
+// Points: tiles of 1024 consecutive points, per-pixel FP64 geometry.
+static int adjoint_points_launch(int interp, const BpArgs& a, const SrcArgs& s, long long count,
+                                 const void* grad, void* partials, size_t partials_bytes,
+                                 double* grad_phase, double* grad_pos, hipStream_t hs) {
+    int st = adjoint_check(interp, grad, partials);
+    if (st) return st;
+    const long long tiles = (count + ADJ_TILE - 1) / ADJ_TILE;
+    const int P = a.pulse_end - a.pulse_begin;
+    if (P <= 0) return BPMI_OK;
+    const size_t need = adjoint_partials_bytes(tiles, P);
+    if (partials_bytes < need)
+        return set_error(BPMI_ERR_INVALID, "partials workspace too small: need %zu bytes", need);
+    if (tiles > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "image too large");
+    const dim3 g((unsigned)tiles);
+    if (interp == BPMI_INTERP_LAGRANGE4)
+        hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LAGRANGE4, SRC_POINTS>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
+    else
+        hipLaunchKernelGGL((bp_adjoint_kernel<INTERP_LINEAR, SRC_POINTS>), g, dim3(ADJ_THREADS), 0, hs, a, s, (const float2*)grad, (float*)partials);
+    st = check_launch("bp_adjoint_kernel");
+    if (st) return st;
+    return adjoint_reduce(a, tiles, partials, grad_phase, grad_pos, hs);
+}
+
+// Grid: 2-D split tiles of the forward's shape (choose_tc_log2 as bpmi_bp_grid).
+static int adjoint_grid_launch(int interp, const BpArgs& a, const SrcArgs& s, const void* grad,
+                               void* partials, size_t partials_bytes, double* grad_phase,
+                               double* grad_pos, hipStream_t hs) {
+    int st = adjoint_check(interp, grad, partials);
+    if (st) return st;
+    const int P = a.pulse_end - a.pulse_begin;
+    if (P <= 0) return BPMI_OK;
+    const int tcl = choose_tc_log2(fabs(s.b1), fabs(s.b3), 1.0, 0.3);
+    const long long tiles = tiles_for(s.rows, s.cols, tcl);
+    const size_t need = adjoint_partials_bytes(tiles, P);
+    if (partials_bytes < need)
+        return set_error(BPMI_ERR_INVALID, "partials workspace too small: need %zu bytes", need);
+    if (tiles > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "image too large");
+    const dim3 g((unsigned)tiles);
+    if (interp == BPMI_INTERP_LAGRANGE4)
+        hipLaunchKernelGGL((bp_adjoint_split_kernel<INTERP_LAGRANGE4>), g, dim3(ADJ_THREADS), 0, hs, a, s, tcl, (const float2*)grad, (float*)partials);
+    else
+        hipLaunchKernelGGL((bp_adjoint_split_kernel<INTERP_LINEAR>), g, dim3(ADJ_THREADS), 0, hs, a, s, tcl, (const float2*)grad, (float*)partials);
+    st = check_launch("bp_adjoint_split_kernel");
+    if (st) return st;
+    return adjoint_reduce(a, tiles, partials, grad_phase, grad_pos, hs);
+}
+
 extern "C" int bpmi_adjoint_workspace_bytes(long long npixels, int pulses, size_t* bytes) {
     if (!bytes) return set_error(BPMI_ERR_INVALID, "bytes is null");
     if (npixels < 0 || pulses < 0) return set_error(BPMI_ERR_INVALID, "bad sizes");
-    *bytes = (size_t)((npixels + ADJ_TILE - 1) / ADJ_TILE) * (size_t)pulses * 5 * sizeof(float);
+    *bytes = adjoint_partials_bytes((npixels + ADJ_TILE - 1) / ADJ_TILE, pulses);
+    return BPMI_OK;
+}
+
+extern "C" int bpmi_grid_adjoint_workspace_bytes(int nx, int ny, int pulses, size_t* bytes) {
+    if (!bytes) return set_error(BPMI_ERR_INVALID, "bytes is null");
+    if (nx < 0 || ny < 0 || pulses < 0) return set_error(BPMI_ERR_INVALID, "bad sizes");
+    long long tiles = 0;
+    for (int l = 3; l <= 6; ++l) {
+        const long long t = tiles_for(ny, nx, l);
+        tiles = t > tiles ? t : tiles;
+    }
+    *bytes = adjoint_partials_bytes(tiles, pulses);
     return BPMI_OK;
 }
 
@@ -551,8 +816,8 @@ extern "C" int bpmi_bp_grid_adjoint(const void* data, int samples, const double*
     BpArgs a{(const float2*)data, pos, weight, phase, samples, pulse_begin, pulse_end,
              range_start, data_dr, k, 0, nullptr};
     SrcArgs s{0.0, 0.0, z0, x0, dx, y0, dy, ny, nx, 1, nullptr, 0};
-    return adjoint_launch(interp, SRC_GRID, a, s, (long long)nx * ny, grad_image, partials,
-                          partials_bytes, grad_phase, grad_pos, (hipStream_t)stream);
+    return adjoint_grid_launch(interp, a, s, grad_image, partials, partials_bytes, grad_phase,
+                               grad_pos, (hipStream_t)stream);
 }
 
 extern "C" int bpmi_bp_points_adjoint(const void* data, int samples, const double* pos,
@@ -570,6 +835,6 @@ extern "C" int bpmi_bp_points_adjoint(const void* data, int samples, const doubl
     BpArgs a{(const float2*)data, pos, weight, phase, samples, pulse_begin, pulse_end,
              range_start, data_dr, k, 0, nullptr};
     SrcArgs s{0, 0, 0, 0, 0, 0, 0, 0, 0, 1, xyz, npoints};
-    return adjoint_launch(interp, SRC_POINTS, a, s, npoints, grad_points, partials,
-                          partials_bytes, grad_phase, grad_pos, (hipStream_t)stream);
+    return adjoint_points_launch(interp, a, s, npoints, grad_points, partials, partials_bytes,
+                                 grad_phase, grad_pos, (hipStream_t)stream);
 }

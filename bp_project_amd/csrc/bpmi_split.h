@@ -381,6 +381,159 @@ __device__ __forceinline__ void split_pixel64(const SrcArgs& s, long long p, int
     }
 }
 
+// Tile reference from the bounding box of the tile's pixels (wave minima,
+// then thread 0 of the workgroup): centre, half extents, radius, slant-range
+// span and integer sample base.  All threads; two barriers.
+__device__ __forceinline__ void tile_reference_bbox(const BpArgs& a, double lo_x, double lo_y,
+                                                    double lo_z, double hi_x, double hi_y,
+                                                    double hi_z, double rmin, double rmax,
+                                                    double (*red)[8], TileRef& ref, int t) {
+    const int lane = t & 63, wave = t >> 6;
+    double v[8] = {lo_x, lo_y, lo_z, -hi_x, -hi_y, -hi_z, rmin, -rmax};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = wave_min(v[j]);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) red[wave][j] = v[j];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double m[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            m[j] = fmin(fmin(red[0][j], red[1][j]), fmin(red[2][j], red[3][j]));
+        ref.cx = 0.5 * (m[0] - m[3]);
+        ref.cy = 0.5 * (m[1] - m[4]);
+        ref.cz = 0.5 * (m[2] - m[5]);
+        ref.hx = 0.5 * (-m[3] - m[0]);
+        ref.hy = 0.5 * (-m[4] - m[1]);
+        ref.hz = 0.5 * (-m[5] - m[2]);
+        ref.rho = sqrt(ref.hx * ref.hx + ref.hy * ref.hy + ref.hz * ref.hz) * (1.0 + 1e-12) + 1e-9;
+        ref.rmin = m[6];
+        ref.rmax = -m[7];
+        const double it = floor((m[6] - a.range_start) / a.data_dr);
+        ref.itile = (it > -1.0e9 && it < 1.0e9) ? (int)it : 0;
+    }
+    __syncthreads();
+}
+
+// Per-pulse parameters of pulse n for a tile with reference ``ref`` (FP64;
+// one thread per pulse): path flag, window base and the reference-point
+// range and phase terms.  vlo / vhi: first and last valid stencil index.
+template <bool ORIGIN_REF>
+__device__ __forceinline__ PulseParams split_pulse_params(const BpArgs& a, const TileRef& ref,
+                                                          int n, double kt, double inv_ddr,
+                                                          int vlo, int vhi) {
+    const double cx = ref.cx, cy = ref.cy, cz = ref.cz;
+    PulseParams q;
+    const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
+    const double ux = cx - ax, uy = cy - ay, uz = cz - az;
+    const double D = sqrt64_fast(ux * ux + uy * uy + uz * uz);
+    const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
+    q.w = (float)pulse_weight(a.weight, n);
+    q.pulse = n;
+    double qlo, qhi, err, err2 = 1e30;
+    bool near;
+    if (ORIGIN_REF) {
+        q.vx = ux; q.vy = uy; q.vz = uz;
+        q.P = ph;
+        q.q2 = (float)(D * D);
+        q.scale = 0.f;
+        q.invr = 0.f;
+        qlo = (ref.rmin - D - a.range_start) * inv_ddr;
+        qhi = (ref.rmax + D - a.range_start) * inv_ddr;
+        const double rm = ref.rmin - D;
+        near = rm <= D || ref.rmin <= 0.0;
+        const double mm = D / fmax(ref.rmin, 1e-30);
+        err = near ? 1e30 : D * D * D * D / (8.0 * rm * rm * rm) + mm * mm * D * D / (2.0 * ref.rmin * (1.0 - mm));
+    } else {
+        const double inv = rcp64(D);
+        q.vx = ux * inv; q.vy = uy * inv; q.vz = uz * inv;
+        q.P = fract64(kt * D + ph);
+        q.q2 = 0.f;
+        q.scale = (float)(0.5 * inv);
+        q.invr = (float)inv;
+        const double B = fabs(q.vx) * ref.hx + fabs(q.vy) * ref.hy + fabs(q.vz) * ref.hz;
+        const double rho = ref.rho;
+        near = D <= 2.0 * rho;
+        const double extra = near ? 0.0 : rho * rho / (2.0 * (D - rho));
+        qlo = (D - B - 1e-6 - a.range_start) * inv_ddr;
+        qhi = (D + B + extra + 1e-6 - a.range_start) * inv_ddr;
+        // Truncation bound of the fourth-order range expansion used below:
+        // sqrt tail h^3 / (16 L^5), 1/(1+m) tail after m^2, and the
+        // (1+m)^-3 factor dropped from the h^2 term.
+        const double dm = D - rho, mm = rho * inv, r2 = rho * rho;
+        const double om = 1.0 - mm;
+        err = near ? 1e30
+                   : r2 * r2 * r2 / (16.0 * dm * dm * dm * dm * dm) +
+                         r2 * 0.5 * inv * mm * mm * mm / om +
+                         r2 * r2 * 0.125 * inv * inv * inv * 3.0 * mm / (om * om * om * om);
+        // second-order residual hs (1 - m): sqrt tail h^2 / (8 L^3) and
+        // the 1/(1+m) tail after m
+        err2 = near ? 1e30 : r2 * r2 / (8.0 * dm * dm * dm) + r2 * 0.5 * inv * mm * mm / om;
+    }
+    int flag = PF_FAST4;
+    int ilo = 0;
+    if (!(qlo > -1.0e9 && qhi < 1.0e9)) {
+        flag = PF_SLOW;
+    } else {
+        ilo = (int)floor(qlo) - 1;
+        const int ihi = (int)floor(qhi) + 1;
+        if (ihi < vlo || ilo > vhi) flag = PF_SKIP;
+        else if (ihi - ilo + 1 > SP_W || near || fabs(kt) * err > 1e-6) flag = PF_SLOW;
+        else if (ORIGIN_REF || fabs(kt) * err2 <= 1e-6) flag = PF_FAST2;
+    }
+    if (flag <= PF_FAST2) {
+        if (ORIGIN_REF) q.qb = (float)(ref.itile - ilo);
+        else q.qb = (float)((D - a.range_start) / a.data_dr - (double)ilo);
+    } else {
+        q.qb = 0.f;
+    }
+    q.flag = flag;
+    q.ilo = ilo;
+    return q;
+}
+
+// Stage the range windows of a pulse block (nb pulses, parameters prm) as
+// weighted polynomial coefficients: slot j of pulse b holds the cubic (or
+// line) through the taps around sample prm[b].ilo + j, zero where the
+// stencil leaves the valid range [vlo, vhi].  All threads of the workgroup.
+template <int INTERP>
+__device__ __forceinline__ void stage_pulse_block(const BpArgs& a, const PulseParams* prm,
+                                                  float4* coef, int nb, int vlo, int vhi,
+                                                  int t) {
+    constexpr int F4 = SlotSize<INTERP>::F4;
+    const int S = a.samples;
+    for (int sidx = t; sidx < nb * SP_W; sidx += SP_THREADS) {
+        const int b = sidx / SP_W, j = sidx - b * SP_W;
+        const PulseParams& q = prm[b];
+        float4 c01 = make_float4(0.f, 0.f, 0.f, 0.f), c23 = c01;
+        const int i = q.ilo + j;
+        if (q.flag > PF_FAST2) continue;
+        if (i >= vlo && i <= vhi) {
+            const float2* row = a.data + (size_t)q.pulse * S;
+            const float w = q.w;
+            if (INTERP == INTERP_LAGRANGE4) {
+                const float2 vm = row[i - 1], v0 = row[i], v1 = row[i + 1], v2 = row[i + 2];
+                // p(t) = v0 + c1 t + c2 t^2 + c3 t^3 through taps at -1, 0, 1, 2
+                const float c1x = -vm.x * (1.f / 3.f) - 0.5f * v0.x + v1.x - v2.x * (1.f / 6.f);
+                const float c1y = -vm.y * (1.f / 3.f) - 0.5f * v0.y + v1.y - v2.y * (1.f / 6.f);
+                const float c2x = 0.5f * vm.x - v0.x + 0.5f * v1.x;
+                const float c2y = 0.5f * vm.y - v0.y + 0.5f * v1.y;
+                const float c3x = (v2.x - vm.x) * (1.f / 6.f) + 0.5f * (v0.x - v1.x);
+                const float c3y = (v2.y - vm.y) * (1.f / 6.f) + 0.5f * (v0.y - v1.y);
+                c01 = make_float4(w * v0.x, w * v0.y, w * c1x, w * c1y);
+                c23 = make_float4(w * c2x, w * c2y, w * c3x, w * c3y);
+            } else {
+                const float2 v0 = row[i], v1 = row[i + 1];
+                c01 = make_float4(w * v0.x, w * v0.y, w * (v1.x - v0.x), w * (v1.y - v0.y));
+            }
+        }
+        coef[(b * SP_W + j) * F4] = c01;
+        if (F4 == 2) coef[(b * SP_W + j) * F4 + 1] = c23;
+    }
+}
+
 // LDS of one split tile: staged window coefficients of a pulse block, the
 // block's per-pulse parameters, the tile reference and a reduction scratch.
 template <int INTERP>
@@ -450,32 +603,7 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
         }
         __syncthreads();
     } else {
-        double v[8] = {lo_x, lo_y, lo_z, -hi_x, -hi_y, -hi_z, rmin, -rmax};
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = wave_min(v[j]);
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) red[wave][j] = v[j];
-        }
-        __syncthreads();
-        if (t == 0) {
-            double m[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                m[j] = fmin(fmin(red[0][j], red[1][j]), fmin(red[2][j], red[3][j]));
-            ref.cx = 0.5 * (m[0] - m[3]);
-            ref.cy = 0.5 * (m[1] - m[4]);
-            ref.cz = 0.5 * (m[2] - m[5]);
-            ref.hx = 0.5 * (-m[3] - m[0]);
-            ref.hy = 0.5 * (-m[4] - m[1]);
-            ref.hz = 0.5 * (-m[5] - m[2]);
-            ref.rho = sqrt(ref.hx * ref.hx + ref.hy * ref.hy + ref.hz * ref.hz) * (1.0 + 1e-12) + 1e-9;
-            ref.rmin = m[6];
-            ref.rmax = -m[7];
-            const double it = floor((m[6] - a.range_start) / a.data_dr);
-            ref.itile = (it > -1.0e9 && it < 1.0e9) ? (int)it : 0;
-        }
-        __syncthreads();
+        tile_reference_bbox(a, lo_x, lo_y, lo_z, hi_x, hi_y, hi_z, rmin, rmax, red, ref, t);
     }
     const double cx = ref.cx, cy = ref.cy, cz = ref.cz;
     // Per-pixel state.
@@ -513,106 +641,10 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
     for (int nb0 = a.pulse_begin; nb0 < a.pulse_end; nb0 += SP_NB) {
         const int nb = min(SP_NB, a.pulse_end - nb0);
         // ---- per-pulse parameters (FP64, one thread per pulse) ------------
-        if (t < nb) {
-            const int n = nb0 + t;
-            PulseParams q;
-            const double ax = a.pos[3 * n], ay = a.pos[3 * n + 1], az = a.pos[3 * n + 2];
-            const double ux = cx - ax, uy = cy - ay, uz = cz - az;
-            const double D = sqrt64_fast(ux * ux + uy * uy + uz * uz);
-            const double ph = pulse_phase(a.phase, n) * INV_TWO_PI;
-            q.w = (float)pulse_weight(a.weight, n);
-            q.pulse = n;
-            double qlo, qhi, err, err2 = 1e30;
-            bool near;
-            if (ORIGIN_REF) {
-                q.vx = ux; q.vy = uy; q.vz = uz;
-                q.P = ph;
-                q.q2 = (float)(D * D);
-                q.scale = 0.f;
-                q.invr = 0.f;
-                qlo = (ref.rmin - D - a.range_start) * inv_ddr;
-                qhi = (ref.rmax + D - a.range_start) * inv_ddr;
-                const double rm = ref.rmin - D;
-                near = rm <= D || ref.rmin <= 0.0;
-                const double mm = D / fmax(ref.rmin, 1e-30);
-                err = near ? 1e30 : D * D * D * D / (8.0 * rm * rm * rm) + mm * mm * D * D / (2.0 * ref.rmin * (1.0 - mm));
-            } else {
-                const double inv = rcp64(D);
-                q.vx = ux * inv; q.vy = uy * inv; q.vz = uz * inv;
-                q.P = fract64(kt * D + ph);
-                q.q2 = 0.f;
-                q.scale = (float)(0.5 * inv);
-                q.invr = (float)inv;
-                const double B = fabs(q.vx) * ref.hx + fabs(q.vy) * ref.hy + fabs(q.vz) * ref.hz;
-                const double rho = ref.rho;
-                near = D <= 2.0 * rho;
-                const double extra = near ? 0.0 : rho * rho / (2.0 * (D - rho));
-                qlo = (D - B - 1e-6 - a.range_start) * inv_ddr;
-                qhi = (D + B + extra + 1e-6 - a.range_start) * inv_ddr;
-                // Truncation bound of the fourth-order range expansion used below:
-                // sqrt tail h^3 / (16 L^5), 1/(1+m) tail after m^2, and the
-                // (1+m)^-3 factor dropped from the h^2 term.
-                const double dm = D - rho, mm = rho * inv, r2 = rho * rho;
-                const double om = 1.0 - mm;
-                err = near ? 1e30
-                           : r2 * r2 * r2 / (16.0 * dm * dm * dm * dm * dm) +
-                                 r2 * 0.5 * inv * mm * mm * mm / om +
-                                 r2 * r2 * 0.125 * inv * inv * inv * 3.0 * mm / (om * om * om * om);
-                // second-order residual hs (1 - m): sqrt tail h^2 / (8 L^3) and
-                // the 1/(1+m) tail after m
-                err2 = near ? 1e30 : r2 * r2 / (8.0 * dm * dm * dm) + r2 * 0.5 * inv * mm * mm / om;
-            }
-            int flag = PF_FAST4;
-            int ilo = 0;
-            if (!(qlo > -1.0e9 && qhi < 1.0e9)) {
-                flag = PF_SLOW;
-            } else {
-                ilo = (int)floor(qlo) - 1;
-                const int ihi = (int)floor(qhi) + 1;
-                if (ihi < vlo || ilo > vhi) flag = PF_SKIP;
-                else if (ihi - ilo + 1 > SP_W || near || fabs(kt) * err > 1e-6) flag = PF_SLOW;
-                else if (ORIGIN_REF || fabs(kt) * err2 <= 1e-6) flag = PF_FAST2;
-            }
-            if (flag <= PF_FAST2) {
-                if (ORIGIN_REF) q.qb = (float)(ref.itile - ilo);
-                else q.qb = (float)((D - a.range_start) / a.data_dr - (double)ilo);
-            } else {
-                q.qb = 0.f;
-            }
-            q.flag = flag;
-            q.ilo = ilo;
-            prm[t] = q;
-        }
+        if (t < nb) prm[t] = split_pulse_params<ORIGIN_REF>(a, ref, nb0 + t, kt, inv_ddr, vlo, vhi);
         __syncthreads();
         // ---- stage range windows as polynomial coefficients ----------------
-        for (int sidx = t; sidx < nb * SP_W; sidx += SP_THREADS) {
-            const int b = sidx / SP_W, j = sidx - b * SP_W;
-            const PulseParams& q = prm[b];
-            float4 c01 = make_float4(0.f, 0.f, 0.f, 0.f), c23 = c01;
-            const int i = q.ilo + j;
-            if (q.flag > PF_FAST2) continue;
-            if (i >= vlo && i <= vhi) {
-                const float2* row = a.data + (size_t)q.pulse * S;
-                const float w = q.w;
-                if (INTERP == INTERP_LAGRANGE4) {
-                    const float2 vm = row[i - 1], v0 = row[i], v1 = row[i + 1], v2 = row[i + 2];
-                    // p(t) = v0 + c1 t + c2 t^2 + c3 t^3 through taps at -1, 0, 1, 2
-                    const float c1x = -vm.x * (1.f / 3.f) - 0.5f * v0.x + v1.x - v2.x * (1.f / 6.f);
-                    const float c1y = -vm.y * (1.f / 3.f) - 0.5f * v0.y + v1.y - v2.y * (1.f / 6.f);
-                    const float c2x = 0.5f * vm.x - v0.x + 0.5f * v1.x;
-                    const float c2y = 0.5f * vm.y - v0.y + 0.5f * v1.y;
-                    const float c3x = (v2.x - vm.x) * (1.f / 6.f) + 0.5f * (v0.x - v1.x);
-                    const float c3y = (v2.y - vm.y) * (1.f / 6.f) + 0.5f * (v0.y - v1.y);
-                    c01 = make_float4(w * v0.x, w * v0.y, w * c1x, w * c1y);
-                    c23 = make_float4(w * c2x, w * c2y, w * c3x, w * c3y);
-                } else {
-                    const float2 v0 = row[i], v1 = row[i + 1];
-                    c01 = make_float4(w * v0.x, w * v0.y, w * (v1.x - v0.x), w * (v1.y - v0.y));
-                }
-            }
-            coef[(b * SP_W + j) * F4] = c01;
-            if (F4 == 2) coef[(b * SP_W + j) * F4 + 1] = c23;
-        }
+        stage_pulse_block<INTERP>(a, prm, coef, nb, vlo, vhi, t);
         __syncthreads();
         // ---- accumulate ------------------------------------------------------
         // One pass per path (staged window with the second- or fourth-order

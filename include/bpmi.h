@@ -141,8 +141,15 @@ int bpmi_target_contributions(const void* data, int samples, const double* pos, 
  * (complex64, PyTorch convention dL = Re sum conj(grad) dI), computes
  * grad_phase[n] = dL/dphase_n and grad_pos[n, 3] = dL/dpos_n (FP64, device)
  * for pulses [pulse_begin, pulse_end).  ``partials`` is caller-provided
- * scratch of bpmi_adjoint_workspace_bytes() bytes.  Deterministic. */
+ * scratch (bpmi_grid_adjoint_workspace_bytes() bytes for the grid adjoint,
+ * bpmi_adjoint_workspace_bytes() for points).  Deterministic.  The grid
+ * adjoint runs on the SPLIT forward's tiles (FP64 tile reference, FP32
+ * residual, staged windows); the points adjoint in per-point FP64 geometry. */
 int bpmi_adjoint_workspace_bytes(long long npixels, int pulses, size_t* bytes);
+/* Workspace of bpmi_bp_grid_adjoint for an nx x ny grid (its 2-D tiles need
+ * more partial rows than npixels / 1024 when the grid is not a multiple of
+ * the tile shape); bpmi_adjoint_workspace_bytes sizes bpmi_bp_points_adjoint. */
+int bpmi_grid_adjoint_workspace_bytes(int nx, int ny, int pulses, size_t* bytes);
 int bpmi_bp_grid_adjoint(const void* data, int samples, const double* pos, const double* weight,
                          const double* phase, int pulse_begin, int pulse_end, double x0,
                          double dx, int nx, double y0, double dy, int ny, double z0,
