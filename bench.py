@@ -435,6 +435,22 @@ def run_autofocus(args, scene):
     b.record()
     synchronize()
     fp64_ms = elapsed_ms(a, b) / 3
+    # the two HIP launches of a gradient step on their own: split forward BP, adjoint
+    rng = np.random.default_rng(1)
+    gimg = torch.from_numpy((rng.standard_normal((scene.grid.ny, scene.grid.nx)) + 1j *
+                             rng.standard_normal((scene.grid.ny, scene.grid.nx)))
+                            .astype(np.complex64)).cuda()
+    kernel_ms = {}
+    for name, fn in (("forward_bp_ms", lambda: op.image(phase, pos)),
+                     ("adjoint_ms", lambda: op.adjoint(phase, pos, gimg))):
+        fn()
+        synchronize()
+        a.record()
+        for _ in range(5):
+            fn()
+        b.record()
+        synchronize()
+        kernel_ms[name] = round(elapsed_ms(a, b) / 5, 3)
     # central differences: 2 images per parameter, 4 parameters (phase, xyz) per pulse
     fd_step_ms = 2 * 4 * scene.pulses * fp64_ms
     with torch.no_grad():
@@ -451,7 +467,7 @@ def run_autofocus(args, scene):
             "iterations": len(res["ms_per_iteration"]),
             "entropy_initial": res["entropy"][0], "entropy_final": res["best_entropy"],
             "entropy_with_true_phase": true_entropy,
-            "fp64_bp_image_ms": round(fp64_ms, 3),
+            "fp64_bp_image_ms": round(fp64_ms, 3), **kernel_ms,
             "fd_fp64_gradient_step_ms_extrapolated": round(fd_step_ms, 1),
             "speedup_vs_fd_fp64": round(fd_step_ms / ms_step, 1),
             "note": "FD baseline = 2 x 4 x pulses FP64 GPU BP images per gradient step "

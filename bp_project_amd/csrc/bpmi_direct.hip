@@ -485,11 +485,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void bp_adjoint_kernel(BpArgs a, SrcAr
 // coefficients (stage_pulse_block).  Per pixel and pulse: the FP64 linear
 // range term and its phase modulo one turn, the FP32 range residual, the
 // staged polynomial's value v and derivative dv at the sample coordinate,
-// and the five products of bp_adjoint_kernel, with 1/d from the tile's 1/D
-// by a second-order expansion in (d - D)/D (|d - D| <= tile radius, so the
-// relative error is below (rho/D)^3).  Pulses whose bound fails (PF_SLOW)
-// take bp_adjoint_kernel's exact FP64 per-pixel path.  Sums are reduced per
-// ADJ_NB pulses through LDS into partials[tile][pulse][5] as there.
+// and the gradient products of bp_adjoint_kernel, with 1/d from the tile's
+// 1/D by a first-order expansion in (d - D)/D (|d - D| <= tile radius rho,
+// relative error below (rho/D)^2).  Pulses whose bound fails (PF_SLOW) take
+// bp_adjoint_kernel's exact FP64 per-pixel path.  Per ADJ_NB pulses the
+// workgroup reduces its sums through LDS and writes partials[tile][pulse][5]
+// in bp_adjoint_kernel's layout.
 constexpr int ADJ_SNB = 16;
 
 template <int INTERP>
@@ -499,7 +500,7 @@ struct AdjSplitLds {
     float4 off[ADJ_SNB];  // (c - a) of each pulse in FP32
     TileRef ref;
     double red8[4][8];
-    float red[ADJ_ROWS][ADJ_THREADS + 1];
+    float4 red[ADJ_ROWS][ADJ_THREADS / 4 + 1];  // rows 16-byte aligned
 };
 
 template <int INTERP>
@@ -567,6 +568,8 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
         stage_pulse_block<INTERP>(a, lds.prm, lds.coef, nb, vlo, vhi, t);
         __syncthreads();
         for (int j0 = 0; j0 < nb; j0 += ADJ_NB) {
+            // per pulse: A = sum conj(g) c (re, im), then sum b (x - c) over
+            // x and y, and sum b; G = -(sum b (x - c) + (c - a) sum b)
             float acc[ADJ_NB][ADJ_VALS];
 #pragma unroll
             for (int jj = 0; jj < ADJ_NB; ++jj) {
@@ -577,9 +580,14 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
                 const int flag = uniform_i32(lds.prm[b].flag);
                 if (flag == PF_SKIP) continue;
                 const PulseParams q = lds.prm[b];
-                const float4 off = lds.off[b];
-                if (flag <= PF_FAST2) {
-                    const bool order4 = flag == PF_FAST4;
+                // Per pixel: h = conj(g) e^{j theta}; A += h v;
+                // b = Re(h (dv / data_dr + j k v)) / d = (hr ur - hi ui) / (data_dr d)
+                // with ur = dv.x - kd v.y, ui = dv.y + kd v.x, kd = k data_dr, and
+                // 1 / (data_dr d) = (1 / (data_dr D)) (1 - (d - D) / D) to first
+                // order (relative error below (rho / D)^2).
+                const float kd = (float)(a.k * a.data_dr);
+                const float r0 = invdr * q.invr, r1 = r0 * q.invr;
+                auto fast = [&]<bool ORDER4>() {
                     const float4* cb = lds.coef + b * SP_W * F4;
 #pragma unroll
                     for (int k = 0; k < SP_PPT; ++k) {
@@ -590,7 +598,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
                         const float m = l32 * q.invr;
                         const float hs = h * q.scale;
                         float corr;
-                        if (order4) {
+                        if (ORDER4) {
                             float pz = fmaf(m, m - 1.f, 1.f);
                             pz = fmaf(-hs, q.scale, pz);
                             corr = hs * pz;
@@ -622,24 +630,21 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
                         }
                         float sn, cs;
                         sincos_turns(turns, &sn, &cs);
-                        // z = w v e^{j theta}, dz = w dv e^{j theta} (w is in the coefficients)
-                        const float zx = v.x * cs - v.y * sn, zy = v.x * sn + v.y * cs;
-                        const float dzx = dv.x * cs - dv.y * sn, dzy = dv.x * sn + dv.y * cs;
-                        // A += conj(g) z
-                        acc[jj][0] += g[k].x * zx + g[k].y * zy;
-                        acc[jj][1] += g[k].x * zy - g[k].y * zx;
-                        // Re(conj(g) (dz / data_dr + j k z)) / d
-                        const float ur = dzx * invdr - k32 * zy;
-                        const float ui = dzy * invdr + k32 * zx;
-                        const float e = dl * q.invr;
-                        const float rinv = q.invr * fmaf(e, e - 1.f, 1.f);
-                        const float br = (g[k].x * ur + g[k].y * ui) * rinv;
-                        // (p_n - x) = -((x - c) + (c - a))
-                        acc[jj][2] -= br * (ex32[k] + off.x);
-                        acc[jj][3] -= br * (ey32[k] + off.y);
-                        acc[jj][4] -= br * off.z;
+                        // (w is in the staged coefficients)
+                        const float hr = g[k].x * cs + g[k].y * sn;
+                        const float hi = g[k].x * sn - g[k].y * cs;
+                        acc[jj][0] += hr * v.x - hi * v.y;
+                        acc[jj][1] += hr * v.y + hi * v.x;
+                        const float ur = fmaf(-kd, v.y, dv.x), ui = fmaf(kd, v.x, dv.y);
+                        const float br = (hr * ur - hi * ui) * fmaf(-dl, r1, r0);
+                        acc[jj][2] += br * ex32[k];
+                        acc[jj][3] += br * ey32[k];
+                        acc[jj][4] += br;
                     }
-                } else {
+                };
+                if (flag == PF_FAST4) fast.template operator()<true>();
+                else if (flag == PF_FAST2) fast.template operator()<false>();
+                else {
                     // exact FP64 geometry, samples from global memory
                     const int n = q.pulse;
                     const float2* row = a.data + (size_t)n * S;
@@ -669,27 +674,46 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
                         const float ur = dv.x * invdr - k32 * v.y;
                         const float ui = dv.y * invdr + k32 * v.x;
                         const float br = (hr * ur - hi * ui) * (float)rinv;
-                        acc[jj][2] -= br * (float)ex;
-                        acc[jj][3] -= br * (float)ey;
-                        acc[jj][4] -= br * (float)ez;
+                        acc[jj][2] += br * ex32[k];
+                        acc[jj][3] += br * ey32[k];
+                        acc[jj][4] += br;
                     }
                 }
             }
 #pragma unroll
             for (int jj = 0; jj < ADJ_NB; ++jj)
 #pragma unroll
-                for (int v = 0; v < ADJ_VALS; ++v) lds.red[jj * ADJ_VALS + v][t] = acc[jj][v];
+                for (int v = 0; v < ADJ_VALS; ++v)
+                    reinterpret_cast<float*>(lds.red[jj * ADJ_VALS + v])[t] = acc[jj][v];
             __syncthreads();
-            if (t < 4 * ADJ_ROWS) {
-                const int r = t >> 2, part = t & 3;
+            // Wave jj sums pulse jj's five rows, eight lanes per row (lane
+            // 8 v + part adds columns 32 part .. 32 part + 31 in a fixed
+            // order), then combines the rows within the wave.
+            static_assert(ADJ_NB == ADJ_THREADS / 64, "one wave per pulse of a group");
+            {
+                const int jj = t >> 6, v = (t >> 3) & 7, part = t & 7;
                 float sum = 0.f;
-                for (int c = part; c < ADJ_THREADS; c += 4) sum += lds.red[r][c];
-                sum += __shfl_xor(sum, 1, 4);
-                sum += __shfl_xor(sum, 2, 4);
-                const int jj = r / ADJ_VALS;
-                if (part == 0 && j0 + jj < nb)
-                    partials[((size_t)tile * P + (nb0 + j0 + jj - a.pulse_begin)) * ADJ_VALS +
-                             (r - jj * ADJ_VALS)] = sum;
+                if (v < ADJ_VALS) {
+                    const float4* row = lds.red[jj * ADJ_VALS + v] + 8 * part;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const float4 x = row[c];
+                        sum += (x.x + x.y) + (x.z + x.w);
+                    }
+                }
+                sum += __shfl_xor(sum, 1, 8);
+                sum += __shfl_xor(sum, 2, 8);
+                sum += __shfl_xor(sum, 4, 8);
+                const float sb = __shfl(sum, 8 * 4, 64);
+                if (part == 0 && v < ADJ_VALS && j0 + jj < nb) {
+                    const float4 off = lds.off[j0 + jj];
+                    const float out = v < 2 ? sum
+                                    : v == 2 ? -(sum + off.x * sb)
+                                    : v == 3 ? -(sum + off.y * sb)
+                                             : -off.z * sb;
+                    partials[((size_t)tile * P + (nb0 + j0 + jj - a.pulse_begin)) * ADJ_VALS + v] =
+                        out;
+                }
             }
             __syncthreads();
         }
