@@ -554,7 +554,7 @@ def run_range_compress(args):
     no oversampling.  Roofline: HBM, compulsory 16 B per raw sample (complex64
     read and written once)."""
     from bp_project_amd.device import DeviceArray, Event, elapsed_ms, synchronize
-    from bp_project_amd.sar_io import fft_length, range_compress_device
+    from bp_project_amd.sar_io import fft_length, range_compress_device, transform_length
     pulses, samples, tp = 8192, 8192, 1e-6
     rng = np.random.default_rng(20260919)
     raw = (rng.standard_normal((pulses, samples), np.float32)
@@ -584,6 +584,7 @@ def run_range_compress(args):
                                  "FFT, spectrum product, inverse FFT, crop) over its time"},
             "config": {"workload": "range_compress", "pulses": pulses, "samples": samples,
                        "tp_s": tp, "fs_hz": 400e6, "bandwidth_hz": 200e6, "nfft": nfft,
+                       "transform_length": transform_length(samples, tp),
                        "oversample": 1, "fft": "hipFFT (rocFFT) complex64"}}
 
 
@@ -593,9 +594,15 @@ def cpu_baseline_range_compress():
     rng = np.random.default_rng(1)
     rows, samples = 256, 8192
     raw = (rng.standard_normal((rows, samples)) + 1j * rng.standard_normal((rows, samples)))
+    raw = raw.astype(np.complex64)
+    # repeat the block until ~10 s of CPU work
     t = time.perf_counter()
-    oracle.range_compress(raw.astype(np.complex64))
+    reps = 0
+    while reps == 0 or time.perf_counter() - t < 10.0:
+        oracle.range_compress(raw)
+        reps += 1
     dt = time.perf_counter() - t
+    rows *= reps
     return {"value": round(rows * samples / dt / 1e6, 3), "unit": "Msample/s", "cores": 1,
             "kind": "port", "seconds": round(dt, 2),
             "sample": f"FP64 oracle (scipy.fft, one worker) range compression of {rows} pulses "
@@ -607,16 +614,29 @@ def cpu_baseline_direct(scene, name):
     import oracle
     g = scene.grid
     interp = "linear" if name == "c1_direct_small" else "lagrange4"
-    rows = 256 if name == "c1_direct_small" else 2
-    pts = g.points()[: rows * g.nx]
-    t = time.perf_counter()
-    oracle.backproject_points(scene.data, scene.pos, pts, scene.r_start, scene.data_dr, scene.k,
-                              scene.weight, scene.phase, interp=interp)
-    dt = time.perf_counter() - t
-    return {"value": round(len(pts) * scene.pulses / dt / 1e9, 4), "unit": "Gpix·pulse/s",
-            "cores": oracle.num_threads(), "kind": "port", "seconds": round(dt, 2),
-            "sample": f"FP64 oracle direct BP of {rows} grid rows ({len(pts)} pixels) x "
-                      f"{scene.pulses} pulses"}
+
+    def run(rows):
+        pts = g.points()[: rows * g.nx]
+        t = time.perf_counter()
+        oracle.backproject_points(scene.data, scene.pos, pts, scene.r_start, scene.data_dr,
+                                  scene.k, scene.weight, scene.phase, interp=interp)
+        return len(pts), time.perf_counter() - t
+
+    # a probe of 2 grid rows sizes a sample of ~10 s of CPU work (whole grids
+    # repeated when the grid is smaller than that)
+    n, dt = run(2)
+    rows = max(1, int(2 * 10.0 / max(dt, 1e-3)))
+    reps = -(-rows // g.ny)
+    rows = min(rows, g.ny)
+    total_px = total_s = 0.0
+    for _ in range(reps):
+        n, dt = run(rows)
+        total_px += n
+        total_s += dt
+    return {"value": round(total_px * scene.pulses / total_s / 1e9, 4), "unit": "Gpix·pulse/s",
+            "cores": oracle.num_threads(), "kind": "port", "seconds": round(total_s, 2),
+            "sample": f"FP64 oracle direct BP of {rows} grid rows x {reps} ({int(total_px)} "
+                      f"pixels) x {scene.pulses} pulses"}
 
 
 def main():

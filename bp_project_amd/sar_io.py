@@ -17,6 +17,8 @@ decode and the matched filter on the GPU.
   correlation with the centred Hamming-windowed LFM reference, optional
   Fourier oversampling, crop so that sample k is the pulse centred at raw
   sample k.  ``range_compress_device`` takes and returns device arrays.
+  Without oversampling the FFTs run at ``transform_length`` rather than the
+  reference's next_fast_len: the same linear correlation, to FP32 rounding.
 
 Constants: C, FC = 1.4 GHz, FS = 400 MHz and the frame marker as in
 fast_bp/sar_io.py:12-15.
@@ -174,6 +176,23 @@ def fft_length(samples, tp, fs=FS):
     """nfft of the reference: scipy.fft.next_fast_len(samples + L - 1)."""
     from scipy import fft
     return int(fft.next_fast_len(int(samples) + int(round(tp * fs)) - 1))
+
+
+def transform_length(samples, tp, oversample=1, fs=FS, nfft=None):
+    """FFT length bpmi_range_compress runs: with oversample 1 the smallest
+    2^a m (m = 1, 3, 5, 9) >= samples + L - 1 (the same linear correlation,
+    faster in rocFFT), else nfft (default: the reference's fft_length)."""
+    nfft = fft_length(samples, tp, fs) if nfft is None else int(nfft)
+    if oversample != 1:
+        return nfft
+    need = int(samples) + int(round(tp * fs)) - 1
+    best = None
+    for m in (1, 3, 5, 9):
+        n = m
+        while n < need:
+            n *= 2
+        best = n if best is None else min(best, n)
+    return best
 
 
 def range_compress_device(raw, tp=1e-6, bandwidth=200e6, chirp_sign=-1, oversample=1, *,

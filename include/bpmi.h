@@ -371,8 +371,10 @@ int bpmi_bp_polar_lut(const void* data, int samples, const double* pos, const do
  * times a periodic Hamming window (L = round(tp fs) samples), through
  * zero-padded complex64 FFTs of length nfft (>= samples + L - 1; the
  * reference uses scipy.fft.next_fast_len(samples + L - 1)); oversample > 1
- * Fourier-interpolates the correlation (scipy.signal.resample) before the
- * crop.  out complex64 [pulses, samples * oversample]: sample k is the pulse
+ * Fourier-interpolates the correlation (scipy.signal.resample) of that
+ * length before the crop.  With oversample 1 the transforms use the
+ * smallest 2^a m >= samples + L - 1 (m = 1, 3, 5, 9) instead, which gives
+ * the same linear correlation (to FP32 rounding) faster.  out complex64 [pulses, samples * oversample]: sample k is the pulse
  * centred at raw sample k / oversample.  FFTs run on hipFFT plans cached per
  * size (their twiddle tables are allocated when a plan is first made); the
  * FFT work area and every intermediate are in the caller's workspace of

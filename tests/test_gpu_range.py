@@ -40,6 +40,20 @@ def test_range_compress_vs_oracle(oracle, pulses, samples, oversample, sign, tp)
     assert err <= 1e-5
 
 
+def test_oversample1_transform_length_is_internal():
+    """With oversample 1 the library picks its own transform length: any
+    valid nfft gives the same output bit for bit."""
+    from bp_project_amd.device import DeviceArray
+    from bp_project_amd.sar_io import fft_length, range_compress_device
+    rng = np.random.default_rng(3)
+    raw = (rng.standard_normal((3, 1500)) + 1j * rng.standard_normal((3, 1500))).astype(np.complex64)
+    d = DeviceArray.from_host(raw)
+    n = fft_length(1500, 1e-6)
+    a = range_compress_device(d, nfft=n).get()
+    b = range_compress_device(d, nfft=2 * n + 1).get()
+    assert np.array_equal(a, b)
+
+
 def test_decode_frames_vs_oracle(tmp_path, oracle):
     from bp_project_amd import _lib
     from bp_project_amd.device import DeviceArray

@@ -58,3 +58,16 @@ def test_fft_length_is_scipy_next_fast_len():
     from scipy import fft
     for samples in (4000, 20000, 1234):
         assert fft_length(samples, 1e-6) == fft.next_fast_len(samples + int(round(1e-6 * FS)) - 1)
+
+
+def test_transform_length_rule():
+    """Oversample 1: smallest 2^a m (m = 1, 3, 5, 9) covering samples + L - 1
+    (brute force over all lengths); oversample > 1 keeps the reference nfft."""
+    from bp_project_amd.sar_io import transform_length
+    for samples in (1, 7, 999, 1500, 4096, 8192, 8193):
+        need = samples + 400 - 1
+        fast = [n for n in range(need, 4 * need + 64)
+                if any(n % m == 0 and ((n // m) & (n // m - 1)) == 0 for m in (1, 3, 5, 9))]
+        assert transform_length(samples, 1e-6) == fast[0]
+        assert transform_length(samples, 1e-6, oversample=2) == fft_length(samples, 1e-6)
+    assert transform_length(8192, 1e-6) == 9216
