@@ -53,6 +53,7 @@ MEASURED_TRAFFIC_BYTES = {  # (workload, mode, kernel, cull)
 OPS_LEAF_LAGRANGE4 = 57      # per leaf pixel*pulse (4-point Lagrange)
 OPS_DIRECT_LINEAR = 35       # per pixel*pulse (linear interpolation)
 OPS_MERGE_KAISER8 = 742      # per parent pixel (two 8x8 Kaiser-sinc child regrids)
+OPS_ADJOINT = 69             # per pixel*pulse (split adjoint, fourth-order fast path)
 OPS_MERGE_LAGRANGE4 = 290    # per parent pixel (two 4x4 Lagrange child regrids)
 OPS_EVAL_KAISER8 = 386       # per output pixel (one 8x8 regrid + remodulation)
 
@@ -473,7 +474,16 @@ def run_autofocus(args, scene):
             "note": "FD baseline = 2 x 4 x pulses FP64 GPU BP images per gradient step "
                     "(measured image time x count); adjoint gradients equal the FD ones "
                     "(tests/test_gpu_autofocus.py), so both reach the same entropy"},
-        "roofline": None,
+        "roofline": {
+            "bound": "valu", "achieved": round(pp * OPS_ADJOINT / (kernel_ms["adjoint_ms"] * 1e-3)
+                                               / 1e12, 3),
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(pp * OPS_ADJOINT / (kernel_ms["adjoint_ms"] * 1e-3) / 1e12
+                          / PEAK_FP32_TFLOPS, 4),
+            "traffic": None, "kernel": "bp_adjoint_split_kernel (+ adjoint_reduce_kernel)",
+            "ops_per_unit": OPS_ADJOINT, "unit_of_work": "pixel*pulse",
+            "note": "dominant kernel of a gradient step (the forward BP is the SPLIT "
+                    "kernel of C3); time = HIP events around one adjoint call"},
         "config": {"workload": "c4_autofocus", "pulses": scene.pulses, "bins": scene.bins,
                    "output": [g.ny, g.nx], "pixel_m": g.dx, "mode": args.mode,
                    "optimizer": "Adam(phase lr 0.05, pos lr 5e-4), 30 iterations"},
@@ -625,14 +635,14 @@ def cpu_baseline_direct(scene, name):
     # a probe of 2 grid rows sizes a sample of ~10 s of CPU work (whole grids
     # repeated when the grid is smaller than that)
     n, dt = run(2)
-    rows = max(1, int(2 * 10.0 / max(dt, 1e-3)))
-    reps = -(-rows // g.ny)
-    rows = min(rows, g.ny)
+    rows = min(g.ny, max(1, int(2 * 10.0 / max(dt, 1e-3))))
     total_px = total_s = 0.0
-    for _ in range(reps):
+    reps = 0
+    while reps == 0 or total_s < 8.0:
         n, dt = run(rows)
         total_px += n
         total_s += dt
+        reps += 1
     return {"value": round(total_px * scene.pulses / total_s / 1e9, 4), "unit": "Gpix·pulse/s",
             "cores": oracle.num_threads(), "kind": "port", "seconds": round(total_s, 2),
             "sample": f"FP64 oracle direct BP of {rows} grid rows x {reps} ({int(total_px)} "

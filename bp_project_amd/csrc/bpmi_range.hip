@@ -6,19 +6,24 @@
 // (hipFFT / rocFFT, batched over pulses) exactly as the reference's scipy
 // code: subtract the row mean, FFT to nfft = next_fast_len(samples + L - 1),
 // multiply by the FFT of conj(reversed reference), inverse FFT, crop
-// [L - 1 - L/2, + samples) (with oversample 1 the transforms run at a
-// rocFFT-friendly length >= samples + L - 1 instead: same correlation).  With oversample > 1 the length-nfft correlation
+// [L - 1 - L/2, + samples).  With oversample 1 the transforms run at a
+// length 2^a m >= samples + L - 1 (m = 1, 3, 5, 9; the same correlation)
+// and, up to RF_NMAX points, in one kernel: a workgroup per pulse holds the
+// row in LDS through both transforms (bpmi_fft.h), so HBM sees one read of
+// the raw row and one write of the result.  With oversample > 1 the length-nfft correlation
 // is Fourier-interpolated to nfft * oversample first (scipy.signal.resample:
 // spectrum copied to both ends, the +N/2 bin halved and mirrored to -N/2),
 // which here is done on the product spectrum directly (resample's forward
 // FFT of the inverse FFT is the identity).  The hand-written kernels do the
 // row mean and padding, the reference spectrum's samples, the spectrum
-// product / upsampling layout and the crop; the FFTs are hipFFT plans cached
-// per (size, batch) whose work area is the caller's workspace.
+// product / upsampling layout and the crop; the FFTs of the other cases are
+// hipFFT plans cached per (size, batch) whose work area is the caller's
+// workspace.
 //
 // The IQ decode gathers the int16 I/Q pairs of selected frames of a raw
 // capture (already in device memory) into complex64 rows.
 #include "bpmi_internal.h"
+#include "bpmi_fft.h"
 
 #include <hipfft/hipfft.h>
 
@@ -31,7 +36,8 @@ namespace bpmi {
 constexpr int RC_THREADS = 256;
 
 // One workgroup per pulse: FP64 row mean, then raw - mean into a row of
-// nfft samples padded with zeros.
+// nfft samples padded with zeros.  Rows whose start is 16-byte aligned are
+// read and written two samples at a time.
 __global__ __launch_bounds__(RC_THREADS) void rc_center_pad_kernel(const float2* __restrict__ raw,
                                                                     long long ld_raw, int samples,
                                                                     int nfft,
@@ -39,8 +45,16 @@ __global__ __launch_bounds__(RC_THREADS) void rc_center_pad_kernel(const float2*
     __shared__ double part[2][RC_THREADS / 64];
     const long long p = blockIdx.x;
     const float2* row = raw + p * ld_raw;
+    float2* dst = out + p * nfft;
+    const bool pairs = ((((uintptr_t)row) | ((uintptr_t)dst)) & 15) == 0;
+    const int np = pairs ? samples / 2 : 0;  // whole pairs read as float4
     double sx = 0.0, sy = 0.0;
-    for (int i = threadIdx.x; i < samples; i += RC_THREADS) {
+    for (int i = threadIdx.x; i < np; i += RC_THREADS) {
+        const float4 v = ((const float4*)row)[i];
+        sx += (double)v.x + (double)v.z;
+        sy += (double)v.y + (double)v.w;
+    }
+    for (int i = 2 * np + threadIdx.x; i < samples; i += RC_THREADS) {
         const float2 v = row[i];
         sx += v.x;
         sy += v.y;
@@ -62,8 +76,11 @@ __global__ __launch_bounds__(RC_THREADS) void rc_center_pad_kernel(const float2*
         my += part[1][w];
     }
     const float fx = (float)(mx / samples), fy = (float)(my / samples);
-    float2* dst = out + p * nfft;
-    for (int i = threadIdx.x; i < nfft; i += RC_THREADS) {
+    for (int i = threadIdx.x; i < np; i += RC_THREADS) {
+        const float4 v = ((const float4*)row)[i];
+        ((float4*)dst)[i] = make_float4(v.x - fx, v.y - fy, v.z - fx, v.w - fy);
+    }
+    for (int i = 2 * np + threadIdx.x; i < nfft; i += RC_THREADS) {
         if (i < samples) {
             const float2 v = row[i];
             dst[i] = make_float2(v.x - fx, v.y - fy);
@@ -98,13 +115,13 @@ __global__ void rc_filter_kernel(int L, double chirp_rate, double fs, int nfft,
 // num = N * os (os = 1: in place).  For os > 1: bins [0, N/2] at the start,
 // bins (N/2, N) at the end, the +N/2 bin (N even) halved and mirrored to
 // num - N/2, zeros between (scipy.signal.resample upsampling).
+// Grid: x over the num output bins of a row, y over pulses.
 __global__ void rc_multiply_kernel(const float2* __restrict__ X, const float2* __restrict__ H,
-                                   int N, int os, long long pulses, float2* __restrict__ Y) {
+                                   int N, int os, float2* __restrict__ Y) {
     const long long num = (long long)N * os;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= pulses * num) return;
-    const long long p = idx / num;
-    const long long j = idx - p * num;
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= num) return;
+    const long long p = blockIdx.y;
     const float2* x = X + p * N;
     long long src = -1;
     float scale = 1.f;
@@ -127,18 +144,141 @@ __global__ void rc_multiply_kernel(const float2* __restrict__ X, const float2* _
         const float2 a = x[src], b = H[src];
         v = make_float2((a.x * b.x - a.y * b.y) * scale, (a.x * b.y + a.y * b.x) * scale);
     }
-    Y[idx] = v;
+    Y[p * num + j] = v;
 }
 
-// out[p, i] = y[p, start + i] * scale, i < count.
+// out[p, i] = y[p, start + i] * scale, i < count.  Grid: x over i, y over p.
 __global__ void rc_crop_kernel(const float2* __restrict__ y, long long ldy, int start, int count,
-                               float scale, long long pulses, float2* __restrict__ out) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= pulses * count) return;
-    const long long p = idx / count;
-    const long long i = idx - p * count;
+                               float scale, float2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const long long p = blockIdx.y;
     const float2 v = y[p * ldy + start + i];
-    out[idx] = make_float2(v.x * scale, v.y * scale);
+    out[p * count + i] = make_float2(v.x * scale, v.y * scale);
+}
+
+// ---- fused path (oversample 1, n <= RF_NMAX points) ---------------------------
+// One workgroup per pulse does the whole matched filter in LDS: row mean
+// (FP64), the mean-free row zero-padded to n points, forward FFT, product
+// with H = FFT(h) / n, inverse FFT, crop -- one read of the raw row and one
+// write of the output row from HBM.
+
+// tw[m] = exp(-2 pi i m / n) in FP64, rounded to complex64.
+__global__ void rf_twiddle_kernel(int n, float2* __restrict__ tw) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)n, &s, &c);
+    tw[m] = make_float2((float)c, (float)s);
+}
+
+// H = FFT(h) / n, h as in rc_filter_kernel (one workgroup).
+__global__ __launch_bounds__(RF_THREADS) void rf_filter_kernel(FftPlan1 plan, int L,
+                                                               double chirp_rate, double fs,
+                                                               const float2* __restrict__ tw,
+                                                               float2* __restrict__ H) {
+    __shared__ float2 buf[RF_SLOTS];
+    const int t = threadIdx.x, n = plan.n;
+    for (int i = t; i < n; i += RF_THREADS) {
+        float2 h = make_float2(0.f, 0.f);
+        if (i < L) {
+            const int m = L - 1 - i;
+            const double tr = (double)(m - L / 2) / fs;
+            double sn, cs;
+            sincos(M_PI * chirp_rate * tr * tr, &sn, &cs);
+            const float w = (float)(0.54 - 0.46 * cos(2.0 * M_PI * (double)m / (double)L));
+            h = make_float2((float)cs * w, -((float)sn * w));
+        }
+        buf[lds_index(i)] = h;
+    }
+    __syncthreads();
+    fft_lds<false>(buf, plan, tw, t);
+    const float scale = 1.f / (float)n;
+    for (int i = t; i < n; i += RF_THREADS) {
+        const float2 v = buf[lds_index(i)];
+        H[i] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+__global__ __launch_bounds__(RF_THREADS) void rf_compress_kernel(
+        const float2* __restrict__ raw, long long ld_raw, int samples, FftPlan1 plan,
+        const float2* __restrict__ H, const float2* __restrict__ tw, int offset,
+        float2* __restrict__ out) {
+    __shared__ float2 buf[RF_SLOTS];
+    __shared__ double part[2][RF_THREADS / 64];
+    const int t = threadIdx.x, n = plan.n;
+    const long long p = blockIdx.x;
+    const float2* row = raw + p * ld_raw;
+    double sx = 0.0, sy = 0.0;
+    for (int i = t; i < n; i += RF_THREADS) {
+        float2 v = make_float2(0.f, 0.f);
+        if (i < samples) {
+            v = row[i];
+            sx += v.x;
+            sy += v.y;
+        }
+        buf[lds_index(i)] = v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+    }
+    const int lane = t & 63, wave = t >> 6;
+    if (lane == 0) {
+        part[0][wave] = sx;
+        part[1][wave] = sy;
+    }
+    __syncthreads();
+    double mx = 0.0, my = 0.0;
+#pragma unroll
+    for (int w = 0; w < RF_THREADS / 64; ++w) {
+        mx += part[0][w];
+        my += part[1][w];
+    }
+    const float fx = (float)(mx / samples), fy = (float)(my / samples);
+    for (int i = t; i < samples; i += RF_THREADS) {
+        const float2 v = buf[lds_index(i)];
+        buf[lds_index(i)] = make_float2(v.x - fx, v.y - fy);
+    }
+    __syncthreads();
+    fft_lds<false>(buf, plan, tw, t);
+    for (int i = t; i < n; i += RF_THREADS) buf[lds_index(i)] = cmul(buf[lds_index(i)], H[i]);
+    __syncthreads();
+    fft_lds<true>(buf, plan, tw, t);
+    float2* dst = out + p * samples;
+    for (int i = t; i < samples; i += RF_THREADS) dst[i] = buf[lds_index(offset + i)];
+}
+
+// Radices of the fused path for n points, or false (n above RF_NMAX, with a
+// factor other than 2, 3, 5, or with a single factor 2).  The power of two
+// 2^a (a >= 2) is covered by passes of 8 and 4 only, which bounds the
+// registers a thread stages per pass.
+static bool fused_plan(int n, FftPlan1* plan) {
+    if (n < 2 || n > RF_NMAX) return false;
+    plan->n = n;
+    plan->passes = 0;
+    int m = n, a = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++a;
+    }
+    if (a == 1) return false;
+    auto add = [&](int r) {
+        if (plan->passes == RF_MAX_PASSES) return false;
+        plan->radix[plan->passes++] = r;
+        return true;
+    };
+    int fours = a % 3 == 1 ? 2 : a % 3 == 2 ? 1 : 0;
+    for (int e = a - 2 * fours; e > 0; e -= 3)
+        if (!add(8)) return false;
+    for (; fours > 0; --fours)
+        if (!add(4)) return false;
+    for (int r : {3, 5})
+        while (m % r == 0) {
+            if (!add(r)) return false;
+            m /= r;
+        }
+    return m == 1;
 }
 
 // Frame decode: out[i, s] = (I, Q) int16 pair s of frame index[i]; an index
@@ -214,7 +354,9 @@ static int fft_exec(FftPlan* plan, void* data, void* work, int direction, hipStr
 
 struct RcLayout {
     int n;                        // transform length used
-    size_t h, x, y, work, total;  // byte offsets in the workspace
+    bool fused;                   // one-kernel LDS path (no hipFFT plans)
+    FftPlan1 plan;
+    size_t h, x, y, work, total;  // byte offsets in the workspace (fused: h, x = twiddles)
     FftPlan *filt, *fwd, *inv;
 };
 
@@ -249,6 +391,15 @@ static int rc_layout(int pulses, int samples, int L, int nfft, int os, RcLayout*
                          nfft, samples + L - 1);
     if ((long long)nfft * os > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "nfft too large");
     nfft = lay->n = transform_length(samples, L, nfft, os);
+    lay->fused = os == 1 && fused_plan(nfft, &lay->plan);
+    if (lay->fused) {
+        lay->h = 0;
+        lay->x = align256((size_t)nfft * 8);
+        lay->y = lay->work = lay->x + align256((size_t)nfft * 8);
+        lay->total = lay->work;
+        lay->filt = lay->fwd = lay->inv = nullptr;
+        return BPMI_OK;
+    }
     int st = fft_plan(nfft, 1, &lay->filt);
     if (!st) st = fft_plan(nfft, pulses, &lay->fwd);
     if (!st) st = fft_plan(nfft * os, pulses, &lay->inv);
@@ -311,11 +462,24 @@ int bpmi_range_compress(const void* raw, int pulses, int samples, long long ld_r
                          workspace_bytes, lay.total);
     nfft = lay.n;
     char* ws = (char*)workspace;
+    hipStream_t hs = (hipStream_t)stream;
+    if (lay.fused) {
+        float2* H = (float2*)(ws + lay.h);
+        float2* tw = (float2*)(ws + lay.x);
+        hipLaunchKernelGGL(rf_twiddle_kernel, dim3(grid_1d(nfft, 256)), dim3(256), 0, hs, nfft, tw);
+        hipLaunchKernelGGL(rf_filter_kernel, dim3(1), dim3(RF_THREADS), 0, hs, lay.plan, L,
+                           (double)chirp_sign * bandwidth / tp, fs, (const float2*)tw, H);
+        st = check_launch("rf_filter_kernel");
+        if (st) return st;
+        hipLaunchKernelGGL(rf_compress_kernel, dim3(pulses), dim3(RF_THREADS), 0, hs,
+                           (const float2*)raw, ld_raw, samples, lay.plan, (const float2*)H,
+                           (const float2*)tw, L - 1 - L / 2, (float2*)out);
+        return check_launch("rf_compress_kernel");
+    }
     float2* H = (float2*)(ws + lay.h);
     float2* X = (float2*)(ws + lay.x);
     float2* Y = oversample > 1 ? (float2*)(ws + lay.y) : X;
     void* work = ws + lay.work;
-    hipStream_t hs = (hipStream_t)stream;
     const long long num = (long long)nfft * oversample;
     hipLaunchKernelGGL(rc_filter_kernel, dim3(grid_1d(nfft, 256)), dim3(256), 0, hs, L,
                        (double)chirp_sign * bandwidth / tp, fs, nfft, H);
@@ -327,16 +491,23 @@ int bpmi_range_compress(const void* raw, int pulses, int samples, long long ld_r
     st = check_launch("rc_center_pad_kernel");
     if (!st) st = fft_exec(lay.fwd, X, work, HIPFFT_FORWARD, hs);
     if (st) return st;
-    hipLaunchKernelGGL(rc_multiply_kernel, dim3(grid_1d((long long)pulses * num, 256)), dim3(256),
-                       0, hs, X, H, nfft, oversample, (long long)pulses, Y);
+    constexpr int ROWS_PER_LAUNCH = 65535;  // grid y limit
+    for (int p0 = 0; p0 < pulses; p0 += ROWS_PER_LAUNCH) {
+        const int rows = pulses - p0 < ROWS_PER_LAUNCH ? pulses - p0 : ROWS_PER_LAUNCH;
+        hipLaunchKernelGGL(rc_multiply_kernel, dim3(grid_1d(num, 256), rows), dim3(256), 0, hs,
+                           X + (size_t)p0 * nfft, H, nfft, oversample, Y + (size_t)p0 * num);
+    }
     st = check_launch("rc_multiply_kernel");
     if (!st) st = fft_exec(lay.inv, Y, work, HIPFFT_BACKWARD, hs);
     if (st) return st;
     const int offset = (L - 1 - L / 2) * oversample;
     const int count = samples * oversample;
-    hipLaunchKernelGGL(rc_crop_kernel, dim3(grid_1d((long long)pulses * count, 256)), dim3(256),
-                       0, hs, Y, num, offset, count, (float)(1.0 / nfft), (long long)pulses,
-                       (float2*)out);
+    for (int p0 = 0; p0 < pulses; p0 += ROWS_PER_LAUNCH) {
+        const int rows = pulses - p0 < ROWS_PER_LAUNCH ? pulses - p0 : ROWS_PER_LAUNCH;
+        hipLaunchKernelGGL(rc_crop_kernel, dim3(grid_1d(count, 256), rows), dim3(256), 0, hs,
+                           Y + (size_t)p0 * num, num, offset, count, (float)(1.0 / nfft),
+                           (float2*)out + (size_t)p0 * count);
+    }
     return check_launch("rc_crop_kernel");
 }
 

@@ -374,7 +374,8 @@ int bpmi_bp_polar_lut(const void* data, int samples, const double* pos, const do
  * Fourier-interpolates the correlation (scipy.signal.resample) of that
  * length before the crop.  With oversample 1 the transforms use the
  * smallest 2^a m >= samples + L - 1 (m = 1, 3, 5, 9) instead, which gives
- * the same linear correlation (to FP32 rounding) faster.  out complex64 [pulses, samples * oversample]: sample k is the pulse
+ * the same linear correlation (to FP32 rounding) faster; up to 9408 points
+ * (a != 1) the whole filter then runs in one kernel with the row in LDS.  out complex64 [pulses, samples * oversample]: sample k is the pulse
  * centred at raw sample k / oversample.  FFTs run on hipFFT plans cached per
  * size (their twiddle tables are allocated when a plan is first made); the
  * FFT work area and every intermediate are in the caller's workspace of
