@@ -651,6 +651,8 @@ def cpu_baseline_direct(scene, name):
 
 def main():
     args = parse()
+    if args.persistent:
+        args.cull = "none"  # the persistent schedule computes every tile
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding of this path is a design note, not built)",

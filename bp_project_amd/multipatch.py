@@ -308,7 +308,10 @@ class MultiPatchProgram:
         key = make_key(kernel, self.mode, self.leaf_interp, work)
 
         def run(tc):
-            self.set_tile_width(tc)
+            # the rebuild happens in the untimed warm-up call of a candidate
+            if tc != self.tile_cols_log2:
+                self.set_tile_width(tc)
+                synchronize()
             self.execute()
         best, times = tune(key, list(candidates), run, force=force)
         self.set_tile_width(best)
