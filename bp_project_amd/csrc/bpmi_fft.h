@@ -187,4 +187,114 @@ __device__ __forceinline__ void fft_lds(float2* buf, const FftPlan1& plan,
     }
 }
 
+// ---- plans -------------------------------------------------------------------
+// Radices for n points: the power of two 2^a (a >= 2) by passes of 8 and 4,
+// then the factors 3 and 5.  passes = 0 when n is out of range (above RF_NMAX,
+// a single factor 2, another prime factor, or more than RF_MAX_PASSES passes).
+constexpr FftPlan1 make_fft_plan(int n) {
+    FftPlan1 plan{n, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+    if (n < 2 || n > RF_NMAX) return plan;
+    int m = n, a = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++a;
+    }
+    if (a == 1) return plan;
+    int fours = a % 3 == 1 ? 2 : a % 3 == 2 ? 1 : 0;
+    int radix[16] = {};
+    int np = 0;
+    for (int e = a - 2 * fours; e > 0; e -= 3) radix[np++] = 8;
+    for (; fours > 0; --fours) radix[np++] = 4;
+    while (m % 3 == 0 && np < 16) {
+        radix[np++] = 3;
+        m /= 3;
+    }
+    while (m % 5 == 0 && np < 16) {
+        radix[np++] = 5;
+        m /= 5;
+    }
+    if (m != 1 || np > RF_MAX_PASSES) return plan;
+    for (int i = 0; i < np; ++i) plan.radix[i] = radix[i];
+    plan.passes = np;
+    return plan;
+}
+
+// ---- passes with compile-time sizes ---------------------------------------------
+// The same pass as fft_pass for a transform length N known at compile time:
+// strides, the j mod NS split and twiddle indices fold into constants, and
+// where the padded layout allows (a stride that is a multiple of 16 points, or
+// the unit-stride outputs of the first pass for R in {4, 8}) the R accesses
+// of a butterfly are one base slot plus immediate offsets.
+template <int R, bool INV, int N, int NS>
+__device__ __forceinline__ void fft_pass_c(float2* buf, const float2* __restrict__ tw, int t) {
+    constexpr int NR = N / R;
+    constexpr int STRIDE = N / (NS * R);
+    constexpr int MAXB = (NR + RF_THREADS - 1) / RF_THREADS;
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int j = t + i * RF_THREADS;
+        if (MAXB * RF_THREADS == NR || j < NR) {
+            if constexpr (NR % 16 == 0) {
+                const int b0 = lds_index(j);
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = buf[b0 + r * (NR + NR / 16)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = buf[lds_index(j + r * NR)];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int j = t + i * RF_THREADS;
+        if (MAXB * RF_THREADS == NR || j < NR) {
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+                const int m = k * STRIDE;
+                float2 w[R];
+                w[0] = make_float2(1.f, 0.f);
+                w[1] = twiddle<INV>(tw, m);
+                if constexpr (R > 2) w[2] = twiddle<INV>(tw, 2 * m);
+                if constexpr (R > 3) w[3] = cmul(w[1], w[2]);
+                if constexpr (R > 4) w[4] = twiddle<INV>(tw, 4 * m);
+                if constexpr (R > 5) {
+#pragma unroll
+                    for (int r = 5; r < R; ++r) w[r] = cmul(w[r - 4], w[4]);
+                }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], w[r]);
+            }
+            dft<R, INV>(v[i]);
+            const int base = (j - k) * R + k;
+            if constexpr (NS % 16 == 0) {
+                const int b0 = lds_index(base);
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[b0 + r * (NS + NS / 16)] = v[i][r];
+            } else if constexpr (NS == 1 && 16 % R == 0) {
+                // base = j R: the R outputs share one block of 16 points
+                const int b0 = lds_index(base);
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[b0 + r] = v[i][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[lds_index(base + r * NS)] = v[i][r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <bool INV, int N, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_lds_c(float2* buf, const float2* __restrict__ tw, int t) {
+    constexpr FftPlan1 plan = make_fft_plan(N);
+    static_assert(plan.passes > 0, "transform length without a plan");
+    if constexpr (P < plan.passes) {
+        constexpr int R = plan.radix[P];
+        fft_pass_c<R, INV, N, NS>(buf, tw, t);
+        fft_lds_c<INV, N, P + 1, NS * R>(buf, tw, t);
+    }
+}
+
 }  // namespace bpmi

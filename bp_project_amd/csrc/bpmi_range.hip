@@ -200,13 +200,15 @@ __global__ __launch_bounds__(RF_THREADS) void rf_filter_kernel(FftPlan1 plan, in
     }
 }
 
+// N > 0: transform length fixed at compile time (fft_lds_c); N = 0: plan.n.
+template <int N>
 __global__ __launch_bounds__(RF_THREADS) void rf_compress_kernel(
         const float2* __restrict__ raw, long long ld_raw, int samples, FftPlan1 plan,
         const float2* __restrict__ H, const float2* __restrict__ tw, int offset,
         float2* __restrict__ out) {
     __shared__ float2 buf[RF_SLOTS];
     __shared__ double part[2][RF_THREADS / 64];
-    const int t = threadIdx.x, n = plan.n;
+    const int t = threadIdx.x, n = N > 0 ? N : plan.n;
     const long long p = blockIdx.x;
     const float2* row = raw + p * ld_raw;
     double sx = 0.0, sy = 0.0;
@@ -241,44 +243,47 @@ __global__ __launch_bounds__(RF_THREADS) void rf_compress_kernel(
         buf[lds_index(i)] = make_float2(v.x - fx, v.y - fy);
     }
     __syncthreads();
-    fft_lds<false>(buf, plan, tw, t);
+    if constexpr (N > 0) fft_lds_c<false, N>(buf, tw, t);
+    else fft_lds<false>(buf, plan, tw, t);
     for (int i = t; i < n; i += RF_THREADS) buf[lds_index(i)] = cmul(buf[lds_index(i)], H[i]);
     __syncthreads();
-    fft_lds<true>(buf, plan, tw, t);
+    if constexpr (N > 0) fft_lds_c<true, N>(buf, tw, t);
+    else fft_lds<true>(buf, plan, tw, t);
     float2* dst = out + p * samples;
     for (int i = t; i < samples; i += RF_THREADS) dst[i] = buf[lds_index(offset + i)];
 }
 
-// Radices of the fused path for n points, or false (n above RF_NMAX, with a
-// factor other than 2, 3, 5, or with a single factor 2).  The power of two
-// 2^a (a >= 2) is covered by passes of 8 and 4 only, which bounds the
-// registers a thread stages per pass.
-static bool fused_plan(int n, FftPlan1* plan) {
-    if (n < 2 || n > RF_NMAX) return false;
-    plan->n = n;
-    plan->passes = 0;
-    int m = n, a = 0;
-    while (m % 2 == 0) {
-        m /= 2;
-        ++a;
+// Transform lengths with a compiled kernel (2^a m, m = 1, 3, 5, 9, 1024 to
+// 9216 points: the lengths range compression picks for 600 to 8800 samples
+// of a 400-tap reference); others run the runtime-plan kernel.
+template <int N>
+static void rf_launch(dim3 grid, hipStream_t hs, const float2* raw, long long ld_raw, int samples,
+                      const FftPlan1& plan, const float2* H, const float2* tw, int offset,
+                      float2* out) {
+    hipLaunchKernelGGL(rf_compress_kernel<N>, grid, dim3(RF_THREADS), 0, hs, raw, ld_raw, samples,
+                       plan, H, tw, offset, out);
+}
+
+template <int... Ns>
+struct RfLengths {
+    static void launch(int n, dim3 grid, hipStream_t hs, const float2* raw, long long ld_raw,
+                       int samples, const FftPlan1& plan, const float2* H, const float2* tw,
+                       int offset, float2* out) {
+        const bool done = ((n == Ns ? (rf_launch<Ns>(grid, hs, raw, ld_raw, samples, plan, H, tw,
+                                                      offset, out),
+                                       true)
+                                    : false) ||
+                           ...);
+        if (!done) rf_launch<0>(grid, hs, raw, ld_raw, samples, plan, H, tw, offset, out);
     }
-    if (a == 1) return false;
-    auto add = [&](int r) {
-        if (plan->passes == RF_MAX_PASSES) return false;
-        plan->radix[plan->passes++] = r;
-        return true;
-    };
-    int fours = a % 3 == 1 ? 2 : a % 3 == 2 ? 1 : 0;
-    for (int e = a - 2 * fours; e > 0; e -= 3)
-        if (!add(8)) return false;
-    for (; fours > 0; --fours)
-        if (!add(4)) return false;
-    for (int r : {3, 5})
-        while (m % r == 0) {
-            if (!add(r)) return false;
-            m /= r;
-        }
-    return m == 1;
+};
+using RfCompiled = RfLengths<1024, 1152, 1280, 1536, 2048, 2304, 2560, 3072, 4096, 4608, 5120,
+                             6144, 8192, 9216>;
+
+// Radices of the fused path for n points (make_fft_plan), or false.
+static bool fused_plan(int n, FftPlan1* plan) {
+    *plan = make_fft_plan(n);
+    return plan->passes > 0;
 }
 
 // Frame decode: out[i, s] = (I, Q) int16 pair s of frame index[i]; an index
@@ -471,9 +476,8 @@ int bpmi_range_compress(const void* raw, int pulses, int samples, long long ld_r
                            (double)chirp_sign * bandwidth / tp, fs, (const float2*)tw, H);
         st = check_launch("rf_filter_kernel");
         if (st) return st;
-        hipLaunchKernelGGL(rf_compress_kernel, dim3(pulses), dim3(RF_THREADS), 0, hs,
-                           (const float2*)raw, ld_raw, samples, lay.plan, (const float2*)H,
-                           (const float2*)tw, L - 1 - L / 2, (float2*)out);
+        RfCompiled::launch(nfft, dim3(pulses), hs, (const float2*)raw, ld_raw, samples, lay.plan,
+                           (const float2*)H, (const float2*)tw, L - 1 - L / 2, (float2*)out);
         return check_launch("rf_compress_kernel");
     }
     float2* H = (float2*)(ws + lay.h);

@@ -22,11 +22,14 @@ def test_fractional_delay_kat_on_gpu(oracle, golden):
 
 
 # oversample 1 runs the one-kernel LDS FFT path for transform lengths up to
-# 9408 points -- 1500 -> 2048 (radices 8 8 8 4), 2048 -> 2304 (8 8 4 3 3),
-# 600 -> 1024 (8 8 4 4), 2100 -> 2560 (8 8 8 5) -- and hipFFT above (9000 ->
-# 10240); oversampling always takes the hipFFT path.
+# 9408 points -- kernels compiled for the length: 1500 -> 2048 (radices
+# 8 8 8 4), 2048 -> 2304 (8 8 4 3 3), 600 -> 1024 (8 8 4 4), 2100 -> 2560
+# (8 8 8 5); the runtime-plan kernel: 100 -> 512 (8 8 8), 200 -> 640
+# (8 4 4 5) -- and hipFFT above (9000 -> 10240); oversampling always takes
+# the hipFFT path.
 @pytest.mark.parametrize("pulses,samples,oversample,sign,tp", [
     (7, 1500, 1, -1, 1e-6), (5, 2048, 1, 1, 1e-6), (3, 600, 1, 1, 1e-6),
+    (3, 100, 1, -1, 1e-6), (2, 200, 1, 1, 1e-6),
     (2, 2100, 1, -1, 1e-6), (2, 9000, 1, -1, 1e-6), (3, 1777, 4, -1, 1e-6),
     (4, 1000, 2, 1, 0.5e-6), (2, 999, 3, -1, 0.25e-6)])
 def test_range_compress_vs_oracle(oracle, pulses, samples, oversample, sign, tp):
