@@ -84,6 +84,8 @@ SIGNATURES = {
                                  _vp],
     "bpmi_ffbp_leaves_multi": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp],
     "bpmi_eval_grid_multi": [_vp, _vp, _i, _i, _i, _vp],
+    "bpmi_eval_grid_multi_tiled": [_vp, _vp, _i, _i, _vp],
+    "bpmi_eval_tiles": [_i, _i, POINTER(c_int)],
     "bpmi_fill_range_phase_lut": [_i, _d, _d, _d, _vp, _vp],
     "bpmi_bp_polar_lut": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _d, _d, _d, _vp, _vp,
                           _vp],

@@ -607,7 +607,7 @@ struct Window {
 };
 
 // Window from the corner stencil origins ir[4], iu[4] (INT_MAX if undefined).
-template <int INTERP>
+template <int INTERP, int CAP = MT_CAP>
 __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, int nr, int nu) {
     constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
     constexpr int LIM = 0x3fffffff;  // stencil origins of any real image are far below
@@ -623,7 +623,7 @@ __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, in
     const int r0 = max(rlo - 1 - FIRST, 0), r1 = min(rhi + 1 + (TAPS - 1 - FIRST), nr - 1);
     const int c0 = max(clo - 1 - FIRST, 0), c1 = min(chi + 1 + (TAPS - 1 - FIRST), nu - 1);
     const int h = r1 - r0 + 1, wd = c1 - c0 + 1;
-    w.lds = defined && h > 0 && wd > 0 && h <= MT_CAP && wd <= MT_CAP && h * wd <= MT_CAP;
+    w.lds = defined && h > 0 && wd > 0 && h <= CAP && wd <= CAP && h * wd <= CAP;
     w.r0 = r0;
     w.c0 = c0;
     w.h = w.lds ? h : 0;
@@ -974,6 +974,82 @@ __global__ __launch_bounds__(256) void eval_grid_multi_fast_kernel(
     const int iy = (int)(p / j.nx), ix = (int)(p - (long long)iy * j.nx);
     ((float2*)j.out)[p] = eval_fast<INTERP>(make_fast_src((const float2*)j.img, j.geo, j.nr, j.nu, j.ld),
                                             j.x0 + ix * j.dx, j.y0 + iy * j.dy, j.k * INV_TWO_PI);
+}
+
+// ---- LDS-tiled Cartesian evaluation (split mode) ------------------------------
+// A workgroup owns ET_X x ET_Y output pixels (one per thread, x fastest: 64
+// pixels along x, the across-track axis the polar range runs along, so the
+// tile spans few angular cells).  As in the tiled merge, the stencil origins
+// of the tile's corners, widened by one cell, bound its footprint in the
+// polar image; that window (up to ET_CAP samples) is staged in LDS and each
+// pixel's 8x8 (4x4) stencil reads it; a pixel outside the window, or a
+// window above ET_CAP, reads global memory with identical arithmetic, so the
+// output equals eval_grid_fast_kernel's bit for bit.
+constexpr int ET_X = 64, ET_Y = 4, ET_CAP = 4096;
+static_assert(ET_X * ET_Y == MT_THREADS, "one pixel per thread");
+
+struct EvalLds {
+    float2 sh[ET_CAP];
+    int corner[4][2];
+    float kpoly[4 * KPOLY_STRIDE];
+};
+
+// Tile (tx, ty) of an nx x ny grid over the polar view f; every thread of
+// the workgroup calls it (two barriers).
+template <int INTERP>
+__device__ __forceinline__ void eval_tile(const FastSrc& f, double x0, double dx, int nx, double y0,
+                                          double dy, int ny, double kt, int tx, int ty,
+                                          float2* __restrict__ out, EvalLds& lds) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < 4) {
+        const int cx = min(tx * ET_X + (t & 1) * (ET_X - 1), nx - 1);
+        const int cy = min(ty * ET_Y + (t >> 1) * (ET_Y - 1), ny - 1);
+        const ChildTap c = child_tap<INTERP>(f, x0 + cx * dx, y0 + cy * dy, 0.0, kt);
+        lds.corner[t][0] = c.ir;
+        lds.corner[t][1] = c.iu;
+    }
+    if (t < 4 * KPOLY_STRIDE) lds.kpoly[t] = c_kaiser_poly[0][t];
+    __syncthreads();
+    int ir[4], iu[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ir[j] = uniform_i32(lds.corner[j][0]);
+        iu[j] = uniform_i32(lds.corner[j][1]);
+    }
+    const Window w = corner_window<INTERP, ET_CAP>(ir, iu, f.nr, f.nu);
+    if (w.lds) stage_window(f, w, lds.sh, wave, lane);
+    __syncthreads();
+    const int ix = tx * ET_X + (t % ET_X), iy = ty * ET_Y + t / ET_X;
+    if (ix < nx && iy < ny) {
+        const ChildTap c = child_tap<INTERP>(f, x0 + ix * dx, y0 + iy * dy, 0.0, kt);
+        out[(long long)iy * nx + ix] = tile_sample<INTERP>(f, c, w, lds.sh, lds.kpoly);
+    }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(MT_THREADS) void eval_grid_tiled_kernel(
+        const float2* __restrict__ img, Geo g, int nr, int nu, int ld, double x0, double dx, int nx,
+        double y0, double dy, int ny, double k, float2* __restrict__ out) {
+    __shared__ EvalLds lds;
+    const int tiles_x = (nx + ET_X - 1) / ET_X;
+    const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+    const double gv[7] = {g.ox, g.oy, g.oz, g.r0, g.dr, g.u0, g.du};
+    eval_tile<INTERP>(make_fast_src(img, gv, nr, nu, ld), x0, dx, nx, y0, dy, ny, k * INV_TWO_PI,
+                      tx, ty, out, lds);
+}
+
+// Many jobs: blocks[b] = (job, tile) with tiles numbered row-major over the
+// job's grid (bpmi_eval_tiles).
+template <int INTERP>
+__global__ __launch_bounds__(MT_THREADS) void eval_grid_multi_tiled_kernel(
+        const bpmi_eval_job* __restrict__ jobs, const int* __restrict__ blocks) {
+    __shared__ EvalLds lds;
+    const bpmi_eval_job& j = jobs[blocks[2 * blockIdx.x]];
+    const int tile = blocks[2 * blockIdx.x + 1];
+    const int tiles_x = (j.nx + ET_X - 1) / ET_X;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    eval_tile<INTERP>(make_fast_src((const float2*)j.img, j.geo, j.nr, j.nu, j.ld), j.x0, j.dx,
+                      j.nx, j.y0, j.dy, j.ny, j.k * INV_TWO_PI, tx, ty, (float2*)j.out, lds);
 }
 
 // Chebyshev interpolation (9 nodes) of the normalised 8-tap Kaiser weights
@@ -1792,13 +1868,16 @@ int bpmi_eval_grid(const void* img, const double* geo, int nr, int nu, int ld, d
     const dim3 gr(grid_1d((long long)nx * ny, 256));
     hipStream_t hs = (hipStream_t)stream;
     if (mode == BPMI_MODE_SPLIT) {
+        const long long tiles = (long long)((nx + ET_X - 1) / ET_X) * ((ny + ET_Y - 1) / ET_Y);
+        if (tiles > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "grid too large");
+        const dim3 gt((unsigned)tiles);
         if (interp == BPMI_INTERP_KAISER8)
-            hipLaunchKernelGGL(eval_grid_fast_kernel<INTERP_KAISER8>, gr, dim3(256), 0, hs, im, g,
-                               nr, nu, ld, x0, dx, nx, y0, dy, ny, k, o);
+            hipLaunchKernelGGL(eval_grid_tiled_kernel<INTERP_KAISER8>, gt, dim3(MT_THREADS), 0, hs,
+                               im, g, nr, nu, ld, x0, dx, nx, y0, dy, ny, k, o);
         else
-            hipLaunchKernelGGL(eval_grid_fast_kernel<INTERP_LAGRANGE4>, gr, dim3(256), 0, hs, im, g,
-                               nr, nu, ld, x0, dx, nx, y0, dy, ny, k, o);
-        return check_launch("eval_grid_fast_kernel");
+            hipLaunchKernelGGL(eval_grid_tiled_kernel<INTERP_LAGRANGE4>, gt, dim3(MT_THREADS), 0,
+                               hs, im, g, nr, nu, ld, x0, dx, nx, y0, dy, ny, k, o);
+        return check_launch("eval_grid_tiled_kernel");
     }
     auto l = [&]<int M, int I>() {
         hipLaunchKernelGGL((eval_grid_kernel<M, I>), gr, dim3(256), 0, hs, im, g, nr, nu, ld, x0,
@@ -1831,6 +1910,30 @@ int bpmi_eval_grid_multi(const bpmi_eval_job* jobs, const int* blocks, int nbloc
     };
     dispatch2(mode, interp, l);
     return check_launch("eval_grid_multi_kernel");
+}
+
+int bpmi_eval_tiles(int nx, int ny, int* tiles) {
+    if (!tiles || nx <= 0 || ny <= 0) return set_error(BPMI_ERR_INVALID, "bad grid shape");
+    const long long t = (long long)((nx + ET_X - 1) / ET_X) * ((ny + ET_Y - 1) / ET_Y);
+    if (t > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "grid too large");
+    *tiles = (int)t;
+    return BPMI_OK;
+}
+
+int bpmi_eval_grid_multi_tiled(const bpmi_eval_job* jobs, const int* blocks, int nblocks,
+                               int interp, void* stream) {
+    int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
+    if (st) return st;
+    if (nblocks <= 0) return BPMI_OK;
+    if (!jobs || !blocks) return set_error(BPMI_ERR_INVALID, "null table");
+    hipStream_t hs = (hipStream_t)stream;
+    if (interp == BPMI_INTERP_KAISER8)
+        hipLaunchKernelGGL(eval_grid_multi_tiled_kernel<INTERP_KAISER8>, dim3(nblocks),
+                           dim3(MT_THREADS), 0, hs, jobs, blocks);
+    else
+        hipLaunchKernelGGL(eval_grid_multi_tiled_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
+                           dim3(MT_THREADS), 0, hs, jobs, blocks);
+    return check_launch("eval_grid_multi_tiled_kernel");
 }
 
 int bpmi_target_contributions(const void* data, int samples, const double* pos, int pulses,

@@ -2,7 +2,8 @@
 
 All leaves of all patches run in one launch (per-node input table), the
 merges of every patch run one launch per subtree height, and every patch's
-Cartesian output is evaluated in one launch; the whole batch is captured
+Cartesian output is evaluated in one launch (split mode: LDS-tiled, 64 x 4
+pixel tiles); the whole batch is captured
 in one HIP graph, so a batch is one graph launch with no host round trips.
 The split-mode leaf tile width is chosen by the autotuner.  With
 ``cull=True`` only the node tiles that each patch's Cartesian output depends
@@ -167,7 +168,13 @@ class MultiPatchProgram:
             jobs[j]["k"] = self.k
             jobs[j]["nr"], jobs[j]["nu"], jobs[j]["ld"] = plan.nr, plan.nu, plan.root.nu
             jobs[j]["nx"], jobs[j]["ny"] = g.nx, g.ny
-            eval_blocks += [(j, s) for s in range(0, g.nx * g.ny, BLOCK)]
+            if self.mode == "split":
+                import ctypes
+                t = ctypes.c_int()
+                _lib.call("bpmi_eval_tiles", g.nx, g.ny, ctypes.byref(t))
+                eval_blocks += [(j, i) for i in range(t.value)]
+            else:
+                eval_blocks += [(j, s) for s in range(0, g.nx * g.ny, BLOCK)]
         self.jobs_host = jobs
         self.job_roots = [self.node_base[j] + pp.plan.root.index
                           for j, pp in enumerate(self.patches)]
@@ -272,8 +279,13 @@ class MultiPatchProgram:
             enqueue_merges(self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
                            self.arena.ptr, table.ptr, table.shape[0], self.k, self.mode,
                            self.merge_interp, s)
-        _lib.call("bpmi_eval_grid_multi", _vp(self.t_jobs.ptr), _vp(self.t_eval_blocks.ptr),
-                  self.n_eval_blocks, mode, _lib.interp_code(self.merge_interp), s)
+        if self.mode == "split":
+            _lib.call("bpmi_eval_grid_multi_tiled", _vp(self.t_jobs.ptr),
+                      _vp(self.t_eval_blocks.ptr), self.n_eval_blocks,
+                      _lib.interp_code(self.merge_interp), s)
+        else:
+            _lib.call("bpmi_eval_grid_multi", _vp(self.t_jobs.ptr), _vp(self.t_eval_blocks.ptr),
+                      self.n_eval_blocks, mode, _lib.interp_code(self.merge_interp), s)
         return self.outputs
 
     def _launch_leaves(self, mode, s):

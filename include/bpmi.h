@@ -287,6 +287,13 @@ typedef struct bpmi_eval_job {
 /* blocks = (job, first_pixel) pairs of 256 output pixels, device arrays. */
 int bpmi_eval_grid_multi(const bpmi_eval_job* jobs, const int* blocks, int nblocks, int mode,
                          int interp, void* stream);
+/* Split mode with LDS-staged stencil windows: blocks = (job, tile) pairs,
+ * bpmi_eval_tiles() tiles of 64 x-columns by 4 y-rows per job grid, row-major.
+ * Bitwise equal to bpmi_eval_grid_multi in BPMI_MODE_SPLIT (bpmi_eval_grid in
+ * split mode runs the same tiled kernel). */
+int bpmi_eval_grid_multi_tiled(const bpmi_eval_job* jobs, const int* blocks, int nblocks,
+                               int interp, void* stream);
+int bpmi_eval_tiles(int nx, int ny, int* tiles);
 
 /* ---- persistent FFBP: one launch for a whole schedule ------------------- */
 /* No single counterpart in the reference: it replaces the per-level launch
