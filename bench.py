@@ -497,7 +497,7 @@ def run_multipatch(args):
     t = time.perf_counter()
     scenes = make_patches(patch_mix(count=64))
     log(f"64 patches generated in {time.perf_counter() - t:.1f} s")
-    cull = args.cull != "none" and args.mode == "split" and not args.persistent
+    cull = args.cull != "none" and args.mode == "split"
     prog = MultiPatchProgram(scenes, mode=args.mode, merge_interp="kaiser8",
                              persistent=args.persistent, cull=cull)
     best, times = prog.autotune()
@@ -651,8 +651,6 @@ def cpu_baseline_direct(scene, name):
 
 def main():
     args = parse()
-    if args.persistent:
-        args.cull = "none"  # the persistent schedule computes every tile
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding of this path is a design note, not built)",

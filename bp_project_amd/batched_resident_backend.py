@@ -100,8 +100,8 @@ class BatchedResidentProgram(ResidentProgram):
             raise ValueError("the persistent schedule runs in split precision only")
         if persistent not in (False, True, "merges"):
             raise ValueError("persistent must be False, True or 'merges'")
-        if cull is not None and (mode != "split" or persistent):
-            raise ValueError("culling applies to the split-mode level launches only")
+        if cull is not None and (mode != "split" or persistent == "merges"):
+            raise ValueError("culling applies to split-mode schedules with leaves")
         self._cull_request = cull
         self.cull = None
         self.computed_pixels = None  # {node: pixels computed} when culled
@@ -141,6 +141,7 @@ class BatchedResidentProgram(ResidentProgram):
         d = self.device
         mode = _lib.mode_code(self.mode)
         tiles = computed = None
+        self._cull_tiles = None
         if self.mode == "split":
             tc = self._leaf_tc_request
             if tc is None:
@@ -148,6 +149,7 @@ class BatchedResidentProgram(ResidentProgram):
             tc = self.leaf_tile_cols_log2 = int(tc)
             if self._cull_request is not None:
                 tiles, computed = self._cull(flat, arena, tc)
+                self._cull_tiles = tiles
         for height, nodes in enumerate(flat.levels):
             if height == 0:
                 if self.mode == "split":
@@ -262,7 +264,7 @@ class BatchedResidentProgram(ResidentProgram):
             flat.node_int, self._node_table.ptr, self._geometry_table.ptr, self._offset_table.ptr,
             self.arena.ptr, self.k, self.leaf_tile_cols_log2, sources=sources.ptr,
             leaf_interp=self.backend.interp, merge_interp=self.backend.merge_interp,
-            leaves=leaves)
+            leaves=leaves, tiles=self._cull_tiles)
         pf = self.persistent
         count = pf.nitems * MERGE_BLOCK
         self._enqueue("bpmi_ffbp_persistent", count, (), name=f"ffbp_persistent_{self.mode}",

@@ -84,8 +84,8 @@ class MultiPatchProgram:
                  tile_cols_log2=6, persistent=False, cull=False):
         if not scenes:
             raise ValueError("no patches")
-        if cull and (mode != "split" or persistent):
-            raise ValueError("culling applies to the split-mode level launches only")
+        if cull and (mode != "split" or persistent == "merges"):
+            raise ValueError("culling applies to split-mode schedules with leaves")
         ks = {round(s.k, 9) for s in scenes}
         if len(ks) != 1:
             raise ValueError("all patches of a batch must share the carrier wavenumber")
@@ -204,6 +204,15 @@ class MultiPatchProgram:
         synchronize()
         return masks
 
+    def _selected_tiles(self):
+        """{node: tile indices} of the culled schedule (None without culling)."""
+        if self._masks is None:
+            return None
+        heights = self.node_int[:, NI_HEIGHT]
+        return {n: (split_tiles_of(m, int(self.node_int[n, NI_NR]), int(self.node_int[n, NI_NU]),
+                                   self.tile_cols_log2) if heights[n] == 0 else merge_tiles_of(m))
+                for n, m in enumerate(self._masks)}
+
     def set_tile_width(self, tile_cols_log2):
         """(Re)build the leaf block table for a split-mode tile width."""
         self.tile_cols_log2 = int(tile_cols_log2)
@@ -250,7 +259,7 @@ class MultiPatchProgram:
             self.arena.ptr, self.k, self.tile_cols_log2, sources=self.t_sources.ptr,
             jobs=self.jobs_host, job_roots=self.job_roots, leaf_interp=self.leaf_interp,
             merge_interp=self.merge_interp, priority=root_height,
-            leaves=self.persistent is True)
+            leaves=self.persistent is True, tiles=self._selected_tiles())
 
     def completion_ms(self):
         """Persistent schedule, after a completed launch: each patch's

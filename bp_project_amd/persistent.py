@@ -45,13 +45,15 @@ def _merge_tiles(nr, nu):
     return t.value
 
 
-def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True):
+def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True, tiles=None):
     """Work list of a forest of FFBP trees.
 
     node_int: int32 [nodes, 8] node table (a, b, nr, nu, side, left, right,
     height); jobs: sequence of (root node, output pixels) per evaluation job;
     leaves=False leaves the leaf tiles out (they are computed before the
-    launch; their completion targets are then 0).
+    launch; their completion targets are then 0); tiles: optional {node:
+    tile indices} of a culled schedule (culling.tile_selection), every node's
+    completion target then being its number of listed tiles.
     Returns (items int32 [n, 4], node_tiles int32 [nodes], job_blocks int32
     [jobs]).  Every item comes after the items it depends on.
     """
@@ -60,12 +62,18 @@ def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True):
     heights = node_int[:, NI_HEIGHT].astype(np.int64)
     prio = np.arange(nodes) if priority is None else np.asarray(priority)
     node_tiles = np.zeros(nodes, np.int32)
+    tile_list = {}
     for n in range(nodes):
         nr, nu = int(node_int[n, NI_NR]), int(node_int[n, NI_NU])
-        if heights[n] == 0:
-            node_tiles[n] = _split_tiles(nr, nu, tile_cols_log2) if leaves else 0
+        if tiles is not None:
+            tile_list[n] = [int(t) for t in tiles[n]]
+        elif heights[n] == 0:
+            tile_list[n] = list(range(_split_tiles(nr, nu, tile_cols_log2)))
         else:
-            node_tiles[n] = _merge_tiles(nr, nu)
+            tile_list[n] = list(range(_merge_tiles(nr, nu)))
+        if heights[n] == 0 and not leaves:
+            tile_list[n] = []
+        node_tiles[n] = len(tile_list[n])
     job_blocks = np.array([-(-int(px) // EVAL_BLOCK) for _, px in jobs], np.int32)
     job_height = [int(heights[root]) for root, _ in jobs]
     items = []
@@ -73,7 +81,7 @@ def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True):
         level = sorted(np.flatnonzero(heights == h), key=lambda n: (prio[n], n))
         kind = WORK_LEAF if h == 0 else WORK_MERGE
         for n in level:
-            items.extend((kind, int(n), t, 0) for t in range(int(node_tiles[n])))
+            items.extend((kind, int(n), t, 0) for t in tile_list[int(n)])
         for j, (root, _) in enumerate(jobs):
             if job_height[j] == h:
                 items.extend((WORK_EVAL, j, b * EVAL_BLOCK, 0) for b in range(int(job_blocks[j])))
@@ -116,7 +124,8 @@ class PersistentFFBP:
     def __init__(self, node_int_host, node_int, node_geo, offsets, arena, k, tile_cols_log2, *,
                  sources=None, data=None, pos=None, weight=None, phase=None, samples=0,
                  range_start=0.0, data_dr=1.0, jobs=None, job_roots=(), leaf_interp="lagrange4",
-                 merge_interp="kaiser8", priority=None, max_wait_s=10.0, grid=0, leaves=True):
+                 merge_interp="kaiser8", priority=None, max_wait_s=10.0, grid=0, leaves=True,
+                 tiles=None):
         if not leaves:
             raise NotImplementedError(
                 "a persistent work list without leaves is switched off: its one GPU run did not "
@@ -128,7 +137,7 @@ class PersistentFFBP:
         job_px = [] if jobs is None else [int(j["nx"]) * int(j["ny"]) for j in jobs]
         items, node_tiles, job_blocks = work_items(node_int_host, tile_cols_log2,
                                                    list(zip(job_roots, job_px)), priority,
-                                                   leaves=self.leaves)
+                                                   leaves=self.leaves, tiles=tiles)
         if not check_order(items, node_int_host, job_roots, leaves=self.leaves):
             raise RuntimeError("persistent work list is not topologically ordered")
         self.items_host = items

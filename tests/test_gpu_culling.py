@@ -106,4 +106,29 @@ def test_cull_rejects_other_modes():
         BatchedResidentProgram(backend, plan, dev, "fp64", s.r_start, s.data_dr, s.k, cull="crop")
     with pytest.raises(ValueError):
         BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
-                               persistent=True, cull="crop")
+                               persistent="merges", cull="crop")
+
+
+def test_culled_persistent_schedule_equals_culled_level_launches():
+    """The persistent work list takes the culled tiles: fewer items, the same
+    root crop bit for bit, over repeated launches."""
+    from bp_project_amd.batched_resident_backend import BatchedResidentProgram
+    from bp_project_amd.device import synchronize
+    s, plan, backend, dev = _setup("kaiser8")
+    level = _program(s, plan, backend, dev, "crop")
+    full = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
+                                  leaf_tile_cols_log2=5, persistent=True)
+    pers = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
+                                  leaf_tile_cols_log2=5, persistent=True, cull="crop")
+    print(f"work items: culled {pers.persistent.nitems}, full {full.persistent.nitems}")
+    assert pers.persistent.nitems < full.persistent.nitems
+    level.execute()
+    synchronize()
+    ref = level.output.get()
+    for rep in range(2):
+        pers.execute()
+        synchronize()
+        assert not pers.persistent.aborted()
+        assert np.array_equal(pers.output.get(), ref), f"launch {rep}"
+    for p in (level, full, pers):
+        p.close()

@@ -75,6 +75,13 @@ def test_culled_multipatch_equals_unculled(scenes):
                    for a, b in zip(full.outputs, culled.outputs))
     with pytest.raises(ValueError):
         MultiPatchProgram(scenes, mode="mixed", cull=True)
+    # the persistent schedule over the same culled tiles
+    pers = MultiPatchProgram(scenes, mode="split", merge_interp="kaiser8", tile_cols_log2=6,
+                             persistent=True, cull=True)
+    pers.execute()
+    synchronize()
+    assert all(np.array_equal(a.get(), b.get()) for a, b in zip(full.outputs, pers.outputs))
+    pers.close()
     full.close()
     culled.close()
 
