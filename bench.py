@@ -65,7 +65,7 @@ def log(*a):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--steps", type=int, default=15)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="c2_ffbp_full",
                    choices=["c2_ffbp_full", "c1_direct_small", "c3_direct_precision",
@@ -109,6 +109,24 @@ class StageTimer:
         for (_, a), (name, b) in zip(self.marks[:-1], self.marks[1:]):
             out.setdefault(name, []).append(elapsed_ms(a, b))
         return out
+
+    def step_durations(self, steps):
+        """Per-step times: the marks after "start" split into ``steps`` equal runs."""
+        from bp_project_amd.device import elapsed_ms
+        seq = [elapsed_ms(a, b) for (_, a), (_, b) in zip(self.marks[:-1], self.marks[1:])]
+        n = len(seq) // steps
+        return [float(sum(seq[i * n:(i + 1) * n])) for i in range(steps)]
+
+
+def step_timing(step_ms):
+    """Median and interquartile range of per-step HIP-event times (consecutive
+    warmed steps on the launch stream), the reference's reporting protocol
+    (resident_experiment.py:87-119)."""
+    v = np.asarray(step_ms, np.float64)
+    q1, med, q3 = np.percentile(v, [25, 50, 75])
+    return {"source": "HIP events around each timed step", "n": int(v.size),
+            "median_ms": round(float(med), 4), "iqr_ms": round(float(q3 - q1), 4),
+            "min_ms": round(float(v.min()), 4), "max_ms": round(float(v.max()), 4)}
 
 
 def ffbp_workload(args, scene):
@@ -187,6 +205,7 @@ def run_ffbp(args, scene):
     ms_total = elapsed_ms(start, end)
     ms_step = ms_total / args.steps
     stages = timer.durations()
+    step_ms = timer.step_durations(args.steps)
     # per-stage algorithmic work
     flat = prog.flat
     nodes_px = flat.node_int[:, 2].astype(np.int64) * flat.node_int[:, 3]
@@ -216,6 +235,7 @@ def run_ffbp(args, scene):
         "ms_per_step": ms_step,
         "value": g.nx * g.ny * scene.pulses / (ms_step * 1e-3) / 1e9,
         "stages": per_stage,
+        "timing": step_timing(step_ms),
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
@@ -694,7 +714,7 @@ def main():
     }
     if "quality" in res:
         line["quality"] = res["quality"]
-    for extra in ("autofocus", "multipatch", "cull"):
+    for extra in ("timing", "autofocus", "multipatch", "cull"):
         if extra in res:
             line[extra] = res[extra]
     if "precision_study" in res:

@@ -1,4 +1,4 @@
-"""Shared fixtures.  GPU tests are marked ``gpu`` and run only on a GPU box
+"""Shared fixtures.  GPU tests are marked ``gpu`` and need a visible MI355X
 (``pytest -m gpu``); everything else runs on the CPU."""
 import json
 import sys
