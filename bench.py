@@ -26,6 +26,7 @@ The JSON line also carries:
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -365,11 +366,16 @@ def run_direct(args, scene, name):
     d = DeviceArray.from_host(scene.data)
     p = DeviceArray.from_host(scene.pos)
     out = DeviceArray.empty(g.shape)
+    # small grids: pulse-split launches through a workspace of partial images
+    ws_bytes = ctypes.c_size_t()
+    _lib.call("bpmi_bp_grid_workspace_bytes", g.nx, g.ny, scene.pulses, ctypes.byref(ws_bytes))
+    ws = DeviceArray.empty((ws_bytes.value // 8,)) if ws_bytes.value else None
 
     def step(mode=args.mode, dst=out):
-        _lib.call("bpmi_bp_grid", vp(d.ptr), scene.bins, vp(p.ptr), None, None, 0, scene.pulses,
-                  g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, 0.0, scene.r_start, scene.data_dr, scene.k,
-                  _lib.mode_code(mode), _lib.interp_code(interp), 0, vp(dst.ptr), None)
+        _lib.call("bpmi_bp_grid_ws", vp(d.ptr), scene.bins, vp(p.ptr), None, None, 0,
+                  scene.pulses, g.x0, g.dx, g.nx, g.y0, g.dy, g.ny, 0.0, scene.r_start,
+                  scene.data_dr, scene.k, _lib.mode_code(mode), _lib.interp_code(interp), 0,
+                  vp(dst.ptr), vp(ws.ptr) if ws else None, ws_bytes.value, None)
     for _ in range(args.warmup):
         step()
     synchronize()
@@ -388,11 +394,13 @@ def run_direct(args, scene, name):
                            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                            "kernel": "bp_split_kernel", "ops_per_unit": ops,
-                           "unit_of_work": "pixel*pulse", "launches_per_step": 1,
+                           "unit_of_work": "pixel*pulse",
+                           "launches_per_step": 2 if ws is not None else 1,
                            "note": "FP32 vector-ALU bound (no MFMA)"},
               "config": {"workload": name, "pulses": scene.pulses, "bins": scene.bins,
                          "output": [g.ny, g.nx], "pixel_m": g.dx, "interp": interp,
-                         "mode": args.mode}}
+                         "mode": args.mode,
+                         "pulse_split_partials": ws_bytes.value // (g.nx * g.ny * 8)}}
     if not args.no_quality or args.precision_study:
         from bp_project_amd.metrics import pslr_islr_deltas
         img = out.get()

@@ -88,7 +88,7 @@ static int choose_tc_log2(double col_step_m, double row_step_m, double col_look,
 
 template <int SRC, bool ORIGIN_REF>
 static int launch_split(int interp, long long tiles, int tc_log2, hipStream_t st, const BpArgs& a,
-                        const SrcArgs& s, LeafTables lt = LeafTables{}) {
+                        const SrcArgs& s, LeafTables lt = LeafTables{}, int chunks = 1) {
     if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
         return set_error(BPMI_ERR_UNSUPPORTED, "split mode supports lagrange4 and linear interpolation");
     // Grids are bounded by the launch limit; split very large images.
@@ -96,11 +96,13 @@ static int launch_split(int interp, long long tiles, int tc_log2, hipStream_t st
     for (long long base = 0; base < tiles; base += max_blocks) {
         const unsigned nb = (unsigned)(tiles - base < max_blocks ? tiles - base : max_blocks);
         if (interp == BPMI_INTERP_LAGRANGE4)
-            hipLaunchKernelGGL((bp_split_kernel<INTERP_LAGRANGE4, SRC, ORIGIN_REF>), dim3(nb),
-                               dim3(SP_THREADS), 0, st, a, s, tc_log2, (int)base, lt);
+            hipLaunchKernelGGL((bp_split_kernel<INTERP_LAGRANGE4, SRC, ORIGIN_REF>),
+                               dim3(nb, chunks), dim3(SP_THREADS), 0, st, a, s, tc_log2,
+                               (int)base, lt);
         else
-            hipLaunchKernelGGL((bp_split_kernel<INTERP_LINEAR, SRC, ORIGIN_REF>), dim3(nb),
-                               dim3(SP_THREADS), 0, st, a, s, tc_log2, (int)base, lt);
+            hipLaunchKernelGGL((bp_split_kernel<INTERP_LINEAR, SRC, ORIGIN_REF>),
+                               dim3(nb, chunks), dim3(SP_THREADS), 0, st, a, s, tc_log2,
+                               (int)base, lt);
     }
     return check_launch("bp_split_kernel");
 }
@@ -108,6 +110,36 @@ static int launch_split(int interp, long long tiles, int tc_log2, hipStream_t st
 static long long tiles_for(int rows, int cols, int tc_log2) {
     const int tc = 1 << tc_log2, tr = SP_TILE >> tc_log2;
     return (long long)((cols + tc - 1) / tc) * (long long)((rows + tr - 1) / tr);
+}
+
+// Pulse-split direct BP of small grids.  A grid of few 1024-pixel tiles
+// leaves most of the 256 CUs idle (64 tiles at 256^2); splitting the pulses
+// into chunks, each tile and chunk its own workgroup writing a partial image,
+// fills the chip, and one pass sums the partials in chunk order (the result
+// does not depend on scheduling).  Chunks keep at least SPLIT_MIN_PULSES
+// pulses so the per-tile set-up stays amortised.
+constexpr long long SPLIT_TARGET_BLOCKS = 2048;
+constexpr int SPLIT_MIN_PULSES = 64;
+
+static int pulse_chunks(long long tiles, int pulses) {
+    if (tiles * 2 >= SPLIT_TARGET_BLOCKS) return 1;
+    long long c = (SPLIT_TARGET_BLOCKS + tiles - 1) / tiles;
+    c = std::min(c, (long long)(pulses / SPLIT_MIN_PULSES));
+    return c < 2 ? 1 : (int)c;
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const float2* __restrict__ part,
+                                                           int parts, long long n, float2* out,
+                                                           int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 v = accumulate ? out[i] : make_float2(0.f, 0.f);
+    for (int c = 0; c < parts; ++c) {
+        const float2 p = part[(long long)c * n + i];
+        v.x += p.x;
+        v.y += p.y;
+    }
+    out[i] = v;
 }
 
 }  // namespace bpmi
@@ -164,11 +196,39 @@ extern "C" int bpmi_bp_polar_ex(const void* data, int samples, const double* pos
     return check_launch("bp_pixel_kernel(polar)");
 }
 
+extern "C" int bpmi_bp_grid_ws(const void* data, int samples, const double* pos,
+                               const double* weight, const double* phase, int pulse_begin,
+                               int pulse_end, double x0, double dx, int nx, double y0, double dy,
+                               int ny, double z0, double range_start, double data_dr, double k,
+                               int mode, int interp, int flags, void* out, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
 extern "C" int bpmi_bp_grid(const void* data, int samples, const double* pos, const double* weight,
                             const double* phase, int pulse_begin, int pulse_end, double x0,
                             double dx, int nx, double y0, double dy, int ny, double z0,
                             double range_start, double data_dr, double k, int mode, int interp,
                             int flags, void* out, void* stream) {
+    return bpmi_bp_grid_ws(data, samples, pos, weight, phase, pulse_begin, pulse_end, x0, dx, nx,
+                           y0, dy, ny, z0, range_start, data_dr, k, mode, interp, flags, out,
+                           nullptr, 0, stream);
+}
+
+extern "C" int bpmi_bp_grid_workspace_bytes(int nx, int ny, int pulses, size_t* bytes) {
+    if (!bytes) return set_error(BPMI_ERR_INVALID, "bytes must be non-null");
+    if (nx <= 0 || ny <= 0 || pulses < 0) return set_error(BPMI_ERR_INVALID, "invalid grid or pulse count");
+    // the tile shape depends on the spacings: the largest chunk count over the shapes
+    int chunks = 1;
+    for (int l = 3; l <= 6; ++l) chunks = std::max(chunks, pulse_chunks(tiles_for(ny, nx, l), pulses));
+    *bytes = chunks > 1 ? (size_t)chunks * (size_t)nx * (size_t)ny * sizeof(float2) : 0;
+    return BPMI_OK;
+}
+
+extern "C" int bpmi_bp_grid_ws(const void* data, int samples, const double* pos,
+                               const double* weight, const double* phase, int pulse_begin,
+                               int pulse_end, double x0, double dx, int nx, double y0, double dy,
+                               int ny, double z0, double range_start, double data_dr, double k,
+                               int mode, int interp, int flags, void* out, void* workspace,
+                               size_t workspace_bytes, void* stream) {
     int st = validate_common(data, samples, pos, pulse_begin, pulse_end, mode, interp, flags, out);
     if (st) return st;
     if (nx <= 0 || ny <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
@@ -183,7 +243,30 @@ extern "C" int bpmi_bp_grid(const void* data, int samples, const double* pos, co
         }
         // cols = x (cross-track, along the look direction), rows = y
         const int tcl = choose_tc_log2(fabs(dx), fabs(dy), 1.0, 0.3);
-        return launch_split<SRC_GRID, false>(interp, tiles_for(ny, nx, tcl), tcl, hs, a, s);
+        const long long tiles = tiles_for(ny, nx, tcl);
+        const long long npix = (long long)nx * ny;
+        const int pulses = pulse_end - pulse_begin;
+        int chunks = pulse_chunks(tiles, pulses);
+        if (workspace) {
+            const size_t fit = workspace_bytes / ((size_t)npix * sizeof(float2));
+            if ((size_t)chunks > fit) chunks = (int)fit;
+        } else {
+            chunks = 1;
+        }
+        if (chunks < 2 || (flags & BPMI_OUT_C128))
+            return launch_split<SRC_GRID, false>(interp, tiles, tcl, hs, a, s);
+        // equal chunks, none empty
+        const int chunk = (pulses + chunks - 1) / chunks;
+        chunks = (pulses + chunk - 1) / chunk;
+        BpArgs ap = a;
+        ap.chunk = chunk;
+        ap.partial = workspace;
+        st = launch_split<SRC_GRID, false>(interp, tiles, tcl, hs, ap, s, LeafTables{}, chunks);
+        if (st) return st;
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(grid_1d(npix, 256)), dim3(256), 0, hs,
+                           (const float2*)workspace, chunks, npix, (float2*)out,
+                           (flags & BPMI_ACCUMULATE) ? 1 : 0);
+        return check_launch("sum_partials_kernel");
     }
     launch_pixel<SRC_GRID>(mode, interp, (long long)nx * ny, hs, a, s);
     return check_launch("bp_pixel_kernel(grid)");

@@ -32,6 +32,11 @@ struct BpArgs {
     double range_start, data_dr, k;
     int flags;
     void* out;
+    // Pulse-split launch (split kernel): chunk > 0 gives workgroup row
+    // blockIdx.y the pulses [pulse_begin + y chunk, + chunk) and its own
+    // partial image at partial + y * (pixels of the image).
+    int chunk = 0;
+    void* partial = nullptr;
 };
 
 __device__ __forceinline__ long long src_count(const SrcArgs& s, int SRC) {
@@ -786,6 +791,13 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
         // batched leaves: (node, tile) from the block table
         leaf_args(lt, lt.blocks[2 * blockIdx.x], &a, &s);
         tile = lt.blocks[2 * blockIdx.x + 1];
+    }
+    if (a.chunk > 0) {
+        const int pb = a.pulse_begin + (int)blockIdx.y * a.chunk;
+        a.pulse_begin = pb;
+        a.pulse_end = min(a.pulse_end, pb + a.chunk);
+        a.out = (float2*)a.partial + (long long)blockIdx.y * src_count(s, SRC);
+        a.flags &= ~BPMI_ACCUMULATE;
     }
     bp_split_tile<INTERP, SRC, ORIGIN_REF>(a, s, tc_log2, tile, lds);
 }

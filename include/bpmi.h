@@ -120,6 +120,26 @@ int bpmi_bp_grid(const void* data, int samples, const double* pos, const double*
                  double data_dr, double k, int mode, int interp, int flags, void* out,
                  void* stream);
 
+/* bpmi_bp_grid with a caller workspace (device, workspace_bytes long).  In
+ * split mode a grid of few tiles (fewer than ~1024 tiles of 1024 pixels)
+ * has its pulse range split into chunks, each written by its own
+ * workgroups to a partial image in the workspace, and the partials are
+ * summed in chunk order into out (added to it with BPMI_ACCUMULATE): the
+ * same image as bpmi_bp_grid up to FP32 summation order, reproducible from
+ * call to call.  The chunk count is capped by what the workspace holds;
+ * without a workspace, or when the grid fills the GPU, this is
+ * bpmi_bp_grid.  Same reference function as bpmi_bp_grid
+ * (backproject_points, fast_bp/ffbp.py:43-55). */
+int bpmi_bp_grid_ws(const void* data, int samples, const double* pos, const double* weight,
+                    const double* phase, int pulse_begin, int pulse_end, double x0, double dx,
+                    int nx, double y0, double dy, int ny, double z0, double range_start,
+                    double data_dr, double k, int mode, int interp, int flags, void* out,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace bytes with which bpmi_bp_grid_ws uses its full chunk count for
+ * an nx x ny grid and that many pulses (0: the grid needs none). */
+int bpmi_bp_grid_workspace_bytes(int nx, int ny, int pulses, size_t* bytes);
+
 /* Arbitrary points xyz f64 [npoints, 3] (device).  Replaces
  * backproject_points (fast_bp/ffbp.py:43-55). */
 int bpmi_bp_points(const void* data, int samples, const double* pos, const double* weight,

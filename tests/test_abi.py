@@ -47,6 +47,23 @@ def test_host_side_validation_and_error_mapping():
         _lib.mode_code("fp16")
 
 
+def test_pulse_split_workspace_sizes():
+    """Pulse-split direct BP: small grids get a workspace of whole partial
+    images (chunks of at least 64 pulses, ~2048 workgroups), grids that
+    fill the GPU get none."""
+    n = ctypes.c_size_t()
+    _lib.call("bpmi_bp_grid_workspace_bytes", 256, 256, 1024, ctypes.byref(n))
+    assert n.value == 16 * 256 * 256 * 8     # 64 tiles -> 16 chunks of 64 pulses
+    _lib.call("bpmi_bp_grid_workspace_bytes", 256, 256, 100, ctypes.byref(n))
+    assert n.value == 0                       # too few pulses to split
+    _lib.call("bpmi_bp_grid_workspace_bytes", 4096, 4096, 16384, ctypes.byref(n))
+    assert n.value == 0                       # 16384 tiles fill the GPU
+    _lib.call("bpmi_bp_grid_workspace_bytes", 100, 7, 4096, ctypes.byref(n))
+    assert n.value % (100 * 7 * 8) == 0 and n.value > 0
+    with pytest.raises(ValueError, match="invalid grid"):
+        _lib.call("bpmi_bp_grid_workspace_bytes", 0, 7, 4096, ctypes.byref(n))
+
+
 def test_program_argument_errors_are_raised_before_device_use():
     from bp_project_amd.resident_backend import ResidentProgram
 
