@@ -623,7 +623,8 @@ def run_range_compress(args):
             "config": {"workload": "range_compress", "pulses": pulses, "samples": samples,
                        "tp_s": tp, "fs_hz": 400e6, "bandwidth_hz": 200e6, "nfft": nfft,
                        "transform_length": transform_length(samples, tp),
-                       "oversample": 1, "fft": "hipFFT (rocFFT) complex64"}}
+                       "oversample": 1,
+                       "fft": "fused LDS kernel, one workgroup per pulse (complex64)"}}
 
 
 def cpu_baseline_range_compress():
