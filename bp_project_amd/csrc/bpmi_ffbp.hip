@@ -685,8 +685,9 @@ __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& 
 // whose pressure otherwise spills scalars into vector lanes in the pixel loop).
 struct MergeLds {
     float2 sh[2 * MT_CAP];
-    int corner[4][4];  // corner j: ir_a, iu_a, ir_b, iu_b
-    double inv[4];     // 1/dr, 1/du of both children
+    double2 row[MT_ROWS];  // per tile row: slant range r, ground range sqrt(r^2 - z^2)
+    int corner[4][4];      // corner j: ir_a, iu_a, ir_b, iu_b
+    double inv[4];         // 1/dr, 1/du of both children
     float kpoly[4 * KPOLY_STRIDE];
 };
 
@@ -756,6 +757,11 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
             lds.inv[2] = fb.inv_dr; lds.inv[3] = fb.inv_du;
         }
         if (t < 4 * KPOLY_STRIDE) lds.kpoly[t] = kp_src[t];
+    } else if (uniform_i32(wave) == 1 && lane < MT_ROWS) {
+        // the row terms of dest_pixel_col, once per tile row
+        const double rr = g[3] + (double)(tr * MT_ROWS + lane) * g[4];
+        const double g2 = rr * rr - g[2] * g[2];
+        lds.row[lane] = make_double2(rr, g2 > 0.0 ? g2 * rsqrt64(g2) : 0.0);
     }
     __syncthreads();
     const FastSrc fa = make_fast_src_inv(img_a, ga, lnr, lnu, lnu, uniform_f64(lds.inv[0]),
@@ -779,8 +785,10 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         for (int j = 0; j < MT_PPT; ++j) {
             const int row = row0 + j * ROW_STEP;
             if (row >= nr) break;
-            double x, y, rr;
-            dest_pixel_col(g, dc, row, &x, &y, &rr);
+            // dest_pixel_col with the row terms from LDS (the same values)
+            const double2 rg = lds.row[t / MT_COLS + j * ROW_STEP];
+            const double rr = rg.x;
+            const double x = g[0] + rg.y * dc.scu, y = g[1] + rg.y * dc.u;
             const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
             const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
             const float2 va = tile_sample<INTERP, COH>(fa, ca, wa, sha, kpoly);
