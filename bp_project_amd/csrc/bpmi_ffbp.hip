@@ -623,7 +623,8 @@ __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, in
     const int r0 = max(rlo - 1 - FIRST, 0), r1 = min(rhi + 1 + (TAPS - 1 - FIRST), nr - 1);
     const int c0 = max(clo - 1 - FIRST, 0), c1 = min(chi + 1 + (TAPS - 1 - FIRST), nu - 1);
     const int h = r1 - r0 + 1, wd = c1 - c0 + 1;
-    w.lds = defined && h > 0 && wd > 0 && h <= CAP && wd <= CAP && h * wd <= CAP;
+    // a window narrower than one stencil holds no stencil: not staged
+    w.lds = defined && h >= TAPS && wd >= TAPS && h <= CAP && wd <= CAP && h * wd <= CAP;
     w.r0 = r0;
     w.c0 = c0;
     w.h = w.lds ? h : 0;
@@ -631,11 +632,16 @@ __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, in
     return w;
 }
 
+// Stencil offsets (dr, du) of c in the window and whether the whole stencil
+// lies in it: one unsigned compare per axis (w.h, w.w >= TAPS when w.lds;
+// c.ok keeps the indices small).
 template <int INTERP>
-__device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w) {
+__device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w, int* dr, int* du) {
     constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
-    return w.lds && c.ir - FIRST >= w.r0 && c.ir - FIRST + TAPS <= w.r0 + w.h &&
-           c.iu - FIRST >= w.c0 && c.iu - FIRST + TAPS <= w.c0 + w.w;
+    *dr = c.ir - (FIRST + w.r0);
+    *du = c.iu - (FIRST + w.c0);
+    return w.lds && (unsigned)*dr <= (unsigned)(w.h - TAPS) &&
+           (unsigned)*du <= (unsigned)(w.w - TAPS);
 }
 
 // Copy the window into LDS: wave v takes rows v, v + 4, .., lane j columns
@@ -669,10 +675,10 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
 template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
                                               const float2* sh, const float* kp) {
-    constexpr int FIRST = FastShape<INTERP>::FIRST;
     if (!c.ok) return make_float2(0.f, 0.f);
-    if (in_window<INTERP>(c, w)) {
-        const float2* base = sh + (c.ir - FIRST - w.r0) * w.w + (c.iu - FIRST - w.c0);
+    int dr, du;
+    if (in_window<INTERP>(c, w, &dr, &du)) {
+        const float2* base = sh + dr * w.w + du;
         const int pitch = w.w;
         return stencil<INTERP>(c, [&](int a) { return base + a * pitch; }, kp);
     }
