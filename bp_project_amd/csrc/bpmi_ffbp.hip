@@ -634,7 +634,7 @@ __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, in
 
 // Stencil offsets (dr, du) of c in the window and whether the whole stencil
 // lies in it: one unsigned compare per axis (w.h, w.w >= TAPS when w.lds;
-// c.ok keeps the indices small).
+// an undefined origin, INT_MAX, lands far outside).
 template <int INTERP>
 __device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w, int* dr, int* du) {
     constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
@@ -675,13 +675,18 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
 template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
                                               const float2* sh, const float* kp) {
-    if (!c.ok) return make_float2(0.f, 0.f);
+    // The window is clipped to the image, so a stencil inside it is valid
+    // (c.ok): only the global-memory path needs the validity test, which is
+    // restated here so that c.ok is not computed on the window path.
     int dr, du;
     if (in_window<INTERP>(c, w, &dr, &du)) {
         const float2* base = sh + dr * w.w + du;
         const int pitch = w.w;
         return stencil<INTERP>(c, [&](int a) { return base + a * pitch; }, kp);
     }
+    const bool ok = c.ir != 0x7fffffff && InterpShape<INTERP>::valid(c.ir, f.nr) &&
+                    InterpShape<INTERP>::valid(c.iu, f.nu);
+    if (!ok) return make_float2(0.f, 0.f);
     return stencil_global<INTERP, COH>(f, c, kp);
 }
 
