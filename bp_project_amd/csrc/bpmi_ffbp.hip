@@ -406,18 +406,17 @@ template <int INTERP>
 __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double y, double r,
                                               double kt) {
     // Branch-free: every quantity is computed, then masked by the validity
-    // conditions (a point on the origin, an index beyond int range, a stencil
-    // leaving the image), which keeps the geometry in straight-line code.
+    // conditions (an index beyond int range, a stencil leaving the image),
+    // which keeps the geometry in straight-line code.  A point on the origin
+    // (s2 = 0) gives 0 * rsqrt(0) = NaN in uu, and NaN fails the range test.
     const double dx = x - f.ox, dy = y - f.oy;
     const double s2 = dx * dx + dy * dy;
-    const bool off_origin = s2 > 0.0;
-    const double s2g = off_origin ? s2 : 1.0;
-    const double uu = dy * rsqrt64(s2g);
-    const double rr2 = s2g + f.oz2;
+    const double uu = dy * rsqrt64(s2);
+    const double rr2 = s2 + f.oz2;
     const double rr = rr2 * rsqrt64(rr2);
     const double qr = (rr - f.r0) * f.inv_dr, qu = (uu - f.u0) * f.inv_du;
     const double fr = floor(qr), fu = floor(qu);
-    const bool defined = off_origin && fabs(fr) < 2147483000.0 && fabs(fu) < 2147483000.0;
+    const bool defined = fabs(fr) < 2147483000.0 && fabs(fu) < 2147483000.0;
     ChildTap c;
     c.ir = defined ? (int)fr : 0x7fffffff;
     c.iu = defined ? (int)fu : 0x7fffffff;
