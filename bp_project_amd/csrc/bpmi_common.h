@@ -52,7 +52,9 @@ __device__ __forceinline__ void sincos_turns(float t, float* s, float* c) {
 // exp(j 2 pi T) for an FP64 phase in turns: exact FP64 reduction, FP32 sincos.
 __device__ __forceinline__ float2 cis_turns64(double turns) {
     float s, c;
-    sincos_turns((float)fract64(turns), &s, &c);
+    // v_fract_f64 equals turns - floor(turns) except that it stays below 1
+    // where the subtraction rounds up to 1; both round to 1.0f here.
+    sincos_turns((float)hw_fract64(turns), &s, &c);
     return make_float2(c, s);
 }
 
