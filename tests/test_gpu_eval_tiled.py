@@ -87,3 +87,26 @@ def test_tiled_evaluation_vs_oracle(oracle, root_image):
     err = np.linalg.norm(tiled - ref) / np.linalg.norm(ref)
     print(f"tiled evaluation vs oracle evaluate_image: {err:.2e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("interp", ["kaiser8", "lagrange4"])
+def test_points_on_the_node_origin_evaluate_to_zero(root_image, interp):
+    """A point straight below or above the root origin has no polar angle
+    (ground distance 0): its value is 0, in the point kernel and in a tiled
+    grid whose middle pixel sits on the origin, and nothing is NaN."""
+    from bp_project_amd.ffbp import evaluate_image
+    s, plan, prog = root_image
+    o = np.asarray(plan.origin, np.float64)
+    img = prog.output.get()
+    pts = np.array([[o[0], o[1], 0.0], [o[0], o[1], o[2]], [o[0] + 1.0, o[1], 0.0]])
+    for mode in ("split", "fp64"):
+        v = evaluate_image(img, plan.origin, plan.r0, plan.dr, plan.u0, plan.du, pts, s.k,
+                           mode=mode, interp=interp)
+        assert np.all(np.isfinite(v)), mode
+        assert v[0] == 0 and v[1] == 0, mode
+    d = 0.5
+    tiled, per_pixel, multi = _three_ways(prog, plan, s.k, (o[0] - 2 * d, d, 5, o[1] - 2 * d, d, 5),
+                                          interp)
+    for out in (tiled, per_pixel, multi):
+        assert np.all(np.isfinite(out))
+        assert out[2, 2] == 0
