@@ -245,66 +245,194 @@ void orc_backproject_points(const float* data, long P, long S, const double* pos
     }
 }
 
-/* ffbp.py:58-76: polar BP demodulated by exp(-j k r); pixels on z = 0. */
+/* ffbp.py:58-76: one pixel (ir, iu) of the polar BP, demodulated by
+ * exp(-j k r); pixels on z = 0. */
+static inline void bp_polar_pixel(const float* data, long P, long S, const double* pos,
+                                  const double* origin, double r0, double dr, double u0,
+                                  double du, long ir, long iu, double r_start, double data_dr,
+                                  double k, const double* weight, const double* phase, int side,
+                                  int interp, double out[2]) {
+    double r = r0 + (double)ir * dr;
+    double ground = sqrt(fmax(0.0, r * r - origin[2] * origin[2]));
+    double u = u0 + (double)iu * du;
+    double x = origin[0] + (double)side * ground * sqrt(fmax(0.0, 1 - u * u));
+    double y = origin[1] + ground * u;
+    double tr = 0.0, ti = 0.0;
+    for (long n = 0; n < P; ++n) {
+        double dx = x - pos[3 * n];
+        double dy = y - pos[3 * n + 1];
+        double d = sqrt(dx * dx + dy * dy + pos[3 * n + 2] * pos[3 * n + 2]);
+        double v[2];
+        orc_sample1(data + 2 * (size_t)n * (size_t)S, S, (d - r_start) / data_dr, interp, v);
+        double a = k * (d - r) + phase[n];
+        mac_rot(&tr, &ti, weight[n], v, a);
+    }
+    out[0] = tr;
+    out[1] = ti;
+}
+
 void orc_bp_polar(const float* data, long P, long S, const double* pos, const double* origin,
                   double r0, double dr, long nr, double u0, double du, long nu,
                   double r_start, double data_dr, double k, const double* weight,
                   const double* phase, int side, int interp, double* out) {
     #pragma omp parallel for schedule(dynamic, 1)
     for (long ir = 0; ir < nr; ++ir) {
-        double r = r0 + (double)ir * dr;
-        double ground = sqrt(fmax(0.0, r * r - origin[2] * origin[2]));
         for (long iu = 0; iu < nu; ++iu) {
-            double u = u0 + (double)iu * du;
-            double x = origin[0] + (double)side * ground * sqrt(fmax(0.0, 1 - u * u));
-            double y = origin[1] + ground * u;
-            double tr = 0.0, ti = 0.0;
-            for (long n = 0; n < P; ++n) {
-                double dx = x - pos[3 * n];
-                double dy = y - pos[3 * n + 1];
-                double d = sqrt(dx * dx + dy * dy + pos[3 * n + 2] * pos[3 * n + 2]);
-                double v[2];
-                orc_sample1(data + 2 * (size_t)n * (size_t)S, S, (d - r_start) / data_dr, interp, v);
-                double a = k * (d - r) + phase[n];
-                mac_rot(&tr, &ti, weight[n], v, a);
-            }
             size_t o = (size_t)ir * (size_t)nu + (size_t)iu;
-            out[2 * o] = tr;
-            out[2 * o + 1] = ti;
+            bp_polar_pixel(data, P, S, pos, origin, r0, dr, u0, du, ir, iu, r_start, data_dr, k,
+                           weight, phase, side, interp, out + 2 * o);
         }
     }
 }
 
-/* ffbp.py:79-95: child polar image -> parent polar grid. */
+/* ffbp.py:79-95: one destination pixel (ir, iu) of a child -> parent regrid. */
+static inline void regrid_pixel(const void* img, int is_c128, long snr, long snu,
+                                const double* src_origin, double sr0, double sdr, double su0,
+                                double sdu, const double* dst_origin, double r0, double dr,
+                                double u0, double du, long ir, long iu, double k, int side,
+                                int interp, double out[2]) {
+    double r = r0 + (double)ir * dr;
+    double gr = sqrt(fmax(0.0, r * r - dst_origin[2] * dst_origin[2]));
+    double u = u0 + (double)iu * du;
+    double x = dst_origin[0] + (double)side * gr * sqrt(fmax(0.0, 1 - u * u));
+    double y = dst_origin[1] + gr * u;
+    double dx = x - src_origin[0];
+    double dy = y - src_origin[1];
+    double g = sqrt(dx * dx + dy * dy);
+    if (g == 0.0) {
+        out[0] = 0.0;
+        out[1] = 0.0;
+        return;
+    }
+    double rr = sqrt(g * g + src_origin[2] * src_origin[2]);
+    double uu = dy / g;
+    double v[2];
+    orc_sample2(img, is_c128, snr, snu, (rr - sr0) / sdr, (uu - su0) / sdu, interp, v);
+    double a = k * (rr - r);
+    double c = cos(a), s = sin(a);
+    out[0] = v[0] * c - v[1] * s;
+    out[1] = v[0] * s + v[1] * c;
+}
+
 void orc_regrid(const void* img, int is_c128, long snr, long snu, const double* src_origin,
                 double sr0, double sdr, double su0, double sdu, const double* dst_origin,
                 double r0, double dr, long nr, double u0, double du, long nu, double k,
                 int side, int interp, double* out) {
     #pragma omp parallel for schedule(dynamic, 4)
     for (long ir = 0; ir < nr; ++ir) {
-        double r = r0 + (double)ir * dr;
-        double gr = sqrt(fmax(0.0, r * r - dst_origin[2] * dst_origin[2]));
         for (long iu = 0; iu < nu; ++iu) {
             size_t o = (size_t)ir * (size_t)nu + (size_t)iu;
-            double u = u0 + (double)iu * du;
-            double x = dst_origin[0] + (double)side * gr * sqrt(fmax(0.0, 1 - u * u));
-            double y = dst_origin[1] + gr * u;
-            double dx = x - src_origin[0];
-            double dy = y - src_origin[1];
-            double g = sqrt(dx * dx + dy * dy);
-            if (g == 0.0) {
-                out[2 * o] = 0.0;
-                out[2 * o + 1] = 0.0;
-                continue;
+            regrid_pixel(img, is_c128, snr, snu, src_origin, sr0, sdr, su0, sdu, dst_origin, r0,
+                         dr, u0, du, ir, iu, k, side, interp, out + 2 * o);
+        }
+    }
+}
+
+/* Block-restricted forms (test infrastructure for sampled full-size checks):
+ * only the pixels of the listed blocks -- (block row, block column) pairs of
+ * a grid of bs x bs-pixel blocks -- are computed, with the same per-pixel
+ * arithmetic as above, and stored into ``out`` as complex64 (out_c64 = 1,
+ * the reference's node storage) or complex128.  accumulate = 1 adds to what
+ * ``out`` holds in that storage type (ffbp.py:165-167, im += regrid(...)).
+ * The other pixels of ``out`` are left untouched. */
+static inline void store_pixel(void* out, int out_c64, int accumulate, size_t o, const double v[2]) {
+    if (out_c64) {
+        float* p = (float*)out + 2 * o;
+        float re = (float)v[0], im = (float)v[1];
+        if (accumulate) {
+            re = p[0] + re;
+            im = p[1] + im;
+        }
+        p[0] = re;
+        p[1] = im;
+    } else {
+        double* p = (double*)out + 2 * o;
+        if (accumulate) {
+            p[0] = p[0] + v[0];
+            p[1] = p[1] + v[1];
+        } else {
+            p[0] = v[0];
+            p[1] = v[1];
+        }
+    }
+}
+
+void orc_bp_polar_blocks(const float* data, long P, long S, const double* pos,
+                         const double* origin, double r0, double dr, long nr, double u0,
+                         double du, long nu, double r_start, double data_dr, double k,
+                         const double* weight, const double* phase, int side, int interp,
+                         const int32_t* blocks, long nblocks, int bs, int out_c64, void* out) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (long b = 0; b < nblocks; ++b) {
+        long r_end = (long)blocks[2 * b] * bs + bs, u_end = (long)blocks[2 * b + 1] * bs + bs;
+        if (r_end > nr) r_end = nr;
+        if (u_end > nu) u_end = nu;
+        for (long ir = (long)blocks[2 * b] * bs; ir < r_end; ++ir)
+            for (long iu = (long)blocks[2 * b + 1] * bs; iu < u_end; ++iu) {
+                double v[2];
+                bp_polar_pixel(data, P, S, pos, origin, r0, dr, u0, du, ir, iu, r_start, data_dr,
+                               k, weight, phase, side, interp, v);
+                store_pixel(out, out_c64, 0, (size_t)ir * (size_t)nu + (size_t)iu, v);
             }
-            double rr = sqrt(g * g + src_origin[2] * src_origin[2]);
-            double uu = dy / g;
-            double v[2];
-            orc_sample2(img, is_c128, snr, snu, (rr - sr0) / sdr, (uu - su0) / sdu, interp, v);
-            double a = k * (rr - r);
-            double c = cos(a), s = sin(a);
-            out[2 * o] = v[0] * c - v[1] * s;
-            out[2 * o + 1] = v[0] * s + v[1] * c;
+    }
+}
+
+void orc_regrid_blocks(const void* img, int is_c128, long snr, long snu, const double* src_origin,
+                       double sr0, double sdr, double su0, double sdu, const double* dst_origin,
+                       double r0, double dr, long nr, double u0, double du, long nu, double k,
+                       int side, int interp, const int32_t* blocks, long nblocks, int bs,
+                       int out_c64, int accumulate, void* out) {
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (long b = 0; b < nblocks; ++b) {
+        long r_end = (long)blocks[2 * b] * bs + bs, u_end = (long)blocks[2 * b + 1] * bs + bs;
+        if (r_end > nr) r_end = nr;
+        if (u_end > nu) u_end = nu;
+        for (long ir = (long)blocks[2 * b] * bs; ir < r_end; ++ir)
+            for (long iu = (long)blocks[2 * b + 1] * bs; iu < u_end; ++iu) {
+                double v[2];
+                regrid_pixel(img, is_c128, snr, snu, src_origin, sr0, sdr, su0, sdu, dst_origin,
+                             r0, dr, u0, du, ir, iu, k, side, interp, v);
+                store_pixel(out, out_c64, accumulate, (size_t)ir * (size_t)nu + (size_t)iu, v);
+            }
+    }
+}
+
+/* Support of a regrid: for every pixel of the listed parent blocks, the
+ * child position regrid_pixel samples (same geometry), its stencil box
+ * (taps from floor(q) + first), widened by ``margin`` cells and clipped to
+ * the child, sets the child blocks (cmask, [ceil(cnr/bs), ceil(cnu/bs)]) it
+ * touches.  Points on the child's origin (ground distance 0) read nothing. */
+void orc_mark_support(const double* pgeo, long pnr, long pnu, const double* cgeo, long cnr,
+                      long cnu, int side, const int32_t* blocks, long nblocks, int bs, int first,
+                      int taps, int margin, uint8_t* cmask) {
+    const long cbu = (cnu + bs - 1) / bs;
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (long b = 0; b < nblocks; ++b) {
+        long r_end = (long)blocks[2 * b] * bs + bs, u_end = (long)blocks[2 * b + 1] * bs + bs;
+        if (r_end > pnr) r_end = pnr;
+        if (u_end > pnu) u_end = pnu;
+        for (long ir = (long)blocks[2 * b] * bs; ir < r_end; ++ir) {
+            double r = pgeo[3] + (double)ir * pgeo[4];
+            double gr = sqrt(fmax(0.0, r * r - pgeo[2] * pgeo[2]));
+            for (long iu = (long)blocks[2 * b + 1] * bs; iu < u_end; ++iu) {
+                double u = pgeo[5] + (double)iu * pgeo[6];
+                double x = pgeo[0] + (double)side * gr * sqrt(fmax(0.0, 1 - u * u));
+                double y = pgeo[1] + gr * u;
+                double dx = x - cgeo[0], dy = y - cgeo[1];
+                double g = sqrt(dx * dx + dy * dy);
+                if (!(g > 0.0)) continue;
+                double rr = sqrt(g * g + cgeo[2] * cgeo[2]);
+                double qr = floor((rr - cgeo[3]) / cgeo[4]), qu = floor((dy / g - cgeo[5]) / cgeo[6]);
+                double lo_r = qr + first - margin, hi_r = qr + first + taps - 1 + margin;
+                double lo_u = qu + first - margin, hi_u = qu + first + taps - 1 + margin;
+                if (hi_r < 0 || lo_r > (double)(cnr - 1) || hi_u < 0 || lo_u > (double)(cnu - 1))
+                    continue;
+                long r0 = lo_r < 0 ? 0 : (long)lo_r, r1 = hi_r > cnr - 1 ? cnr - 1 : (long)hi_r;
+                long u0 = lo_u < 0 ? 0 : (long)lo_u, u1 = hi_u > cnu - 1 ? cnu - 1 : (long)hi_u;
+                for (long br = r0 / bs; br <= r1 / bs; ++br)
+                    for (long bu = u0 / bs; bu <= u1 / bs; ++bu)
+                        __atomic_store_n(&cmask[br * cbu + bu], (uint8_t)1, __ATOMIC_RELAXED);
+            }
         }
     }
 }

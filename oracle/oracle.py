@@ -92,6 +92,11 @@ def _declare(L):
                                _vp, _vp, _i, _i, _vp]
     L.orc_regrid.argtypes = [_vp, _i, _l, _l, _vp, _d, _d, _d, _d, _vp, _d, _d, _l, _d, _d, _l,
                              _d, _i, _i, _vp]
+    L.orc_bp_polar_blocks.argtypes = [_vp, _l, _l, _vp, _vp, _d, _d, _l, _d, _d, _l, _d, _d, _d,
+                                      _vp, _vp, _i, _i, _vp, _l, _i, _i, _vp]
+    L.orc_regrid_blocks.argtypes = [_vp, _i, _l, _l, _vp, _d, _d, _d, _d, _vp, _d, _d, _l, _d, _d,
+                                    _l, _d, _i, _i, _vp, _l, _i, _i, _i, _vp]
+    L.orc_mark_support.argtypes = [_vp, _l, _l, _vp, _l, _l, _i, _vp, _l, _i, _i, _i, _i, _vp]
     L.orc_evaluate_image.argtypes = [_vp, _i, _l, _l, _vp, _d, _d, _d, _d, _vp, _l, _d, _i, _vp]
     L.orc_target_contributions.argtypes = [_vp, _l, _l, _vp, _vp, _l, _d, _d, _d, _i, _vp]
     L.orc_backproject_points_adjoint.argtypes = [_vp, _l, _l, _vp, _vp, _l, _d, _d, _d, _vp, _vp,
@@ -308,6 +313,38 @@ class PolarImage:
         return self.u0 + self.du * np.arange(self.image.shape[1])
 
 
+def _node_grid_rule(pos, extent, side, wavelength, range_spacing, angular_oversample):
+    """The node polar grid of fast_bp/ffbp.py:126-152 as a function of the
+    pulse span [a, b): (origin, r0, nr, u0, du, nu).  The sector boundary is
+    sampled at 129 points per edge, seen from the node's mean position."""
+    root_origin = pos.mean(axis=0)
+    rmin, rmax, umin, umax = extent
+    us = np.linspace(umin, umax, 129)
+    rs = np.linspace(rmin, rmax, 129)
+    rb = np.r_[np.full_like(us, rmin), np.full_like(us, rmax), rs, rs]
+    ub = np.r_[us, us, np.full_like(rs, umin), np.full_like(rs, umax)]
+    g = np.sqrt(rb ** 2 - root_origin[2] ** 2)
+    bx = root_origin[0] + side * g * np.sqrt(1 - ub ** 2)
+    by = root_origin[1] + g * ub
+
+    def grid_of(a, b):
+        origin = pos[a:b].mean(axis=0)
+        dx = bx - origin[0]
+        dy = by - origin[1]
+        gr = np.sqrt(dx * dx + dy * dy)
+        rr = np.sqrt(gr * gr + origin[2] ** 2)
+        uu = dy / gr
+        aperture = np.linalg.norm(np.ptp(pos[a:b, :2], axis=0))
+        du = min(.025, wavelength / (2 * max(aperture, .1) * angular_oversample))
+        u0 = max(-.995, float(uu.min()) - 8 * du)
+        u1 = min(.995, float(uu.max()) + 8 * du)
+        r0 = float(rr.min()) - 8 * range_spacing
+        nr = int(np.ceil((rr.max() - r0) / range_spacing)) + 9
+        nu = int(np.ceil((u1 - u0) / du)) + 1
+        return origin, r0, nr, u0, du, nu
+    return grid_of
+
+
 def form_ffbp(data, pos, r_start, data_dr, fc, extent, side=1, leaf_size=128, range_spacing=.25,
               angular_oversample=2.5, weight=None, phase=None, policy="parity",
               leaf_interp="lagrange4", merge_interp="lagrange4", collect_nodes=None):
@@ -332,31 +369,12 @@ def form_ffbp(data, pos, r_start, data_dr, fc, extent, side=1, leaf_size=128, ra
     weight = np.ascontiguousarray(weight, dtype=np.float64)
     phase = np.ascontiguousarray(phase, dtype=np.float64)
     store = np.complex64 if policy == "parity" else np.complex128
-    root_origin = pos.mean(axis=0)
     rmin, rmax, umin, umax = extent
-    us = np.linspace(umin, umax, 129)
-    rs = np.linspace(rmin, rmax, 129)
-    rb = np.r_[np.full_like(us, rmin), np.full_like(us, rmax), rs, rs]
-    ub = np.r_[us, us, np.full_like(rs, umin), np.full_like(rs, umax)]
-    g = np.sqrt(rb ** 2 - root_origin[2] ** 2)
-    bx = root_origin[0] + side * g * np.sqrt(1 - ub ** 2)
-    by = root_origin[1] + g * ub
+    grid_of = _node_grid_rule(pos, extent, side, wavelength, range_spacing, angular_oversample)
     counts = {"leaves": 0, "merges": 0, "leaf_pixel_pulse_products": 0, "merged_pixels": 0}
 
     def node(a, b):
-        origin = pos[a:b].mean(axis=0)
-        dx = bx - origin[0]
-        dy = by - origin[1]
-        gr = np.sqrt(dx * dx + dy * dy)
-        rr = np.sqrt(gr * gr + origin[2] ** 2)
-        uu = dy / gr
-        aperture = np.linalg.norm(np.ptp(pos[a:b, :2], axis=0))
-        du = min(.025, wavelength / (2 * max(aperture, .1) * angular_oversample))
-        u0 = max(-.995, float(uu.min()) - 8 * du)
-        u1 = min(.995, float(uu.max()) + 8 * du)
-        r0 = float(rr.min()) - 8 * range_spacing
-        nr = int(np.ceil((rr.max() - r0) / range_spacing)) + 9
-        nu = int(np.ceil((u1 - u0) / du)) + 1
+        origin, r0, nr, u0, du, nu = grid_of(a, b)
         if collect_nodes is not None:
             collect_nodes.append((a, b, origin.copy(), r0, nr, u0, du, nu))
         if b - a <= leaf_size:
@@ -389,6 +407,155 @@ def form_ffbp(data, pos, r_start, data_dr, fc, extent, side=1, leaf_size=128, ra
     counts["elapsed_s"] = time.perf_counter() - start
     counts["shape"] = list(result.image.shape)
     return result, counts
+
+
+# Stencil of each interpolator: first tap offset from floor(q), tap count
+# (bp_oracle.c: sample1/sample2).
+STENCIL = {"lagrange4": (-1, 4), "linear": (0, 2), "kaiser8": (-3, 8)}
+
+
+def _mark_stencils(mask, nr, nu, qr, qu, interp, bs, margin=1):
+    """Set the blocks of ``mask`` (bs x bs pixels per entry) that the stencils
+    at fractional positions (qr, qu), widened by ``margin`` cells, touch."""
+    first, taps = STENCIL[interp]
+    ok = np.isfinite(qr) & np.isfinite(qu)
+    fr, fu = np.floor(qr[ok]), np.floor(qu[ok])
+    lo_r, hi_r = fr + first - margin, fr + first + taps - 1 + margin
+    lo_u, hi_u = fu + first - margin, fu + first + taps - 1 + margin
+    keep = (hi_r >= 0) & (lo_r <= nr - 1) & (hi_u >= 0) & (lo_u <= nu - 1)
+    br0 = (np.clip(lo_r[keep], 0, nr - 1) // bs).astype(np.int64)
+    br1 = (np.clip(hi_r[keep], 0, nr - 1) // bs).astype(np.int64)
+    bu0 = (np.clip(lo_u[keep], 0, nu - 1) // bs).astype(np.int64)
+    bu1 = (np.clip(hi_u[keep], 0, nu - 1) // bs).astype(np.int64)
+    span = (taps + 2 * margin + bs - 2) // bs + 1   # blocks a box can cross per axis
+    for a in range(span):
+        for b in range(span):
+            mask[np.minimum(br0 + a, br1), np.minimum(bu0 + b, bu1)] = True
+
+
+def ffbp_at_points(data, pos, r_start, data_dr, fc, extent, xyz, side=1, leaf_size=128,
+                   range_spacing=.25, angular_oversample=2.5, weight=None, phase=None,
+                   policy="fp64", leaf_interp="lagrange4", merge_interp="lagrange4",
+                   eval_interp=None, block=8):
+    """``evaluate_image(form_ffbp(...), xyz)`` computing only what those
+    values depend on.
+
+    Same tree, node grids and per-pixel arithmetic as ``form_ffbp`` (the C
+    block kernels share the pixel bodies of bp_polar/regrid), so every value
+    equals the full computation's bit for bit; the saving is that a node
+    pixel is computed only when it lies in a block of ``block`` x ``block``
+    pixels that some stencil of a required parent pixel (or of an output
+    point, at the root crop) touches, each stencil widened by one cell on
+    every side.  Pixels outside those blocks stay 0 and are never read by a
+    computed pixel.  Returns (values complex128 [M], info dict)."""
+    if policy not in ("parity", "fp64"):
+        raise ValueError("policy must be parity or fp64")
+    eval_interp = merge_interp if eval_interp is None else eval_interp
+    start = time.perf_counter()
+    k = 4 * np.pi * fc / C
+    wavelength = C / abs(fc)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    data = _c64(data)
+    weight = np.ones(len(pos)) if weight is None else weight
+    phase = np.zeros(len(pos)) if phase is None else phase
+    weight = np.ascontiguousarray(weight, dtype=np.float64)
+    phase = np.ascontiguousarray(phase, dtype=np.float64)
+    store = np.complex64 if policy == "parity" else np.complex128
+    rmin, rmax, umin, umax = extent
+    xyz = _f64(xyz).reshape(-1, 3)
+    grid_of = _node_grid_rule(pos, extent, side, wavelength, range_spacing, angular_oversample)
+
+    # the tree in pre-order: (a, b, geometry, nr, nu, children)
+    nodes = []
+
+    def build(a, b):
+        origin, r0, nr, u0, du, nu = grid_of(a, b)
+        i = len(nodes)
+        nodes.append({"a": a, "b": b, "geo": (origin, r0, range_spacing, u0, du), "nr": nr,
+                      "nu": nu, "kids": ()})
+        if b - a > leaf_size:
+            mid = (a + b) // 2
+            nodes[i]["kids"] = (build(a, mid), build(mid, b))
+        return i
+    build(0, len(pos))
+
+    def blocks_of(n):
+        return np.zeros((-(-n["nr"] // block), -(-n["nu"] // block)), bool)
+
+    # root crop (ffbp.py:172-177) and the root blocks the evaluation reads
+    root = nodes[0]
+    origin, r0, dr, u0, du = root["geo"]
+    rr_ax = r0 + dr * np.arange(root["nr"])
+    uu_ax = u0 + du * np.arange(root["nu"])
+    ri = np.flatnonzero((rr_ax >= rmin) & (rr_ax <= rmax))
+    ui = np.flatnonzero((uu_ax >= umin) & (uu_ax <= umax))
+    crop = (int(ri[0]), int(ri[-1]) + 1, int(ui[0]), int(ui[-1]) + 1)
+    cr0, cu0 = r0 + ri[0] * dr, u0 + ui[0] * du
+    dxp = xyz[:, 0] - origin[0]
+    dyp = xyz[:, 1] - origin[1]
+    gp = np.sqrt(dxp * dxp + dyp * dyp)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        gp = np.where(gp == 0.0, np.nan, gp)
+        qr = (np.sqrt(gp * gp + origin[2] ** 2) - cr0) / dr + crop[0]
+        qu = (dyp / gp - cu0) / du + crop[2]
+    root["mask"] = blocks_of(root)
+    _mark_stencils(root["mask"], root["nr"], root["nu"], qr, qu, eval_interp, block)
+    # parents before children: each child's blocks from its parent's
+    first, taps = STENCIL[merge_interp]
+    for n in nodes:
+        blocks = np.ascontiguousarray(np.argwhere(n["mask"]), dtype=np.int32)
+        n["blocks"] = blocks
+        pgeo = np.r_[n["geo"][0], n["geo"][1:]].astype(np.float64)
+        for c in n["kids"]:
+            child = nodes[c]
+            child["mask"] = blocks_of(child)
+            cgeo = np.r_[child["geo"][0], child["geo"][1:]].astype(np.float64)
+            lib().orc_mark_support(_ptr(pgeo), n["nr"], n["nu"], _ptr(cgeo), child["nr"],
+                                   child["nu"], int(side), _ptr(blocks), len(blocks), block,
+                                   first, taps, 1, _ptr(child["mask"]))
+    info = {"node_pixels": 0, "computed_pixels": 0, "nodes": len(nodes)}
+    out_c64 = int(store == np.complex64)
+
+    def compute(i):
+        n = nodes[i]
+        origin, r0, dr, u0, du = n["geo"]
+        blocks = n["blocks"]
+        info["node_pixels"] += n["nr"] * n["nu"]
+        info["computed_pixels"] += int(min(len(blocks) * block * block, n["nr"] * n["nu"]))
+        out = np.zeros((n["nr"], n["nu"]), store)
+        if not n["kids"]:
+            a, b = n["a"], n["b"]
+            d = np.ascontiguousarray(data[a:b])
+            lib().orc_bp_polar_blocks(
+                _ptr(d), b - a, d.shape[1], _ptr(np.ascontiguousarray(pos[a:b])),
+                _ptr(np.ascontiguousarray(origin)), float(r0), float(dr), n["nr"], float(u0),
+                float(du), n["nu"], float(r_start), float(data_dr), float(k),
+                _ptr(np.ascontiguousarray(weight[a:b])), _ptr(np.ascontiguousarray(phase[a:b])),
+                int(side), _interp(leaf_interp), _ptr(blocks), len(blocks), block, out_c64,
+                _ptr(out))
+            return out
+        # ffbp.py:162-167: the first child's regrid stored, the second's added,
+        # both in the storage type
+        for j, c in enumerate(n["kids"]):
+            child_img = compute(c)
+            co, sr0, sdr, su0, sdu = nodes[c]["geo"]
+            lib().orc_regrid_blocks(
+                _ptr(child_img), int(child_img.dtype == np.complex128), child_img.shape[0],
+                child_img.shape[1], _ptr(np.ascontiguousarray(co)), float(sr0), float(sdr),
+                float(su0), float(sdu), _ptr(np.ascontiguousarray(origin)), float(r0), float(dr),
+                n["nr"], float(u0), float(du), n["nu"], float(k), int(side),
+                _interp(merge_interp), _ptr(blocks), len(blocks), block, out_c64, j, _ptr(out))
+            del child_img
+        return out
+
+    img = compute(0)
+    cropped = np.ascontiguousarray(img[crop[0]:crop[1], crop[2]:crop[3]])
+    del img
+    values = evaluate_image(cropped, origin, cr0, dr, cu0, du, xyz, k, interp=eval_interp)
+    info["computed_fraction"] = info["computed_pixels"] / info["node_pixels"]
+    info["crop_shape"] = [crop[1] - crop[0], crop[3] - crop[2]]
+    info["elapsed_s"] = time.perf_counter() - start
+    return values, info
 
 
 # ---------------------------------------------------------------------------
