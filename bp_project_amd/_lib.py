@@ -65,6 +65,8 @@ SIGNATURES = {
     "bpmi_bp_grid_ws": [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _i, _d, _d, _i, _d, _d, _d, _d,
                         _i, _i, _i, _vp, _vp, _sz, _vp],
     "bpmi_bp_grid_workspace_bytes": [_i, _i, _i, POINTER(c_size_t)],
+    "bpmi_bp_grid_noalloc": [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _i, _d, _d, _i, _d, _d, _d,
+                             _d, _i, _i, _i, _vp, _vp],
     "bpmi_bp_points": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _ll, _d, _d, _d, _i, _i, _i, _vp,
                        _vp],
     "bpmi_target_contributions": [_vp, _i, _vp, _i, _vp, _ll, _d, _d, _d, _i, _i, _vp, _vp],

@@ -19,6 +19,8 @@
 //    fall back, per tile and pulse, to an exact FP64-geometry path.
 #include "bpmi_split.h"
 
+#include <mutex>
+
 namespace bpmi {
 
 
@@ -126,6 +128,45 @@ static int pulse_chunks(long long tiles, int pulses) {
     long long c = (SPLIT_TARGET_BLOCKS + tiles - 1) / tiles;
     c = std::min(c, (long long)(pulses / SPLIT_MIN_PULSES));
     return c < 2 ? 1 : (int)c;
+}
+
+// Partial images of bpmi_bp_grid_noalloc.  Each GPU the library runs on
+// holds its own copy of this array until the process ends; host code never
+// passes the array's name to a kernel (on the host it is only a stand-in
+// for the symbol), it uses the device address that grid_scratch_address()
+// resolves once per device.  Size: a grid splits its pulses only when it has
+// T < 1024 tiles of 1024 pixels, into at most ceil(2048 / T) chunks, so its
+// partials take at most (2048 + T - 1) * 1024 * 8 bytes < 24 MiB; 24 MiB
+// therefore holds every grid's partials (256 x 256 over 1024 pulses: 16
+// partials, 8 MiB), and the size check in bpmi_bp_grid_noalloc guards that
+// bound.  Every chunk overwrites all pixels of its partial, so nothing in it
+// has to start at zero.
+constexpr size_t GRID_SCRATCH_BYTES = (size_t)24 << 20;
+static __device__ __align__(256) unsigned char scratch[GRID_SCRATCH_BYTES];
+
+constexpr int GRID_SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[GRID_SCRATCH_MAX_DEVICES] = {};
+
+// Device address of ``scratch`` on the current device (resolved on the
+// first call per device, with that device current).
+static int grid_scratch_address(void** ptr) {
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st) return st;
+    if (dev < 0 || dev >= GRID_SCRATCH_MAX_DEVICES)
+        return set_error(BPMI_ERR_UNSUPPORTED, "device index %d beyond the %d devices with scratch",
+                         dev, GRID_SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        st = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (st) return st;
+        if (!p) return set_error(BPMI_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return BPMI_OK;
 }
 
 __global__ __launch_bounds__(256) void sum_partials_kernel(const float2* __restrict__ part,
@@ -270,6 +311,34 @@ extern "C" int bpmi_bp_grid_ws(const void* data, int samples, const double* pos,
     }
     launch_pixel<SRC_GRID>(mode, interp, (long long)nx * ny, hs, a, s);
     return check_launch("bp_pixel_kernel(grid)");
+}
+
+extern "C" int bpmi_bp_grid_noalloc(const void* data, int samples, const double* pos,
+                                    const double* weight, const double* phase, int pulse_begin,
+                                    int pulse_end, double x0, double dx, int nx, double y0,
+                                    double dy, int ny, double z0, double range_start,
+                                    double data_dr, double k, int mode, int interp, int flags,
+                                    void* out, void* stream) {
+    int st = validate_common(data, samples, pos, pulse_begin, pulse_end, mode, interp, flags, out);
+    if (st) return st;
+    if (nx <= 0 || ny <= 0) return set_error(BPMI_ERR_INVALID, "empty image grids are unsupported");
+    size_t need = 0;
+    st = bpmi_bp_grid_workspace_bytes(nx, ny, pulse_end - pulse_begin, &need);
+    if (st) return st;
+    if (mode != BPMI_MODE_SPLIT) need = 0;
+    if (need > GRID_SCRATCH_BYTES)
+        return set_error(BPMI_ERR_INVALID,
+                         "bpmi_bp_grid_noalloc: a %d x %d grid over %d pulses needs %zu bytes of "
+                         "partial images, above the %zu-byte static scratch (use bpmi_bp_grid_ws)",
+                         nx, ny, pulse_end - pulse_begin, need, GRID_SCRATCH_BYTES);
+    void* ws = nullptr;
+    if (need > 0) {
+        st = grid_scratch_address(&ws);
+        if (st) return st;
+    }
+    return bpmi_bp_grid_ws(data, samples, pos, weight, phase, pulse_begin, pulse_end, x0, dx, nx,
+                           y0, dy, ny, z0, range_start, data_dr, k, mode, interp, flags, out, ws,
+                           need, stream);
 }
 
 extern "C" int bpmi_bp_points(const void* data, int samples, const double* pos,

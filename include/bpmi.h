@@ -140,6 +140,23 @@ int bpmi_bp_grid_ws(const void* data, int samples, const double* pos, const doub
  * an nx x ny grid and that many pulses (0: the grid needs none). */
 int bpmi_bp_grid_workspace_bytes(int nx, int ny, int pulses, size_t* bytes);
 
+/* bpmi_bp_grid_ws that allocates nothing and takes no workspace: the
+ * pulse-split partial images live in a static 24 MiB device array of the
+ * library (one per GPU, held until the process ends), so the output equals
+ * bpmi_bp_grid_ws with bpmi_bp_grid_workspace_bytes() bytes bit for bit.
+ * Partials above 24 MiB are rejected (BPMI_ERR_INVALID); by the chunk rule no
+ * grid needs more (at most (2048 + T - 1) * 8 KiB for T < 1024 tiles of 1024
+ * pixels; grids of 1024 tiles or more need none).  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Calls that overlap on one GPU share
+ * the array and race; the first call on a device resolves the array's
+ * address, so make it outside graph capture. */
+int bpmi_bp_grid_noalloc(const void* data, int samples, const double* pos, const double* weight,
+                         const double* phase, int pulse_begin, int pulse_end, double x0,
+                         double dx, int nx, double y0, double dy, int ny, double z0,
+                         double range_start, double data_dr, double k, int mode, int interp,
+                         int flags, void* out, void* stream);
+
 /* Arbitrary points xyz f64 [npoints, 3] (device).  Replaces
  * backproject_points (fast_bp/ffbp.py:43-55). */
 int bpmi_bp_points(const void* data, int samples, const double* pos, const double* weight,

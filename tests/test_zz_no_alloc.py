@@ -1,0 +1,87 @@
+"""bpmi_bp_grid_noalloc (pulse-split direct BP with its partial images in the
+library's static device array) against bpmi_bp_grid_ws with a workspace of
+bpmi_bp_grid_workspace_bytes(): the same chunks and the same kernels, so the
+images must be equal bit for bit.  Two calls in a row, the second on a
+smaller grid with fewer pulses (the array still holds the first call's
+partials), then a grid too large to split (no scratch) and a non-split mode."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from bp_project_amd import _lib
+from bp_project_amd.device import DeviceArray, synchronize
+
+pytestmark = pytest.mark.gpu
+_vp = _lib.c_void_p
+
+
+@pytest.fixture(scope="module")
+def c1():
+    from bp_project_amd.scenes import workload_scene
+    s = workload_scene("c1_direct_small")
+    return s, DeviceArray.from_host(s.data), DeviceArray.from_host(s.pos)
+
+
+def _args(s, dev, p, nx, ny, pulses, mode, interp, flags):
+    g = s.grid
+    # a sub-grid of the scene's grid, centred on it, at the same spacing
+    x0 = g.x0 + g.dx * (g.nx - nx) / 2
+    y0 = g.y0 + g.dy * (g.ny - ny) / 2
+    return (_vp(dev.ptr), s.bins, _vp(p.ptr), None, None, 0, pulses, x0, g.dx, nx, y0, g.dy, ny,
+            0.0, s.r_start, s.data_dr, s.k, mode,
+            _lib.interp_code(interp), flags)
+
+
+def _pair(s, dev, p, nx, ny, pulses, mode=_lib.MODE_SPLIT, interp="linear", flags=0, prior=None):
+    """(no-alloc image, workspace image, workspace bytes) for one call shape."""
+    a = _args(s, dev, p, nx, ny, pulses, mode, interp, flags)
+    n = ctypes.c_size_t()
+    _lib.call("bpmi_bp_grid_workspace_bytes", nx, ny, pulses, ctypes.byref(n))
+    ws = DeviceArray.empty((max(n.value, 8) // 8,))
+    out_ws = DeviceArray.from_host(prior) if prior is not None else DeviceArray.empty((ny, nx))
+    out_na = DeviceArray.from_host(prior) if prior is not None else DeviceArray.empty((ny, nx))
+    print(f"grid {nx}x{ny}, {pulses} pulses, mode {mode}, {interp}, flags {flags}: "
+          f"workspace {n.value / 2**20:.2f} MiB", flush=True)
+    _lib.call("bpmi_bp_grid_ws", *a, _vp(out_ws.ptr), _vp(ws.ptr) if n.value else None,
+              n.value, None)
+    _lib.call("bpmi_bp_grid_noalloc", *a, _vp(out_na.ptr), None)
+    synchronize()
+    return out_na.get(), out_ws.get(), n.value
+
+
+def test_noalloc_equals_workspace_entry_two_calls(c1):
+    s, dev, p = c1
+    g = s.grid
+    # 1: the C1 grid (256 x 256, 1024 pulses): 16 partials, 8 MiB of scratch
+    na, ws, nbytes = _pair(s, dev, p, g.nx, g.ny, s.pulses)
+    assert nbytes == 16 * g.nx * g.ny * 8
+    assert np.isfinite(na).all() and np.abs(na).max() > 0
+    assert np.array_equal(na, ws)
+    # 2: smaller grid, fewer pulses (ragged chunks), Lagrange-4, accumulated
+    # onto a prior image; the scratch still holds call 1's partials
+    prior = (np.arange(90 * 100, dtype=np.float32).reshape(90, 100) * (1 + 0.5j)).astype(
+        np.complex64)
+    na2, ws2, nbytes2 = _pair(s, dev, p, 100, 90, 700, interp="lagrange4",
+                              flags=_lib.ACCUMULATE, prior=prior)
+    assert 0 < nbytes2 < nbytes
+    assert np.array_equal(na2, ws2)
+    assert not np.array_equal(na2, prior)
+
+
+def test_noalloc_without_scratch_and_other_modes(c1):
+    s, dev, p = c1
+    # 1024 x 1024 has 1024 tiles: no pulse split, no scratch
+    na, ws, nbytes = _pair(s, dev, p, 1024, 1024, 256)
+    assert nbytes == 0
+    assert np.array_equal(na, ws)
+    # the FP64 per-pixel kernel takes no scratch in any case
+    na, ws, _ = _pair(s, dev, p, 64, 48, 300, mode=_lib.MODE_FP64)
+    assert np.array_equal(na, ws)
+
+
+def test_noalloc_argument_errors(c1):
+    s, dev, p = c1
+    a = list(_args(s, dev, p, 0, 16, 64, _lib.MODE_SPLIT, "linear", 0))
+    with pytest.raises(ValueError, match="empty image grids"):
+        _lib.call("bpmi_bp_grid_noalloc", *a, _vp(dev.ptr), None)
