@@ -63,7 +63,12 @@ def patch_mix(count=64, seed=20260921, scale=1.0):
 
 
 def make_patches(specs):
-    return [stripmap_scene(**s) for s in specs]
+    """The scenes of ``specs``, generated concurrently (one thread per
+    scene; each scene is the same as generated alone)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from .scenes import _scene_threads
+    with ThreadPoolExecutor(max_workers=_scene_threads()) as pool:
+        return list(pool.map(lambda s: stripmap_scene(**s, threads=1), specs))
 
 
 class PatchPlan:

@@ -22,6 +22,8 @@ each target), plus complex Gaussian clutter at 20 dB SNR.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -198,9 +200,51 @@ class Scene:
         return self.data.shape[1]
 
 
+SCENE_ROWS = 512     # pulses per generation chunk (fixes the clutter streams)
+
+
+def _scene_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _fill_rows(data, p0, p1, true_pos, targets, amp, k, r_start, data_dr, phase_error, scale,
+               seed):
+    """Rows [p0, p1) of a stripmap scene: every target's truncated sinc
+    response (targets in order), the per-pulse phase error, then complex
+    Gaussian clutter drawn from the chunk's own seeded stream, so the scene
+    does not depend on how many threads fill it."""
+    rows = data[p0:p1]
+    rows[...] = 0
+    bins = data.shape[1]
+    res = C / (2 * BANDWIDTH)
+    offsets = np.arange(-SINC_HALF_BINS, SINC_HALF_BINS + 1)
+    flat = rows.reshape(-1)
+    base = (np.arange(p1 - p0) * bins)[:, None]
+    for t in range(len(targets)):
+        d = np.linalg.norm(true_pos[p0:p1] - targets[t], axis=1)
+        centre = np.rint((d - r_start) / data_dr).astype(np.int64)
+        idx = centre[:, None] + offsets[None, :]
+        valid = (idx >= 0) & (idx < bins)
+        vals = amp[t] * np.exp(-1j * k * d)[:, None] * np.sinc(
+            (r_start + data_dr * idx - d[:, None]) / res)
+        # (row, bin) pairs are distinct within one target, so += is exact
+        flat[(base + idx)[valid]] += vals[valid].astype(np.complex64)
+    if np.any(phase_error[p0:p1]):
+        rows *= np.exp(1j * phase_error[p0:p1]).astype(np.complex64)[:, None]
+    if scale is not None:
+        g = np.random.default_rng([seed, p0 // SCENE_ROWS, 0x5CE7E])
+        rows.real += g.standard_normal(rows.shape, dtype=np.float32) * scale
+        rows.imag += g.standard_normal(rows.shape, dtype=np.float32) * scale
+
+
 def stripmap_scene(pulses, bins, n_targets, grid_n, grid_spacing, seed, *, ground_range=5000.0,
                    slant_range=None, patch=None, curved=False, motion_rms=0.0,
-                   phase_walk_rms=0.0, along_track_error=0.0, snr_db=20.0, target_margin=0.1):
+                   phase_walk_rms=0.0, along_track_error=0.0, snr_db=20.0, target_margin=0.1,
+                   threads=None):
     """Seeded stripmap scene.
 
     ``patch`` is the side of the square target patch in metres (default: the
@@ -210,6 +254,10 @@ def stripmap_scene(pulses, bins, n_targets, grid_n, grid_spacing, seed, *, groun
     adds a seeded 3-axis random walk scaled to that RMS (metres) to the
     true antenna path.  ``phase_walk_rms`` (rad) and ``along_track_error`` (m)
     corrupt the data relative to the nominal ``pos`` for autofocus studies.
+    The data are filled in chunks of SCENE_ROWS pulses on ``threads`` threads
+    (default: the usable cores, at most 16); the clutter of each chunk comes
+    from its own stream seeded by (seed, chunk), so the scene is the same for
+    any thread count.
     """
     rng = np.random.default_rng(seed)
     if slant_range is not None:
@@ -244,36 +292,20 @@ def stripmap_scene(pulses, bins, n_targets, grid_n, grid_spacing, seed, *, groun
     amp = rng.uniform(0.5, 1.0, n_targets) * np.exp(1j * rng.uniform(0, 2 * np.pi, n_targets))
     centre_slant = float(np.hypot(ground_range, ALTITUDE))
     r_start = centre_slant - 0.5 * bins * data_dr
-    res = C / (2 * BANDWIDTH)
-    data = np.zeros((pulses, bins), np.complex64)
-    w = 2 * SINC_HALF_BINS + 1
-    offsets = np.arange(-SINC_HALF_BINS, SINC_HALF_BINS + 1)
-    for t in range(n_targets):
-        d = np.linalg.norm(true_pos - targets[t], axis=1)
-        centre = np.rint((d - r_start) / data_dr).astype(np.int64)
-        idx = centre[:, None] + offsets[None, :]
-        valid = (idx >= 0) & (idx < bins)
-        rbin = r_start + data_dr * idx
-        vals = amp[t] * np.exp(-1j * k * d)[:, None] * np.sinc((rbin - d[:, None]) / res)
-        rows = np.broadcast_to(np.arange(pulses)[:, None], idx.shape)
-        # (row, bin) pairs are distinct within one target, so += is exact
-        data[rows[valid], idx[valid]] += vals[valid].astype(np.complex64)
     phase_error = np.zeros(pulses)
     if phase_walk_rms > 0:
         walk = np.cumsum(rng.standard_normal(pulses))
         walk -= walk.mean()
         phase_error = walk * phase_walk_rms / np.sqrt(np.mean(walk ** 2))
-        data *= np.exp(1j * phase_error).astype(np.complex64)[:, None]
+    data = np.empty((pulses, bins), np.complex64)
+    scale = None
     if snr_db is not None:
-        sigma2 = (7.0 / 12.0) / 10 ** (snr_db / 10)
-        scale = np.sqrt(sigma2 / 2)
-        noise = rng.standard_normal((pulses, bins), dtype=np.float32)
-        noise = noise * np.float32(scale)
-        data.real += noise
-        noise = rng.standard_normal((pulses, bins), dtype=np.float32)
-        noise = noise * np.float32(scale)
-        data.imag += noise
-        del noise
+        scale = np.float32(np.sqrt((7.0 / 12.0) / 10 ** (snr_db / 10) / 2))
+    chunks = range(0, pulses, SCENE_ROWS)
+    with ThreadPoolExecutor(max_workers=threads or _scene_threads()) as pool:
+        list(pool.map(lambda p0: _fill_rows(data, p0, min(pulses, p0 + SCENE_ROWS), true_pos,
+                                            targets, amp, k, r_start, data_dr, phase_error,
+                                            scale, seed), chunks))
     g0 = grid_n * grid_spacing / 2
     grid = CartesianGrid(x0=ground_range - g0, dx=grid_spacing, nx=grid_n, y0=-g0,
                          dy=grid_spacing, ny=grid_n)
