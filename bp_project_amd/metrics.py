@@ -7,8 +7,9 @@
   (sar_quality/metrics.py:6-22).
 * ``impulse_metrics(x, z)`` -- 1-D half-power IRW, first-minima mainlobe,
   PSLR and ISLR (sar_quality/metrics.py:42-74).
-* ``profile_metrics`` / ``analyze_target`` -- the qualified-minima 1-D/2-D
-  point-response measurements (vancouver_bp/quality.py:73-239).
+* ``profile_metrics`` -- the qualified-null 1-D point-response measurement
+  (the convention of vancouver_bp/quality.py:73-176), ``cut_deltas`` and
+  ``pslr_islr_deltas`` -- its PSLR/ISLR differences between two images.
 * ``peak_phase_error`` -- phase difference at each target's peak pixel.
 * ``timing_summary`` -- median/IQR of timing samples.
 """
@@ -19,10 +20,6 @@ import math
 import numpy as np
 from scipy.ndimage import gaussian_filter1d
 from scipy.signal import find_peaks
-
-
-def _trapz(y, x, axis=-1):
-    return np.trapezoid(y, x, axis=axis)
 
 
 def _db(value):
@@ -106,159 +103,175 @@ def image_entropy(z):
     return float(-np.sum(p[nz] * np.log(p[nz])))
 
 
+# ---------------------------------------------------------------------------
+# 1-D point-response measurements
+#
+# A cut z(x) through a point target is measured on its normalised power
+# p = |z|^2 / |z(peak)|^2.  The impulse-response width is the distance between
+# the two half-power crossings (linear interpolation between samples).  The
+# mainlobe ends at the first null on each side; PSLR is the highest power
+# outside the mainlobe and ISLR the power integrated (trapezoid rule) outside
+# it over the power integrated inside it, both in dB.
+#
+# How the nulls are found fixes the metric, so the constants below are the
+# reference's (vancouver_bp/quality.py:73-176, "qualified first minima"):
+# minima are looked for on the power smoothed by a Gaussian of
+# NULL_SMOOTH_PER_IRW x IRW samples, clamped to NULL_SMOOTH_LIMITS; a null is
+# the minimum nearest the mainlobe whose unsmoothed power is at most
+# NULL_DEPTHS[0] of the peak, or failing that NULL_DEPTHS[1]; the ratios are
+# reliable when both nulls lie at or below RELIABLE_NULL_DB, the integration
+# window is not truncated and the half-power region is one piece.
+# impulse_metrics is the plain variant of sar_quality/metrics.py:42-74:
+# nulls are the nearest raw local minima.
+# ---------------------------------------------------------------------------
+NULL_SMOOTH_PER_IRW = 0.04
+NULL_SMOOTH_LIMITS = (0.5, 4.0)
+NULL_DEPTHS = (0.1, 0.25)
+RELIABLE_NULL_DB = -15.0
+
+
+def _level_crossing(xa, xb, pa, pb, level=0.5):
+    """x where the line through (xa, pa), (xb, pb) reaches ``level``."""
+    if pa == pb:
+        return float(0.5 * (xa + xb))
+    return float(xa + (xb - xa) * (level - pa) / (pb - pa))
+
+
+def _half_power(x, p, peak):
+    """(last index left of the peak with p <= 1/2, first such index right of
+    it, left crossing, right crossing), or None when a side never drops."""
+    left = np.flatnonzero(p[:peak] <= 0.5)
+    right = np.flatnonzero(p[peak + 1:] <= 0.5)
+    if left.size == 0 or right.size == 0:
+        return None
+    lo, hi = int(left[-1]), int(right[0]) + peak + 1
+    return (lo, hi, _level_crossing(x[lo], x[lo + 1], p[lo], p[lo + 1]),
+            _level_crossing(x[hi - 1], x[hi], p[hi - 1], p[hi]))
+
+
+def _local_minima(v):
+    return find_peaks(-v)[0]
+
+
+def _qualified_nulls(x, p, lo, hi, irw):
+    """Indices of the first qualified null left of ``lo`` and right of
+    ``hi`` (None where there is none)."""
+    step = float(np.median(np.diff(x)))
+    sigma = float(np.clip(NULL_SMOOTH_PER_IRW * irw / step, *NULL_SMOOTH_LIMITS))
+    mins = _local_minima(gaussian_filter1d(p, sigma, mode="nearest"))
+    interior = mins[(mins > 0) & (mins < len(p) - 1)]
+    sides = (interior[interior < lo][::-1], interior[interior > hi])
+    nulls = []
+    for cand in sides:
+        pick = None
+        for depth in NULL_DEPTHS:
+            deep = cand[p[cand] <= depth]
+            if deep.size:
+                pick = int(deep[0])
+                break
+        nulls.append(pick)
+    return nulls[0], nulls[1]
+
+
+def _lobe_ratios(x, p, ml, mr):
+    """(PSLR dB, ISLR dB) for the mainlobe [ml, mr] of normalised power p."""
+    inside = float(np.trapezoid(p[ml:mr + 1], x[ml:mr + 1]))
+    outside = float(np.trapezoid(p[:ml + 1], x[:ml + 1]) + np.trapezoid(p[mr:], x[mr:]))
+    peak_side = float(max(p[:ml].max(initial=0.0), p[mr + 1:].max(initial=0.0)))
+    return _db(peak_side), _db(outside / inside)
+
+
 def impulse_metrics(x, z):
-    """1-D half-power IRW; first-minima mainlobe; finite-cut PSLR/ISLR."""
+    """1-D half-power IRW, mainlobe between the nearest raw minima, and the
+    PSLR/ISLR of the finite cut."""
     x = np.asarray(x, float)
-    p = abs(np.asarray(z)) ** 2
+    p = np.abs(np.asarray(z)) ** 2
     if not np.isfinite(p).all() or p.max() <= 0:
-        return dict(valid=False, reason="Invalid power")
+        return dict(valid=False, reason="power is not finite and positive")
     peak = int(np.argmax(p))
     p = p / p[peak]
-    left = np.flatnonzero(p[:peak] <= .5)
-    right = np.flatnonzero(p[peak + 1:] <= .5) + peak + 1
-    if not len(left) or not len(right):
-        return dict(valid=False, reason="Half-power crossing absent")
-    il, ir = int(left[-1]), int(right[0])
-    xl = np.interp(.5, p[il:il + 2], x[il:il + 2])
-    xr = np.interp(.5, p[ir - 1:ir + 1][::-1], x[ir - 1:ir + 1][::-1])
-    mins = find_peaks(-p)[0]
-    ml, mr = mins[mins < il], mins[mins > ir]
-    if not len(ml) or not len(mr):
-        return dict(valid=True, irw_m=float(xr - xl), peak_offset_m=float(x[peak]),
-                    sidelobes_valid=False, reason="First minima absent")
-    ml, mr = int(ml[-1]), int(mr[0])
-    em = _trapz(p[ml:mr + 1], x[ml:mr + 1])
-    es = _trapz(p[:ml + 1], x[:ml + 1]) + _trapz(p[mr:], x[mr:])
-    return dict(valid=True, sidelobes_valid=True, irw_m=float(xr - xl),
-                peak_offset_m=float(x[peak]),
-                pslr_db=float(10 * np.log10(max(p[:ml].max(), p[mr + 1:].max()))),
-                islr_db=float(10 * np.log10(es / em)),
+    hp = _half_power(x, p, peak)
+    if hp is None:
+        return dict(valid=False, reason="no half-power point on one side of the peak")
+    lo, hi, xl, xr = hp
+    out = dict(valid=True, irw_m=float(xr - xl), peak_offset_m=float(x[peak]))
+    mins = _local_minima(p)
+    left, right = mins[mins < lo], mins[mins > hi]
+    if left.size == 0 or right.size == 0:
+        return dict(out, sidelobes_valid=False, reason="no minimum beyond the half-power points")
+    ml, mr = int(left[-1]), int(right[0])
+    pslr, islr = _lobe_ratios(x, p, ml, mr)
+    return dict(out, sidelobes_valid=True, pslr_db=pslr, islr_db=islr,
                 mainlobe_bounds_m=[float(x[ml]), float(x[mr])])
 
 
-def _crossing(x0, x1, p0, p1, level=0.5):
-    if p1 == p0:
-        return float((x0 + x1) / 2)
-    return float(x0 + (level - p0) * (x1 - x0) / (p1 - p0))
-
-
-def _qualified_minima(x, p, il, ir, irw):
-    dx = float(np.median(np.diff(x)))
-    sigma = min(4.0, max(0.5, 0.04 * irw / dx))
-    smooth = gaussian_filter1d(p, sigma, mode="nearest")
-    minima = find_peaks(-smooth)[0]
-    left = minima[(minima < il) & (minima > 0)][::-1]
-    right = minima[(minima > ir) & (minima < len(p) - 1)]
-
-    def first_deep(indices):
-        for limit in (0.1, 0.25):
-            deep = [int(i) for i in indices if p[i] <= limit]
-            if deep:
-                return deep[0]
-        return None
-
-    return first_deep(left), first_deep(right), sigma
-
-
 def profile_metrics(x, z, *, expected_irw_m=None, integration_half_width_m=None, peak_index=None):
-    """1-D point response with qualified first minima (vancouver_bp/quality.py:73-176)."""
-    x = np.asarray(x, dtype=np.float64)
+    """Point response of one cut with the qualified-null convention above.
+
+    ``integration_half_width_m`` restricts the measurement to that distance
+    either side of the peak; ``peak_index`` names the peak sample (default:
+    the cut's maximum).  Returns a dict with ``valid`` (IRW measured),
+    ``sidelobes_valid`` (PSLR/ISLR reliable), ``irw_m``, ``pslr_db``,
+    ``islr_db``, the mainlobe bounds and null depths, and ``reason`` or
+    ``warnings`` where something is missing or doubtful."""
+    x = np.asarray(x, np.float64)
     z = np.asarray(z)
-    if x.ndim != 1 or z.ndim != 1 or len(x) != len(z) or len(x) < 5:
-        raise ValueError("x and z must be matching one-dimensional arrays with at least five samples")
-    if not np.all(np.isfinite(x)) or np.any(np.diff(x) <= 0):
+    if x.ndim != 1 or x.shape != z.shape or x.size < 5:
+        raise ValueError("x and z must be 1-D, of equal length, with at least 5 samples")
+    if not np.isfinite(x).all() or np.any(np.diff(x) <= 0):
         raise ValueError("x must be finite and strictly increasing")
     power = np.abs(z.astype(np.complex128)) ** 2
-    result = dict(valid=False, sidelobes_valid=False, warnings=[])
-    if not np.all(np.isfinite(power)) or not np.any(power > 0):
-        return dict(result, reason="nonfinite or zero power")
-    peak_all = int(np.argmax(power)) if peak_index is None else int(peak_index)
-    if not 0 <= peak_all < len(power) or power[peak_all] <= 0:
-        raise ValueError("peak_index must identify a positive-power sample inside the supplied cut")
-    centre = float(x[peak_all])
+    out = dict(valid=False, sidelobes_valid=False, warnings=[])
+    if not np.isfinite(power).all() or not np.any(power > 0):
+        return dict(out, reason="power is not finite and positive")
+    peak = int(np.argmax(power)) if peak_index is None else int(peak_index)
+    if not (0 <= peak < power.size) or power[peak] <= 0:
+        raise ValueError("peak_index must name a sample of the cut with positive power")
+    keep = np.arange(x.size)
     truncated = False
     if integration_half_width_m is not None:
-        hw = float(integration_half_width_m)
-        if hw <= 0:
+        half = float(integration_half_width_m)
+        if half <= 0:
             raise ValueError("integration_half_width_m must be positive")
-        truncated = centre - hw < x[0] or centre + hw > x[-1]
-        mask = (x >= centre - hw) & (x <= centre + hw)
-        idx = np.flatnonzero(mask)
-        x, power = x[mask], power[mask]
-    else:
-        idx = np.arange(len(x))
-    if len(x) < 5:
-        return dict(result, reason="integration window contains fewer than five samples")
-    peak = int(peak_all - idx[0])
-    p = power / float(power[peak])
-    dx = np.diff(x)
-    result.update(peak_coordinate_m=float(x[peak]), peak_sample_index=int(idx[peak]),
-                  step_m=float(np.median(dx)), integration_bounds_m=[float(x[0]), float(x[-1])],
-                  integration_window_truncated=bool(truncated))
-    below_left = np.flatnonzero(p[:peak] <= 0.5)
-    below_right = np.flatnonzero(p[peak + 1:] <= 0.5) + peak + 1
-    if not len(below_left) or not len(below_right):
-        return dict(result, reason="half-power crossings absent on one or both sides")
-    il, ir = int(below_left[-1]), int(below_right[0])
-    xl = _crossing(x[il], x[il + 1], p[il], p[il + 1])
-    xr = _crossing(x[ir - 1], x[ir], p[ir - 1], p[ir])
-    irw = float(xr - xl)
-    result.update(valid=True, irw_m=irw, half_power_bounds_m=[xl, xr])
+        centre = x[peak]
+        truncated = bool(centre - half < x[0] or centre + half > x[-1])
+        keep = np.flatnonzero(np.abs(x - centre) <= half)
+        x, power = x[keep], power[keep]
+    if x.size < 5:
+        return dict(out, reason="fewer than 5 samples inside the integration window")
+    peak -= int(keep[0])
+    p = power / power[peak]
+    out.update(peak_coordinate_m=float(x[peak]), peak_sample_index=int(keep[peak]),
+               step_m=float(np.median(np.diff(x))), integration_bounds_m=[float(x[0]), float(x[-1])],
+               integration_window_truncated=truncated)
+    hp = _half_power(x, p, peak)
+    if hp is None:
+        return dict(out, reason="no half-power point on one side of the peak")
+    lo, hi, xl, xr = hp
+    out.update(valid=True, irw_m=float(xr - xl), half_power_bounds_m=[xl, xr])
     if expected_irw_m is not None:
-        result["irw_to_expected_ratio"] = irw / float(expected_irw_m)
-    ml, mr, _ = _qualified_minima(x, p, il, ir, irw)
+        out["irw_to_expected_ratio"] = out["irw_m"] / float(expected_irw_m)
+    ml, mr = _qualified_nulls(x, p, lo, hi, out["irw_m"])
     if ml is None or mr is None:
-        return dict(result, reason="qualified first minima absent; sidelobe ratios not identified")
-    if ml == 0 or mr == len(p) - 1:
-        return dict(result, reason="mainlobe touches integration boundary")
-    main = float(_trapz(p[ml:mr + 1], x[ml:mr + 1]))
-    side = float(_trapz(p[:ml + 1], x[:ml + 1]) + _trapz(p[mr:], x[mr:]))
-    side_values = np.r_[p[:ml], p[mr + 1:]]
-    if main <= 0 or not len(side_values):
-        return dict(result, reason="main/side integration regions invalid")
+        return dict(out, reason="no qualified null on one side; no sidelobe ratios")
+    if ml == 0 or mr == p.size - 1:
+        return dict(out, reason="a null lies on the window edge")
+    pslr, islr = _lobe_ratios(x, p, ml, mr)
     depths = [_db(p[ml]), _db(p[mr])]
-    half_components = int(np.count_nonzero(np.diff(np.r_[False, p >= .5, False].astype(int)) == 1))
-    deep = bool(max(depths) <= -15.0)
-    reliable = deep and not truncated and half_components == 1
-    result.update(sidelobes_valid=bool(reliable), pslr_db=_db(float(np.max(side_values))),
-                  islr_db=_db(side / main), mainlobe_bounds_m=[float(x[ml]), float(x[mr])],
-                  mainlobe_minima_db=depths,
-                  mainlobe_sample_indices=[int(idx[ml]), int(idx[mr])],
-                  half_power_components=half_components)
+    rises = np.diff(np.r_[0, (p >= 0.5).astype(np.int8), 0])
+    pieces = int(np.count_nonzero(rises == 1))
+    deep = max(depths) <= RELIABLE_NULL_DB
+    out.update(sidelobes_valid=bool(deep and not truncated and pieces == 1), pslr_db=pslr,
+               islr_db=islr, mainlobe_bounds_m=[float(x[ml]), float(x[mr])],
+               mainlobe_minima_db=depths, mainlobe_sample_indices=[int(keep[ml]), int(keep[mr])],
+               half_power_components=pieces)
     if not deep:
-        result["warnings"].append("mainlobe minima shallower than -15 dB")
+        out["warnings"].append(f"a null is shallower than {RELIABLE_NULL_DB:g} dB")
     if truncated:
-        result["warnings"].append("requested integration window exceeds supplied cut")
-    if half_components != 1:
-        result["warnings"].append("multiple half-power components")
-    return result
-
-
-def analyze_target(r, y, image, *, peak_hint=None, search_radius_m=None,
-                   integration_half_widths_m=None):
-    """Range and azimuth cuts through one target of image[r, y]
-    (vancouver_bp/quality.py:179-239, 1-D part)."""
-    r, y = np.asarray(r, float), np.asarray(y, float)
-    image = np.asarray(image)
-    if image.shape != (len(r), len(y)):
-        raise ValueError("image shape must be (len(r), len(y))")
-    power = np.abs(image.astype(np.complex128)) ** 2
-    if not np.all(np.isfinite(power)) or not np.any(power > 0):
-        raise ValueError("target image has nonfinite or zero power")
-    selection = np.ones(image.shape, bool)
-    if peak_hint is not None and search_radius_m is not None:
-        dr, dy = search_radius_m if np.ndim(search_radius_m) else (search_radius_m,) * 2
-        selection = ((np.abs(r[:, None] - peak_hint[0]) <= dr)
-                     & (np.abs(y[None, :] - peak_hint[1]) <= dy))
-        if not np.any(selection):
-            raise ValueError("peak search window contains no pixels")
-    ir, iy = np.unravel_index(np.argmax(np.where(selection, power, -np.inf)), image.shape)
-    half = (None, None) if integration_half_widths_m is None else integration_half_widths_m
-    rng = profile_metrics(r, image[:, iy], integration_half_width_m=half[0], peak_index=ir)
-    az = profile_metrics(y, image[ir, :], integration_half_width_m=half[1], peak_index=iy)
-    return dict(peak=dict(range_m=float(r[ir]), azimuth_m=float(y[iy]), index=[int(ir), int(iy)],
-                          amplitude=float(np.sqrt(power[ir, iy]))),
-                range=rng, azimuth=az)
+        out["warnings"].append("integration window extends beyond the cut")
+    if pieces != 1:
+        out["warnings"].append("the half-power region has more than one piece")
+    return out
 
 
 def peak_phase_error(reference, estimate, peaks):
@@ -288,28 +301,39 @@ def target_peaks(image, grid, targets, search_m=2.0):
     return peaks
 
 
+def cut_deltas(axis, ref_cut, est_cut, peak_index, half_width_m=8.0):
+    """PSLR/ISLR of one 1-D cut of ``est`` minus the same of ``ref`` (both
+    measured about ``peak_index`` within +-half_width_m), or None when either
+    has no sidelobe ratios.  A decreasing axis is measured reversed."""
+    axis = np.asarray(axis, np.float64)
+    ref_cut, est_cut = np.asarray(ref_cut), np.asarray(est_cut)
+    if axis.size > 1 and axis[1] < axis[0]:
+        axis, ref_cut, est_cut = axis[::-1], ref_cut[::-1], est_cut[::-1]
+        peak_index = axis.size - 1 - peak_index
+    a = profile_metrics(axis, ref_cut, integration_half_width_m=half_width_m,
+                        peak_index=peak_index)
+    b = profile_metrics(axis, est_cut, integration_half_width_m=half_width_m,
+                        peak_index=peak_index)
+    if "pslr_db" not in a or "pslr_db" not in b:
+        return None
+    return dict(pslr_delta_db=b["pslr_db"] - a["pslr_db"], islr_delta_db=b["islr_db"] - a["islr_db"],
+                pslr_ref_db=a["pslr_db"], islr_ref_db=a["islr_db"])
+
+
 def pslr_islr_deltas(reference, estimate, grid, peaks, half_width_m=8.0):
-    """PSLR/ISLR (dB) of range (x) and azimuth (y) cuts through each peak in
-    ``estimate`` minus the same in ``reference``; returns the worst absolute
-    deltas and per-target values."""
-    x, y = grid.x, grid.y
+    """PSLR/ISLR (dB) of the range (x) and azimuth (y) cuts through each peak
+    (row, col) of two images [ny, nx] on ``grid``: estimate minus reference.
+    Returns the worst absolute delta and the per-target records."""
     rows = []
     for (i, j) in peaks:
         rec = {}
-        for name, ref_cut, est_cut, axis, idx in (
-                ("range", reference[i, :], estimate[i, :], x, j),
-                ("azimuth", reference[:, j], estimate[:, j], y, i)):
-            ax = axis if axis[1] > axis[0] else axis[::-1]
-            rc = ref_cut if axis[1] > axis[0] else ref_cut[::-1]
-            ec = est_cut if axis[1] > axis[0] else est_cut[::-1]
-            pk = idx if axis[1] > axis[0] else len(axis) - 1 - idx
-            a = profile_metrics(ax, rc, integration_half_width_m=half_width_m, peak_index=pk)
-            b = profile_metrics(ax, ec, integration_half_width_m=half_width_m, peak_index=pk)
-            if "pslr_db" in a and "pslr_db" in b:
-                rec[name] = dict(pslr_delta_db=b["pslr_db"] - a["pslr_db"],
-                                 islr_delta_db=b["islr_db"] - a["islr_db"],
-                                 pslr_ref_db=a["pslr_db"], islr_ref_db=a["islr_db"])
+        for name, axis, ref_cut, est_cut, at in (
+                ("range", grid.x, reference[i, :], estimate[i, :], j),
+                ("azimuth", grid.y, reference[:, j], estimate[:, j], i)):
+            d = cut_deltas(axis, ref_cut, est_cut, at, half_width_m)
+            if d is not None:
+                rec[name] = d
         rows.append(rec)
-    deltas = [abs(v[k]) for rec in rows for v in rec.values()
-              for k in ("pslr_delta_db", "islr_delta_db")]
-    return {"max_abs_delta_db": float(max(deltas)) if deltas else None, "targets": rows}
+    worst = [abs(v[key]) for rec in rows for v in rec.values()
+             for key in ("pslr_delta_db", "islr_delta_db")]
+    return {"max_abs_delta_db": float(max(worst)) if worst else None, "targets": rows}
