@@ -62,7 +62,7 @@ def test_cut_deltas_identity_perturbation_and_reversed_axis():
     d = metrics.cut_deltas(x, z, z, 64)
     assert d["pslr_delta_db"] == 0 and d["islr_delta_db"] == 0
     assert abs(d["pslr_ref_db"] + 13.26) < 0.3      # 4 samples per sidelobe
-    z2 = z + 0.1 * np.sinc((x - 3.0) / 1.0)       # a raised sidelobe at +3 m
+    z2 = z + 0.2 * np.sinc((x - 3.0) / 1.0)       # a raised sidelobe at +3 m
     d2 = metrics.cut_deltas(x, z, z2, 64)
     assert d2["pslr_delta_db"] > 1.0 and d2["islr_delta_db"] > 0.5
     d3 = metrics.cut_deltas(x[::-1], z[::-1], z2[::-1], 64)
