@@ -252,7 +252,8 @@ def stripmap_scene(pulses, bins, n_targets, grid_n, grid_spacing, seed, *, groun
     scene centre.  ``curved`` bends the nominal track (quadratic cross-track
     bow of 10 m plus a 2 m altitude swell over the aperture); ``motion_rms``
     adds a seeded 3-axis random walk scaled to that RMS (metres) to the
-    true antenna path.  ``phase_walk_rms`` (rad) and ``along_track_error`` (m)
+    antenna path, which the positions ``pos`` include (a known, irregular
+    track).  ``phase_walk_rms`` (rad) and ``along_track_error`` (m)
     corrupt the data relative to the nominal ``pos`` for autofocus studies.
     The data are filled in chunks of SCENE_ROWS pulses on ``threads`` threads
     (default: the usable cores, at most 16); the clutter of each chunk comes
@@ -272,12 +273,12 @@ def stripmap_scene(pulses, bins, n_targets, grid_n, grid_spacing, seed, *, groun
         nominal[:, 0] += 10.0 * s * s
         nominal[:, 2] += 2.0 * np.sin(np.pi * s)
     pos = nominal + rng.normal(scale=1e-3, size=nominal.shape)
-    true_pos = pos.copy()
     if motion_rms > 0:
         walk = np.cumsum(rng.standard_normal((pulses, 3)), axis=0)
         walk -= walk.mean(axis=0)
         walk *= motion_rms / np.sqrt(np.mean(walk ** 2, axis=0))
-        true_pos += walk
+        pos += walk
+    true_pos = pos.copy()
     if along_track_error > 0:
         drift = np.cumsum(rng.standard_normal(pulses))
         drift -= drift.mean()

@@ -1,0 +1,301 @@
+"""Every stripmap workload at its own full size on the GPU, against the FP64
+oracle on the same seeded inputs.
+
+* C2 (FFBP, 8192 pulses x 8192 bins -> 4096^2 at 0.25 m, leaf 16, 9 merge
+  levels, 8-tap Kaiser-sinc merges and evaluation): the benchmark's split
+  path (tiles culled to the grid footprint) and the FP64 mode, against the
+  oracle's complex128 FFBP of the same plan evaluated at 4096 seeded pixels
+  (16 seeded 16 x 16 blocks) plus a 65-pixel range and azimuth cut through
+  each of the 200 targets (oracle.ffbp_at_points computes only the node
+  pixels those values depend on); two consecutive steps are bitwise equal
+  (the reference's determinism check, fast_bp/gpu_baseline.py:542-545).
+* C3 (direct BP, 16384 x 16384 -> 4096^2, curved track with 5 cm motion,
+  15 km slant): split, fp32, mixed and dfloat against oracle.backproject_points
+  at 4096 seeded pixels plus the target cuts.  The all-FP32 mode is expected
+  to fail the 1e-4 gate at 15 km (the precision study's finding).
+* C4 (autofocus, 4096 x 4096 -> 1024^2): the HIP adjoint against the oracle
+  adjoint on 64 random pulses for the same image gradient; the autograd
+  entropy gradient against the oracle's FP64 gradient (FP64 image, entropy,
+  adjoint) and against central differences (h = 1e-4) of the FP64-mode GPU
+  entropy on a few pulses.
+* C5 (64 uneven patches): batched graph equals per-patch launches bit for
+  bit; the smallest and largest patches against the oracle FFBP + evaluation.
+
+Tolerances: split/mixed/dfloat NRMSE <= 1e-4 and peak phase <= 0.01 rad (the
+target bar); the FP64 GPU mode stores node images as complex64 (the
+reference's rounding points, fast_bp/ffbp.py:45,60,81), so it is held to
+1e-5 against the complex128 oracle; PSLR/ISLR deltas <= 0.01 dB.
+"""
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from bp_project_amd.metrics import cut_deltas, target_peaks
+
+pytestmark = pytest.mark.gpu
+
+HALF_CUT = 32           # pixels either side of a peak (8 m at 0.25 m)
+
+
+def seeded_blocks(grid, seed, blocks=16, size=16):
+    """Flat pixel indices of ``blocks`` seeded size x size blocks (4096 px)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(blocks):
+        iy = int(rng.integers(0, grid.ny - size))
+        ix = int(rng.integers(0, grid.nx - size))
+        yy, xx = np.meshgrid(np.arange(iy, iy + size), np.arange(ix, ix + size), indexing="ij")
+        out.append((yy * grid.nx + xx).ravel())
+    return np.concatenate(out)
+
+
+def cuts(grid, peaks, half=HALF_CUT):
+    """Per peak (row, col): (range-cut flat indices, azimuth-cut flat indices,
+    peak position in each cut)."""
+    out = []
+    for i, j in peaks:
+        cols = np.arange(max(0, j - half), min(grid.nx, j + half + 1))
+        rows = np.arange(max(0, i - half), min(grid.ny, i + half + 1))
+        out.append((i * grid.nx + cols, rows * grid.nx + j, j - cols[0], i - rows[0], cols, rows))
+    return out
+
+
+def compare(name, grid, image, lookup, sample, peak_cuts, nrmse_max, phase_max, pis_max):
+    """NRMSE over the sampled pixels, phase at each peak and PSLR/ISLR deltas
+    of every cut, ``image`` (GPU [ny, nx]) against the oracle values
+    (``lookup``: flat index -> complex128)."""
+    flat = image.ravel()
+    sel = np.unique(np.concatenate([sample] + [np.r_[a, b] for a, b, *_ in peak_cuts]))
+    ref = lookup(sel)
+    nrmse = float(np.linalg.norm(flat[sel].astype(np.complex128) - ref) / np.linalg.norm(ref))
+    phases, deltas, measured = [], [], 0
+    for rc, ac, pr, pa, cols, rows in peak_cuts:
+        p = rc[pr]
+        phases.append(abs(float(np.angle(flat[p] * np.conj(lookup(np.array([p]))[0])))))
+        for axis, idx, at in ((grid.x[cols], rc, pr), (grid.y[rows], ac, pa)):
+            d = cut_deltas(axis, lookup(idx), flat[idx], at, half_width_m=HALF_CUT * abs(grid.dx))
+            if d is not None:
+                measured += 1
+                deltas += [abs(d["pslr_delta_db"]), abs(d["islr_delta_db"])]
+    worst = max(deltas) if deltas else None
+    print(f"{name}: {len(sel)} pixels, NRMSE {nrmse:.3e}, max peak phase {max(phases):.3e} rad, "
+          f"max |PSLR/ISLR delta| {worst} dB over {measured} cuts", flush=True)
+    assert nrmse <= nrmse_max, (name, nrmse)
+    assert max(phases) <= phase_max, (name, max(phases))
+    assert measured >= len(peak_cuts), (name, measured)
+    assert worst <= pis_max, (name, worst)
+    return nrmse
+
+
+def table_lookup(indices, values):
+    order = np.argsort(indices)
+    keys, vals = np.asarray(indices)[order], np.asarray(values)[order]
+
+    def lookup(idx):
+        pos = np.searchsorted(keys, idx)
+        assert np.array_equal(keys[pos], idx)
+        return vals[pos]
+    return lookup
+
+
+# ---------------------------------------------------------------------------
+# C2: full-size FFBP
+# ---------------------------------------------------------------------------
+
+def test_c2_ffbp_full_vs_oracle(oracle):
+    import bench
+    from bp_project_amd.device import synchronize
+    from bp_project_amd.scenes import polar_extent_for_grid, workload_scene
+    t0 = time.perf_counter()
+    s = workload_scene("c2_ffbp_full")
+    g = s.grid
+    args = SimpleNamespace(leaf=16, range_spacing=0.25, angular_oversample=2.5,
+                           merge_interp="kaiser8", persistent=False)
+    plan, backend, dev, make = bench.ffbp_workload(args, s)
+    prog, cart, step, _ = make("split", "grid")
+    step()
+    synchronize()
+    split = cart.get()
+    step()
+    synchronize()
+    assert np.array_equal(cart.get(), split), "two consecutive steps differ"
+    prog.close()
+    prog, cart, step, _ = make("fp64")
+    step()
+    synchronize()
+    fp64 = cart.get()
+    prog.close()
+    print(f"C2 GPU images (split culled, fp64) in {time.perf_counter() - t0:.1f} s", flush=True)
+    peaks = target_peaks(np.abs(fp64), g, s.targets)
+    assert len(peaks) == len(s.targets) == 200
+    peak_cuts = cuts(g, peaks)
+    sample = seeded_blocks(g, 20260918)
+    sel = np.unique(np.concatenate([sample] + [np.r_[a, b] for a, b, *_ in peak_cuts]))
+    pts = g.points()[sel]
+    t0 = time.perf_counter()
+    extent = polar_extent_for_grid(g, s.pos.mean(axis=0), margin=2.0)
+    vals, info = oracle.ffbp_at_points(s.data, s.pos, s.r_start, s.data_dr, s.fc, extent, pts,
+                                       side=1, leaf_size=16, range_spacing=0.25,
+                                       angular_oversample=2.5, weight=s.weight, phase=s.phase,
+                                       policy="fp64", leaf_interp="lagrange4",
+                                       merge_interp="kaiser8")
+    print(f"C2 oracle at {len(sel)} pixels: {time.perf_counter() - t0:.1f} s, "
+          f"{info['computed_fraction']:.3f} of the node pixels", flush=True)
+    assert info["crop_shape"] == list(plan.shape)
+    lookup = table_lookup(sel, vals)
+    compare("C2 split", g, split, lookup, sample, peak_cuts, 1e-4, 0.01, 0.01)
+    compare("C2 fp64", g, fp64, lookup, sample, peak_cuts, 1e-5, 1e-3, 0.01)
+
+
+# ---------------------------------------------------------------------------
+# C3: full-size direct BP, four precision modes
+# ---------------------------------------------------------------------------
+
+def test_c3_precision_modes_vs_oracle(oracle):
+    import ctypes
+    from bp_project_amd import _lib
+    from bp_project_amd.device import DeviceArray, synchronize
+    from bp_project_amd.scenes import workload_scene
+    vp = _lib.c_void_p
+    s = workload_scene("c3_direct_precision")
+    g = s.grid
+    d = DeviceArray.from_host(s.data)
+    p = DeviceArray.from_host(s.pos)
+    n = ctypes.c_size_t()
+    _lib.call("bpmi_bp_grid_workspace_bytes", g.nx, g.ny, s.pulses, ctypes.byref(n))
+    assert n.value == 0           # 16384 tiles: no pulse split
+    images = {}
+    for mode in ("split", "mixed", "dfloat", "fp32"):
+        out = DeviceArray.empty(g.shape)
+        t0 = time.perf_counter()
+        _lib.call("bpmi_bp_grid", vp(d.ptr), s.bins, vp(p.ptr), None, None, 0, s.pulses, g.x0,
+                  g.dx, g.nx, g.y0, g.dy, g.ny, 0.0, s.r_start, s.data_dr, s.k,
+                  _lib.mode_code(mode), _lib.interp_code("lagrange4"), 0, vp(out.ptr), None)
+        synchronize()
+        images[mode] = out.get()
+        print(f"C3 {mode}: {time.perf_counter() - t0:.2f} s", flush=True)
+    peaks = target_peaks(np.abs(images["split"]), g, s.targets)
+    assert len(peaks) == 200
+    peak_cuts = cuts(g, peaks)
+    sample = np.random.default_rng(20260919).choice(g.nx * g.ny, 4096, replace=False)
+    sel = np.unique(np.concatenate([sample] + [np.r_[a, b] for a, b, *_ in peak_cuts]))
+    t0 = time.perf_counter()
+    vals = oracle.backproject_points(s.data, s.pos, g.points()[sel], s.r_start, s.data_dr, s.k,
+                                     s.weight, s.phase, interp="lagrange4")
+    print(f"C3 oracle at {len(sel)} pixels: {time.perf_counter() - t0:.1f} s", flush=True)
+    lookup = table_lookup(sel, vals)
+    # every peak is a local maximum of the oracle's own cuts
+    for rc, ac, pr, pa, *_ in peak_cuts:
+        for idx, at in ((rc, pr), (ac, pa)):
+            a = np.abs(lookup(idx))
+            assert a[at] >= a[max(0, at - 1)] and a[at] >= a[min(len(a) - 1, at + 1)]
+    for mode in ("split", "mixed", "dfloat"):
+        compare(f"C3 {mode}", g, images[mode], lookup, sample, peak_cuts, 1e-4, 0.01, 0.01)
+    flat = images["fp32"].ravel()[sel].astype(np.complex128)
+    e32 = float(np.linalg.norm(flat - vals) / np.linalg.norm(vals))
+    print(f"C3 fp32: NRMSE {e32:.3e} (all-FP32 range and phase at 15 km)", flush=True)
+    assert e32 > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# C4: full-size adjoint and entropy gradient
+# ---------------------------------------------------------------------------
+
+def test_c4_adjoint_and_gradient_vs_oracle(oracle):
+    import torch
+    from bp_project_amd.autofocus import (GridOperator, backproject, entropy,
+                                          finite_difference_gradient)
+    from bp_project_amd.scenes import workload_scene
+    s = workload_scene("c4_autofocus")
+    g = s.grid
+    op = GridOperator.from_scene(s, mode="split", interp="lagrange4")
+    t0 = time.perf_counter()
+    img64 = oracle.backproject_points(s.data, s.pos, g.points(), s.r_start, s.data_dr, s.k,
+                                      s.weight, s.phase, interp="lagrange4")
+    E64, grad_img = oracle.entropy_and_grad(img64)
+    print(f"C4 oracle FP64 image + entropy {E64:.6f}: {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    sel = np.sort(np.random.default_rng(20260920).choice(s.pulses, 64, replace=False))
+    ref_phi, ref_pos = oracle.backproject_points_adjoint(
+        s.data[sel], s.pos[sel], g.points(), s.r_start, s.data_dr, s.k, s.weight[sel],
+        s.phase[sel], grad_img, interp="lagrange4")
+    phase = torch.from_numpy(s.phase.copy()).cuda()
+    pos = torch.from_numpy(s.pos.copy()).cuda()
+    # the adjoint alone, for the oracle's image gradient
+    gt = torch.from_numpy(grad_img.astype(np.complex64).reshape(g.shape)).cuda()
+    gphi, gpos = op.adjoint(phase, pos, gt)
+    gphi, gpos = gphi.cpu().numpy(), gpos.cpu().numpy()
+    e_phi = np.linalg.norm(gphi[sel] - ref_phi) / np.linalg.norm(ref_phi)
+    e_pos = np.linalg.norm(gpos[sel] - ref_pos) / np.linalg.norm(ref_pos)
+    print(f"C4 adjoint vs oracle adjoint (64 pulses): phase {e_phi:.2e}, position {e_pos:.2e}",
+          flush=True)
+    assert e_phi <= 2e-3 and e_pos <= 2e-3
+    # the whole gradient: GPU image, entropy and adjoint through autograd
+    ph = phase.clone().requires_grad_(True)
+    ps = pos.clone().requires_grad_(True)
+    E = entropy(backproject(ph, ps, op))
+    E.backward()
+    aphi, apos = ph.grad.cpu().numpy(), ps.grad.cpu().numpy()
+    print(f"C4 entropy: GPU {float(E.detach()):.6f}, oracle {E64:.6f}", flush=True)
+    assert abs(float(E.detach()) - E64) <= 1e-5 * abs(E64)
+    e_phi = np.linalg.norm(aphi[sel] - ref_phi) / np.linalg.norm(ref_phi)
+    e_pos = np.linalg.norm(apos[sel] - ref_pos) / np.linalg.norm(ref_pos)
+    print(f"C4 autograd vs FP64 oracle gradient (64 pulses): phase {e_phi:.2e}, "
+          f"position {e_pos:.2e}", flush=True)
+    assert e_phi <= 2e-3 and e_pos <= 2e-3
+    # central differences of the FP64-mode GPU entropy, h = 1e-4
+    few = sel[:4]
+    fphi, fpos = finite_difference_gradient(op, s.phase, s.pos, few, h=1e-4, mode="fp64")
+    scale_phi = np.abs(aphi).max()
+    scale_pos = np.abs(apos).max(axis=0)
+    print("C4 FD phase", fphi, "autograd", aphi[few], flush=True)
+    assert np.all(np.abs(aphi[few] - fphi) <= 5e-3 * scale_phi)
+    assert np.all(np.abs(apos[few] - fpos) <= 2e-2 * scale_pos[None, :] + 1e-6)
+
+
+# ---------------------------------------------------------------------------
+# C5: 64 uneven patches
+# ---------------------------------------------------------------------------
+
+def test_c5_multipatch_full_vs_sequential_and_oracle(oracle):
+    from bp_project_amd.device import synchronize
+    from bp_project_amd.multipatch import (MultiPatchProgram, make_patches, patch_mix,
+                                           sequential_reference)
+    from bp_project_amd.scenes import polar_extent_for_grid
+    t0 = time.perf_counter()
+    scenes = make_patches(patch_mix(count=64))
+    print(f"C5: 64 patches in {time.perf_counter() - t0:.1f} s", flush=True)
+    prog = MultiPatchProgram(scenes, mode="split", merge_interp="kaiser8", cull=True)
+    assert prog.capture_graph()
+    prog.replay()
+    synchronize()
+    batched = [o.get() for o in prog.outputs]
+    prog.replay()
+    synchronize()
+    assert all(np.array_equal(a, o.get()) for a, o in zip(batched, prog.outputs))
+    seq, _, _, progs = sequential_reference(scenes, mode="split", merge_interp="kaiser8",
+                                            tile_cols_log2=prog.tile_cols_log2, cull=True)
+    assert all(np.array_equal(a, b.get()) for a, b in zip(batched, seq))
+    for _, p_, _, _ in progs:
+        p_.close()
+    prog.close()
+    work = [s.grid.nx * s.grid.ny * s.pulses for s in scenes]
+    for which in (int(np.argmin(work)), int(np.argmax(work))):
+        s = scenes[which]
+        t0 = time.perf_counter()
+        extent = polar_extent_for_grid(s.grid, s.pos.mean(axis=0), margin=2.0)
+        res, _ = oracle.form_ffbp(s.data, s.pos, s.r_start, s.data_dr, s.fc, extent, side=1,
+                                  leaf_size=16, range_spacing=0.5, angular_oversample=2.5,
+                                  weight=s.weight, phase=s.phase, policy="fp64",
+                                  leaf_interp="lagrange4", merge_interp="kaiser8")
+        ref = oracle.evaluate_image(res.image, res.origin, res.r0, res.dr, res.u0, res.du,
+                                    s.grid.points(), s.k, interp="kaiser8").reshape(s.grid.shape)
+        got = batched[which].astype(np.complex128)
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        peaks = target_peaks(np.abs(ref), s.grid, s.targets)
+        ph = max(abs(float(np.angle(got[i, j] * np.conj(ref[i, j])))) for i, j in peaks)
+        print(f"C5 patch {which} ({s.pulses} pulses, {s.grid.nx}^2): NRMSE {err:.3e}, "
+              f"max peak phase {ph:.3e} rad, oracle {time.perf_counter() - t0:.1f} s", flush=True)
+        assert err <= 1e-4 and ph <= 0.01
