@@ -38,9 +38,12 @@ _c = _lib
 _vp = _lib.c_void_p
 
 
-def _check_mode(mode, allowed=ALL_MODES):
-    if mode not in allowed:
-        raise ValueError(f"mode must be one of {allowed}")
+def check_mode(mode):
+    """The reference's mode error (fast_bp/gpu_backend.py:442-443) over the
+    accepted tuple: MODES plus this build's dfloat and split."""
+    if mode not in ALL_MODES:
+        raise ValueError(f"mode must be one of {ALL_MODES}")
+
 
 
 @dataclass
@@ -251,7 +254,7 @@ class HipBaseline:
 
     def run_ffbp(self, plan, device, mode, range_start, data_dr, k):
         """Eager depth-first FFBP (fast_bp/gpu_backend.py:441-469)."""
-        _check_mode(mode)
+        check_mode(mode)
         wall_start = time.perf_counter()
         start, end = Event(), Event()
         start.record()
@@ -269,7 +272,7 @@ class HipBaseline:
     def run_direct(self, plan, device, mode, range_start, data_dr, k, chunk_pulses=64):
         """Direct BP on the cropped root grid, accumulated over pulse chunks
         (fast_bp/gpu_backend.py:471-555)."""
-        _check_mode(mode)
+        check_mode(mode)
         if chunk_pulses <= 0:
             raise ValueError("chunk_pulses must be positive")
         count = plan.nr * plan.nu

@@ -1,180 +1,235 @@
-"""CPU structural audit of FFBP regrid support (fast_bp/resident_plan_audit.py),
-for the 4-point Lagrange stencil of the reference and the 8-tap Kaiser-sinc
-stencil of this implementation.
+"""Support of an FFBP plan: which node pixels the output depends on, and
+whether every one of them finds its whole interpolation stencil inside the
+child image it reads.
 
-``audit_support`` propagates, edge by edge from the root crop down to the
-leaves, every source sample a required destination pixel's stencil may read
-(FP64 polar geometry of fast_bp/ffbp.py:79-95), and counts the required
-stencils that leave their source image -- those read zeros (the validity rule
-of ``sample2``, ffbp.py:35), so a failed audit means the plan's guard cells
-are too narrow for that interpolator.  Zero-weight stencil samples are
-included (conservative), as in the reference.
-``extend_plan_radial_guard`` pads every node radially (resident_plan_audit.py:
-14-57), keeping all sample coordinates and the output grid.
+``walk_support`` is the one engine.  From a mask of required root pixels it
+visits the tree edges (parent, child) depth first -- an edge, then the whole
+subtree under that child -- and maps the parent's required pixels into the
+child with the regrid geometry (fast_bp/ffbp.py:79-95).  Each pixel's
+stencil (4x4 Lagrange, fast_bp/ffbp.py:32-40, or this build's 8x8
+Kaiser-sinc) either lies inside the child -- then its cells become required
+child pixels -- or it does not, and the reference's sample2 reads zeros there
+(ffbp.py:35).  At pixel granularity with only inside stencils propagated the
+walk gives the exact necessary sets; at the tile granularity of the culled
+schedules, with boxes widened, it gives the conservative tile masks that the
+device marking must contain (tests/culling_reference.py).
 
-No device code: the audit is a property of the FP64 plan.
+``audit_support`` (the counterpart of fast_bp/resident_plan_audit.py:60-138;
+its report keys are kept for callers that read them) runs the walk from the
+root crop and reports, per edge, the required pixels whose stencils fall
+outside the child: a plan that passes has guard cells wide enough for that
+interpolator.  ``extend_plan_radial_guard`` widens every node radially
+(resident_plan_audit.py:15-61), ``minimum_guard`` searches the smallest
+widening that passes.
+
+Host numpy code: the audit is a property of the FP64 plan.
 """
 from __future__ import annotations
 
 import copy
+from dataclasses import dataclass
 
 import numpy as np
 
 from .plan import iter_nodes
 
-# stencil offsets and validity margin of each interpolator: taps at
-# floor(q) + FIRST .. floor(q) + FIRST + TAPS - 1, valid when every tap is inside
+# interpolator -> (offset of the first tap from floor(q), taps per axis)
 STENCILS = {"lagrange4": (-1, 4), "kaiser8": (-3, 8)}
 
 
-def extend_plan_radial_guard(plan, extra_cells):
-    """Deep copy of ``plan`` with ``extra_cells`` more radial cells on both
-    sides of every node; node sample coordinates and the output grid are kept
-    and the root crop moves by ``extra_cells``."""
-    if isinstance(extra_cells, bool) or int(extra_cells) != extra_cells or extra_cells < 0:
-        raise ValueError("extra_cells must be a nonnegative integer")
-    extra = int(extra_cells)
-    result = copy.deepcopy(plan)
-    counts = {"leaves": 0, "merges": 0, "leaf_pixel_pulse_products": 0, "merged_pixels": 0,
-              "nodes": 0}
-    for node in iter_nodes(result.root):
-        node.r0 -= extra * node.dr
-        node.nr += 2 * extra
-        counts["nodes"] += 1
-        if node.is_leaf:
-            counts["leaves"] += 1
-            counts["leaf_pixel_pulse_products"] += node.nr * node.nu * (node.b - node.a)
-        else:
-            counts["merges"] += 1
-            counts["merged_pixels"] += 2 * node.nr * node.nu
-    result.crop_r = tuple(int(v + extra) for v in plan.crop_r)
-    counts["shape"] = [int(result.nr), int(result.nu)]
-    counts["direct_pixel_pulse_products"] = int(result.nr * result.nu *
-                                                (result.root.b - result.root.a))
-    result.counts = dict(plan.counts) | counts
-    previous = getattr(plan, "guard_extension", {}).get("cumulative_extra_cells_per_side", 0)
-    result.guard_extension = {
-        "extra_cells_per_side": extra,
-        "cumulative_extra_cells_per_side": int(previous + extra),
-        "changed_axes": "r only; every node padded on both sides",
-        "output_grid_unchanged": True,
-        "root_crop_r_before": list(plan.crop_r),
-        "root_crop_r_after": list(result.crop_r),
-        "root_shape_before": [int(plan.root.nr), int(plan.root.nu)],
-        "root_shape_after": [int(result.root.nr), int(result.root.nu)],
-    }
-    for name in ("r0", "dr", "nr", "u0", "du", "nu", "side", "crop_u"):
-        if getattr(result, name) != getattr(plan, name):
-            raise AssertionError(name)
-    return result
-
-
 def source_coordinates(destination, source, flat_index):
-    """Fractional source-grid coordinates (qr, qu) of destination pixels and
-    whether they are defined (off the source origin): ffbp.py:79-95."""
-    ir, iu = np.divmod(np.asarray(flat_index, np.int64), destination.nu)
-    r = destination.r0 + ir * destination.dr
-    u = destination.u0 + iu * destination.du
-    gr = np.sqrt(np.maximum(0., r * r - destination.origin[2] ** 2))
-    x = destination.origin[0] + destination.side * gr * np.sqrt(np.maximum(0., 1. - u * u))
-    y = destination.origin[1] + gr * u
-    dx = x - source.origin[0]
-    dy = y - source.origin[1]
-    g = np.sqrt(dx * dx + dy * dy)
-    rr = np.sqrt(g * g + source.origin[2] ** 2)
+    """(qr, qu, defined): fractional row and column in ``source`` that the
+    regrid samples for destination pixels ``flat_index`` (row-major), and
+    whether the point is off the source origin (a zero ground distance has
+    no angle).  Geometry of fast_bp/ffbp.py:79-95 in FP64."""
+    row, col = np.divmod(np.asarray(flat_index, np.int64), destination.nu)
+    o, so = destination.origin, source.origin
+    r = destination.r0 + row * destination.dr
+    u = destination.u0 + col * destination.du
+    ground = np.sqrt(np.maximum(0.0, r * r - o[2] * o[2]))
+    ex = o[0] + destination.side * ground * np.sqrt(np.maximum(0.0, 1.0 - u * u)) - so[0]
+    ey = o[1] + ground * u - so[1]
+    g = np.sqrt(ex * ex + ey * ey)
     with np.errstate(divide="ignore", invalid="ignore"):
-        qr = (rr - source.r0) / source.dr
-        qu = (dy / g - source.u0) / source.du
-    defined = np.isfinite(qr) & np.isfinite(qu) & (g != 0.)
-    return qr, qu, defined
+        qr = (np.sqrt(g * g + so[2] * so[2]) - source.r0) / source.dr
+        qu = (ey / g - source.u0) / source.du
+    return qr, qu, (g != 0.0) & np.isfinite(qr) & np.isfinite(qu)
 
 
-def audit_support(plan, interp="lagrange4", batch_size=65536, sites=None):
-    """Propagate required stencil support from the root crop to the leaves.
-    Returns the reference's report (resident_plan_audit.py:60-138) with the
-    interpolator named.  ``sites``: optional dict that receives every node's
-    required-site mask (bool [nr, nu]) by node index."""
+@dataclass
+class EdgeSupport:
+    parent: object          # plan node
+    child: object
+    required: int           # required parent pixels
+    outside: int            # of those, stencils not wholly inside the child (or undefined)
+    margin: int | None      # least distance (cells) from a defined stencil to the child edge
+    first_outside: dict | None
+    child_mask: np.ndarray  # bool [ceil(nr / tile), ceil(nu / tile)] of the child
+
+
+def _pixels(mask, nr, nu, tile):
+    if tile == 1:
+        return np.flatnonzero(mask.ravel())
+    full = np.repeat(np.repeat(mask, tile, 0), tile, 1)[:nr, :nu]
+    return np.flatnonzero(full.ravel())
+
+
+def _edge(parent, child, mask, stencil, tile, widen, inside_only, batch):
+    first, taps = stencil
+    nr, nu = child.nr, child.nu
+    out = np.zeros((-(-nr // tile), -(-nu // tile)), bool)
+    pix = _pixels(mask, parent.nr, parent.nu, tile)
+    outside, margin, example = 0, None, None
+    span = (taps + 2 * widen + tile - 2) // tile + 1      # tiles a box can cross per axis
+    for s in range(0, len(pix), batch):
+        p = pix[s:s + batch]
+        qr, qu, defined = source_coordinates(parent, child, p)
+        lo_r = np.floor(np.where(defined, qr, -1.0)).astype(np.int64) + first
+        lo_u = np.floor(np.where(defined, qu, -1.0)).astype(np.int64) + first
+        room = np.minimum(np.minimum(lo_r, lo_u),
+                          np.minimum(nr - taps - lo_r, nu - taps - lo_u))
+        inside = defined & (room >= 0)
+        outside += int(np.count_nonzero(~inside))
+        if room.size:
+            m = int(room.min())
+            margin = m if margin is None else min(margin, m)
+        if example is None and not inside.all():
+            k = int(np.flatnonzero(~inside)[0])
+            example = {"destination_flat_index": int(p[k]), "qr": float(qr[k]),
+                       "qu": float(qu[k])}
+        keep = inside if inside_only else defined
+        a0, b0 = lo_r[keep] - widen, lo_u[keep] - widen
+        a1, b1 = a0 + taps - 1 + 2 * widen, b0 + taps - 1 + 2 * widen
+        hit = (a1 >= 0) & (a0 < nr) & (b1 >= 0) & (b0 < nu)
+        t0r, t1r = np.clip(a0[hit], 0, nr - 1) // tile, np.clip(a1[hit], 0, nr - 1) // tile
+        t0u, t1u = np.clip(b0[hit], 0, nu - 1) // tile, np.clip(b1[hit], 0, nu - 1) // tile
+        for i in range(span):
+            for j in range(span):
+                out[np.minimum(t0r + i, t1r), np.minimum(t0u + j, t1u)] = True
+    return EdgeSupport(parent, child, len(pix), outside, margin, example, out)
+
+
+def walk_support(plan, interp, root_mask, *, tile=1, widen=0, inside_only=True,
+                 batch=1 << 16):
+    """Yield an ``EdgeSupport`` per tree edge, depth first (an edge, then the
+    subtree below its child), starting from ``root_mask`` (bool, per pixel
+    when ``tile`` is 1, else per tile x tile block of the root).  A child's
+    mask marks the cells (``tile`` 1) or tiles that the stencils of its
+    parent's required pixels cover, each stencil box widened by ``widen``
+    cells and clipped to the child; ``inside_only`` propagates only stencils
+    wholly inside the child (those that read anything)."""
     if interp not in STENCILS:
         raise ValueError(f"interp must be one of {tuple(STENCILS)}")
-    first, taps = STENCILS[interp]
-    last = first + taps - 1
-    records = []
-    root_required = np.zeros((plan.root.nr, plan.root.nu), dtype=bool)
-    root_required[slice(*plan.crop_r), slice(*plan.crop_u)] = True
+    stencil = STENCILS[interp]
 
-    def propagate(destination, source, required):
-        indexes = np.flatnonzero(required.ravel())
-        child_required = np.zeros(source.nr * source.nu, dtype=bool)
-        invalid_count = 0
-        minimum_margin = np.inf
-        invalid_example = None
-        for begin in range(0, len(indexes), batch_size):
-            p = indexes[begin:begin + batch_size]
-            qr, qu, defined = source_coordinates(destination, source, p)
-            sr = np.floor(np.where(defined, qr, -1.)).astype(np.int64)
-            su = np.floor(np.where(defined, qu, -1.)).astype(np.int64)
-            margin = np.minimum.reduce((sr + first, su + first, source.nr - 1 - last - sr,
-                                        source.nu - 1 - last - su))
-            valid = defined & (margin >= 0)
-            invalid_count += int(np.count_nonzero(~valid))
-            if len(margin):
-                minimum_margin = min(minimum_margin, int(margin.min()))
-            if invalid_example is None and np.any(~valid):
-                k = int(np.flatnonzero(~valid)[0])
-                invalid_example = {"destination_flat_index": int(p[k]), "qr": float(qr[k]),
-                                   "qu": float(qu[k])}
-            sr, su = sr[valid], su[valid]
-            for a in range(first, last + 1):
-                for b in range(first, last + 1):
-                    child_required[(sr + a) * source.nu + su + b] = True
-        records.append({
-            "destination_index": int(destination.index),
-            "source_index": int(source.index),
-            "required_destination_sites": int(len(indexes)),
-            "invalid_required_stencils": invalid_count,
-            "minimum_integer_support_margin_cells": (None if not np.isfinite(minimum_margin)
-                                                     else int(minimum_margin)),
-            "required_source_sites": int(np.count_nonzero(child_required)),
-            "invalid_example": invalid_example,
-        })
-        return child_required.reshape(source.nr, source.nu)
-
-    def walk(node, required):
-        # edges in the reference's order: an edge, then the whole subtree below it
-        if sites is not None:
-            sites[node.index] = required
+    def visit(node, mask):
         if node.is_leaf:
             return
         for child in (node.left, node.right):
-            walk(child, propagate(node, child, required))
+            e = _edge(node, child, mask, stencil, tile, widen, inside_only, batch)
+            yield e
+            yield from visit(child, e.child_mask)
 
-    walk(plan.root, root_required)
-    failures = [row for row in records if row["invalid_required_stencils"]]
+    yield from visit(plan.root, np.asarray(root_mask, bool))
+
+
+def crop_mask(plan):
+    """Pixel mask of the root crop (the output region of form_ffbp)."""
+    m = np.zeros((plan.root.nr, plan.root.nu), bool)
+    m[slice(*plan.crop_r), slice(*plan.crop_u)] = True
+    return m
+
+
+def audit_support(plan, interp="lagrange4", batch_size=65536, sites=None):
+    """Per-edge support audit of ``plan`` from the root crop down.  ``sites``
+    (a dict) receives each node's required-pixel mask by node index.
+    Report keys follow fast_bp/resident_plan_audit.py:60-138."""
+    taps = STENCILS.get(interp, (0, 0))[1]
+    root = crop_mask(plan)
+    if sites is not None:
+        sites[plan.root.index] = root
+    records = []
+    for e in walk_support(plan, interp, root, batch=batch_size):
+        if sites is not None:
+            sites[e.child.index] = e.child_mask
+        records.append({
+            "destination_index": int(e.parent.index),
+            "source_index": int(e.child.index),
+            "required_destination_sites": e.required,
+            "invalid_required_stencils": e.outside,
+            "minimum_integer_support_margin_cells": e.margin,
+            "required_source_sites": int(np.count_nonzero(e.child_mask)),
+            "invalid_example": e.first_outside,
+        })
+    bad = sum(1 for r in records if r["invalid_required_stencils"])
     return {
-        "method": f"CPU FP64 geometry; conservative {taps}x{taps} ({interp}) support "
-                  "propagation from final crop",
+        "method": f"host FP64 plan geometry; {taps}x{taps} {interp} stencils of every "
+                  "required pixel, propagated from the root crop to the leaves",
         "interp": interp,
-        "passed": not failures,
+        "passed": bad == 0,
         "checked_edges": len(records),
-        "invalid_required_stencils": int(sum(row["invalid_required_stencils"] for row in records)),
-        "edges_with_invalid_required_stencils": len(failures),
+        "invalid_required_stencils": int(sum(r["invalid_required_stencils"] for r in records)),
+        "edges_with_invalid_required_stencils": bad,
         "records": records,
         "limitations": [
-            "Audits regrid support only; not leaf raw range support or interpolation accuracy",
-            "FP64 index decisions; the GPU's FP64 geometry rounds the same way except within "
-            "an ulp of an integer boundary",
-            "Conservative inclusion of zero-weight stencil samples can overstate downstream "
-            "requirements",
-            "Invalid stencil counts are per tree edge, not unique final output pixels",
+            "merge and evaluation stencils only: the leaves' range-sample support and the "
+            "interpolation error are not audited",
+            "floor() of FP64 coordinates decides each stencil; the device computes the same "
+            "coordinates and can differ only within rounding of an integer",
+            "every stencil cell counts as required, including cells whose weight is zero",
+            "outside stencils are counted once per edge, not per output pixel",
         ],
     }
 
 
+def extend_plan_radial_guard(plan, extra_cells):
+    """Copy of ``plan`` whose nodes have ``extra_cells`` more range cells on
+    each side: every node's r0 moves down by that many steps and nr grows by
+    twice as many; angles, the output grid and every pixel's position are
+    unchanged, and the root crop moves up by ``extra_cells`` rows."""
+    if isinstance(extra_cells, bool) or int(extra_cells) != extra_cells or extra_cells < 0:
+        raise ValueError("extra_cells must be a nonnegative integer")
+    n = int(extra_cells)
+    out = copy.deepcopy(plan)
+    leaves = merges = leaf_pp = merged = nodes = 0
+    for node in iter_nodes(out.root):
+        node.r0 -= n * node.dr
+        node.nr += 2 * n
+        nodes += 1
+        if node.is_leaf:
+            leaves += 1
+            leaf_pp += node.nr * node.nu * (node.b - node.a)
+        else:
+            merges += 1
+            merged += 2 * node.nr * node.nu
+    out.crop_r = (int(plan.crop_r[0] + n), int(plan.crop_r[1] + n))
+    for name in ("r0", "dr", "nr", "u0", "du", "nu", "side", "crop_u"):
+        if getattr(out, name) != getattr(plan, name):
+            raise AssertionError(f"output grid changed: {name}")
+    before = getattr(plan, "guard_extension", {}).get("cumulative_extra_cells_per_side", 0)
+    out.counts = dict(plan.counts, leaves=leaves, merges=merges, nodes=nodes,
+                      leaf_pixel_pulse_products=leaf_pp, merged_pixels=merged,
+                      shape=[int(out.nr), int(out.nu)],
+                      direct_pixel_pulse_products=int(out.nr * out.nu
+                                                      * (out.root.b - out.root.a)))
+    out.guard_extension = {
+        "extra_cells_per_side": n,
+        "cumulative_extra_cells_per_side": int(before + n),
+        "changed_axes": "range only, both ends of every node",
+        "output_grid_unchanged": True,
+        "root_crop_r_before": list(plan.crop_r),
+        "root_crop_r_after": list(out.crop_r),
+        "root_shape_before": [int(plan.root.nr), int(plan.root.nu)],
+        "root_shape_after": [int(out.root.nr), int(out.root.nu)],
+    }
+    return out
+
+
 def minimum_guard(plan, interp="kaiser8", limit=32, step=2):
-    """Smallest extra radial guard (cells per side, multiple of ``step``) whose
-    extended plan passes the audit, or None up to ``limit``."""
+    """Smallest radial widening (cells per side, a multiple of ``step``, at
+    most ``limit``) whose plan passes ``audit_support``, or None."""
     for extra in range(0, limit + 1, step):
-        candidate = extend_plan_radial_guard(plan, extra) if extra else plan
-        if audit_support(candidate, interp)["passed"]:
+        if audit_support(extend_plan_radial_guard(plan, extra) if extra else plan,
+                         interp)["passed"]:
             return extra
     return None

@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Graph, Stream, current_stream, stream_handle
-from .gpu_backend import ALL_MODES, MODES, geo_array, leaf_tile_width, origin_reference  # noqa: F401
+from .gpu_backend import ALL_MODES, MODES, check_mode, geo_array, leaf_tile_width, origin_reference  # noqa: F401
 
 _vp = _lib.c_void_p
 
@@ -50,8 +50,7 @@ class ResidentProgram:
 
     def __init__(self, backend, plan, device, mode, range_start, data_dr, k,
                  algorithm="ffbp", variant="original", chunk_pulses=64):
-        if mode not in ALL_MODES:
-            raise ValueError(f"mode must be one of {MODES}")
+        check_mode(mode)
         if algorithm not in ("direct", "ffbp"):
             raise ValueError("algorithm must be direct or ffbp")
         if variant not in ("original", "fused"):
@@ -107,7 +106,7 @@ class ResidentProgram:
             "workspace_buffer_count": len(self._buffers),
             "origin_table_bytes": int(np.asarray(plan.origins).nbytes),
             "total_owned_device_bytes": workspace_bytes,
-            "workspace_policy": "all schedule buffers preallocated; no liveness reuse",
+            "workspace_policy": "one device buffer per node image, all made at construction",
             "input_data_shared": True,
             "output_dtype": "complex64",
             "zeroing_in_execute": algorithm == "direct",
@@ -236,10 +235,10 @@ class ResidentProgram:
         return self.output
 
     def capture_graph(self, warmup=True):
-        """Capture ``execute`` into a HIP graph outside timed regions.
-
-        Returns False and records ``graph_error`` if capture fails.  This setup
-        method may synchronise; ``execute`` and ``replay`` never do.
+        """Record one ``execute`` as a HIP graph on a private stream (after an
+        untimed warm-up execution there).  True on success; on failure False,
+        with the exception text in ``metadata["graph_error"]``.  Set-up code:
+        it synchronises the device, which ``execute`` and ``replay`` never do.
         """
         if self._closed:
             raise RuntimeError("program is closed")
@@ -265,7 +264,8 @@ class ResidentProgram:
             return False
 
     def replay(self):
-        """Enqueue the captured graph on the current stream, without a fence."""
+        """Launch the recorded graph on the current stream and return the
+        output array (asynchronous; the caller orders and waits)."""
         if self._closed:
             raise RuntimeError("program is closed")
         if self._graph is None:
@@ -274,7 +274,8 @@ class ResidentProgram:
         return self.output
 
     def close(self):
-        """Drop owned buffers after the caller has synchronised its work."""
+        """Release the graph, stream and buffers; the caller first waits for
+        any work that still uses them."""
         if self._graph is not None:
             self._graph.close()
         self._graph = None

@@ -1,42 +1,22 @@
-"""Numpy restatement of required-tile marking (bp_project_amd/culling.py and
-the marking kernels), pixel by pixel, for small plans in tests.
+"""Host restatement of required-tile marking (bp_project_amd/culling.py and
+the marking kernels) for small plans in tests.
 
-``reference_masks`` propagates 32 x 32 tile masks from the root down: every
+``reference_masks`` propagates 32 x 32 tile masks from the root down with the
+plan audit's support walk (bp_project_amd.plan_audit.walk_support): every
 pixel of a marked parent tile whose stencil origin is defined marks the child
 tiles its stencil box touches.  ``ok_only`` counts only pixels whose stencil
 lies inside the child (the pixels that read anything: the exact necessary
 set); ``widen`` widens each box by that many cells (the device widens by 1).
+``grid_root_tiles`` restates the root tiles a Cartesian evaluation reads.
 """
 import numpy as np
 
 from bp_project_amd.culling import TILE, crop_tiles
-from bp_project_amd.plan import iter_nodes
-from bp_project_amd.plan_audit import STENCILS, source_coordinates
+from bp_project_amd.plan_audit import STENCILS, crop_mask, walk_support
 
 
 def _tiles_of(nr, nu, tile=TILE):
     return -(-nr // tile), -(-nu // tile)
-
-
-def _pixels_of(mask, nr, nu, tile=TILE):
-    pix = np.repeat(np.repeat(mask, tile, 0), tile, 1)[:nr, :nu]
-    return np.flatnonzero(pix.ravel())
-
-
-def _mark(mask, nr, nu, ir, iu, first, taps, widen, ok_only, tile=TILE):
-    lo_r, hi_r = ir + first - widen, ir + first + taps - 1 + widen
-    lo_u, hi_u = iu + first - widen, iu + first + taps - 1 + widen
-    if ok_only:
-        keep = (ir + first >= 0) & (ir + first + taps - 1 <= nr - 1) & \
-               (iu + first >= 0) & (iu + first + taps - 1 <= nu - 1)
-    else:
-        keep = (hi_r >= 0) & (lo_r <= nr - 1) & (hi_u >= 0) & (lo_u <= nu - 1)
-    lo_r, hi_r = np.clip(lo_r[keep], 0, nr - 1), np.clip(hi_r[keep], 0, nr - 1)
-    lo_u, hi_u = np.clip(lo_u[keep], 0, nu - 1), np.clip(hi_u[keep], 0, nu - 1)
-    for a0, a1, b0, b1 in {(int(a), int(b), int(c), int(d))
-                           for a, b, c, d in zip(lo_r // tile, hi_r // tile, lo_u // tile,
-                                                 hi_u // tile)}:
-        mask[a0:a1 + 1, b0:b1 + 1] = True
 
 
 def grid_root_tiles(plan, grid, interp="kaiser8", widen=1, ok_only=False):
@@ -73,22 +53,11 @@ def grid_root_tiles(plan, grid, interp="kaiser8", widen=1, ok_only=False):
 
 
 def reference_masks(plan, interp="kaiser8", widen=1, ok_only=False, root_mask=None, tile=TILE):
-    """{node index: bool tile mask} from the root crop (or ``root_mask``) down;
-    ``tile`` = 1 gives pixel masks (root: the crop pixels)."""
-    first, taps = STENCILS[interp]
-    if root_mask is None and tile == 1:
-        root_mask = np.zeros((plan.root.nr, plan.root.nu), bool)
-        root_mask[slice(*plan.crop_r), slice(*plan.crop_u)] = True
-    masks = {plan.root.index: crop_tiles(plan) if root_mask is None else root_mask}
-    for node in iter_nodes(plan.root):  # pre-order: parents before children
-        if node.is_leaf:
-            continue
-        pixels = _pixels_of(masks[node.index], node.nr, node.nu, tile)
-        for child in (node.left, node.right):
-            mask = np.zeros(_tiles_of(child.nr, child.nu, tile), bool)
-            qr, qu, defined = source_coordinates(node, child, pixels)
-            ir = np.floor(qr[defined]).astype(np.int64)
-            iu = np.floor(qu[defined]).astype(np.int64)
-            _mark(mask, child.nr, child.nu, ir, iu, first, taps, widen, ok_only, tile)
-            masks[child.index] = mask
+    """{node index: bool tile mask} from the root crop (or ``root_mask``) down,
+    by the plan audit's support walk; ``tile`` = 1 gives pixel masks."""
+    if root_mask is None:
+        root_mask = crop_mask(plan) if tile == 1 else crop_tiles(plan)
+    masks = {plan.root.index: root_mask}
+    for e in walk_support(plan, interp, root_mask, tile=tile, widen=widen, inside_only=ok_only):
+        masks[e.child.index] = e.child_mask
     return masks

@@ -1,6 +1,7 @@
 """Host side of required-tile culling: tile bookkeeping against brute force,
-and the numpy restatement of the marking (tests/culling_reference.py)
-against the plan audit's independent per-pixel support propagation."""
+and the marking restatement (tests/culling_reference.py, the plan audit's
+support walk at tile granularity) against the pixel-exact support sets of
+the same walk, which the plan-audit tests pin by brute force."""
 import numpy as np
 import pytest
 
@@ -72,8 +73,8 @@ def test_tile_selection_pixel_counts():
 @pytest.mark.parametrize("interp", ["kaiser8", "lagrange4"])
 def test_reference_marking_equals_audit_support(interp):
     """Pixel-level marking (tile 1, pixels that read, no widening) equals the
-    required sites that the plan audit propagates independently; the 32 x 32
-    tile marking contains them and still leaves tiles unmarked."""
+    audit's required sites; the 32 x 32 tile marking contains them and still
+    leaves tiles unmarked."""
     from culling_reference import reference_masks
     from bp_project_amd.plan_audit import audit_support
     _, plan = _plan()
@@ -92,7 +93,7 @@ def test_reference_marking_equals_audit_support(interp):
 
 @pytest.mark.parametrize("stencil", [(-2, 8), (-3, 7), (-4, 8)])
 def test_planted_stencil_defects_are_caught(monkeypatch, stencil):
-    """A restatement with a stencil one cell off disagrees with the audit."""
+    """A marking with a stencil one cell off disagrees with the audit's sites."""
     import culling_reference
     from bp_project_amd.plan_audit import audit_support
     _, plan = _plan()
