@@ -58,12 +58,16 @@ __device__ __forceinline__ float2 cis_turns64(double turns) {
     return make_float2(c, s);
 }
 
-// 1/sqrt(x) in FP64 from the hardware estimate plus two Newton steps
-// (x > 0, normal): relative error at the FP64 rounding level, no IEEE
-// square-root or division sequence.
 // 1/sqrt(x): hardware estimate (v_rsq_f64, relative error <= 5.2e-8 over
 // [1e-3, 1e9] on gfx950) and one Newton step, relative error <= 4.2e-15 --
-// 2.5e-11 m on a 6 km range, a 1.6e-9-turn phase error at X band.
+// 2.5e-11 m on a 6 km range, a 1.6e-9-turn phase error at X band -- for
+// x > 0 and normal.  Contract at x = 0: the estimate is +inf and the Newton
+// step forms 0 * inf, so the result is NaN (not inf), and so is x * rsqrt64(x).
+// Callers rely on that NaN to reject a point on an origin without a
+// separate test (child_tap, and through it the merge, evaluation and
+// tile-marking kernels; the split and adjoint direct-BP pixel loops, where
+// the NaN range fails floor_index): clamping x here would turn such points
+// into defined taps.
 __device__ __forceinline__ double rsqrt64(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     const double e = fma(-0.5 * x, y * y, 0.5);

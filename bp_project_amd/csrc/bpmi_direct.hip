@@ -585,7 +585,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void bp_adjoint_kernel(BpArgs a, SrcAr
                 const double ex = px[k] - ax, ey = py[k] - ay, ez = pz[k] - az;
                 const double s2 = ex * ex + ey * ey + ez * ez;
                 const double rinv = rsqrt64(s2);
-                const double d = s2 * rinv;
+                const double d = s2 * rinv;     // NaN for s2 = 0: skipped below
                 const double q = (d - a.range_start) * inv_ddr;
                 int i;
                 if (!floor_index(q, &i)) continue;
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
                         const double ex = dxk[k] + ux, ey = dyk[k] + uy;
                         const double s2 = ex * ex + ey * ey + ez * ez;
                         const double rinv = rsqrt64(s2);
-                        const double d = s2 * rinv;
+                        const double d = s2 * rinv;     // NaN for s2 = 0: skipped below
                         const double qq = (d - a.range_start) * inv_ddr;
                         int i;
                         if (!floor_index(qq, &i)) continue;

@@ -1,6 +1,6 @@
 """One C2-sized FFBP for hardware-counter collection.
 
-    python -m bp_project_amd.profile_merges [--pulses 8192] [--no-untiled] [--cull grid]
+    python profile_merges.py [--pulses 8192] [--no-untiled] [--cull grid]
 
 Runs the split-precision FFBP of the 8192-pulse x 8192-bin stripmap workload
 (leaf 16, factor 2, 8-tap Kaiser-sinc merges) once through the level-batched
@@ -16,15 +16,20 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
-from . import _lib
-from .batched_resident_backend import BatchedResidentProgram, merge_block_table
-from .device import DeviceArray, synchronize
-from .gpu_backend import HipBaseline, build_ffbp_plan
-from .scenes import polar_extent_for_grid, workload_scene
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+
+from bp_project_amd import _lib  # noqa: E402
+from bp_project_amd.batched_resident_backend import (BatchedResidentProgram,  # noqa: E402
+                                                     merge_block_table)
+from bp_project_amd.device import DeviceArray, synchronize  # noqa: E402
+from bp_project_amd.gpu_backend import HipBaseline, build_ffbp_plan  # noqa: E402
+from bp_project_amd.scenes import polar_extent_for_grid, workload_scene  # noqa: E402
 
 _vp = ctypes.c_void_p
 COPY_BYTES = 1 << 30

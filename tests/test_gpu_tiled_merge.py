@@ -4,7 +4,8 @@ within 1e-4 of the FP64-mode merge, both when each child's footprint fits
 the LDS tile (coarse child u sampling) and when it does not and the kernel
 gathers from global memory (child u sampling 8x finer than the parent's).
 Part of the parent lies outside the children's valid stencil range, so
-zero-filled pixels are covered too."""
+zero-filled pixels are covered too, and so is a tile containing a pixel on
+a child's origin."""
 import ctypes
 
 import numpy as np
@@ -78,3 +79,60 @@ def test_tiled_merge_matches_untiled(child_du, interp):
     assert 0 < zero < p0 // 2
     assert np.array_equal(a, b)
     assert np.linalg.norm(a - c) <= 1e-4 * np.linalg.norm(c)
+
+
+@pytest.mark.parametrize("interp", ["kaiser8", "lagrange4"])
+def test_tile_with_a_pixel_on_a_child_origin(interp):
+    """Parent origin (0, 0, 0), children at (0, 10, 0): parent pixel (3, 2)
+    (r = 9.25 + 3 * 0.25 = 10, u = 0.5 + 2 * 0.25 = 1) lies on both children's
+    origin, where the tiled merge's geometry produces NaN (rsqrt64(0)) and the
+    pixel must read nothing: equal to the untiled merge bit for bit, the
+    origin pixel 0, nothing NaN."""
+    from bp_project_amd import _lib
+    from bp_project_amd.batched_resident_backend import merge_block_table
+    from bp_project_amd.device import DeviceArray, synchronize
+    from bp_project_amd.gpu_backend import ensure_kaiser_table
+    from bp_project_amd.plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, NI_SIDE
+    ensure_kaiser_table()
+    geo = [(0.0, 0.0, 0.0, 9.25, 0.25, 0.5, 0.25), (0.0, 10.0, 0.0, 0.5, 0.25, -0.9, 0.03),
+           (0.0, 10.0, 0.0, 0.5, 0.25, -0.9, 0.03)]
+    shapes = [(8, 3), (96, 64), (96, 64)]
+    node_int = np.zeros((3, 8), np.int32)
+    offsets = np.zeros(3, np.int64)
+    cursor = 0
+    for i, (nr, nu) in enumerate(shapes):
+        node_int[i, NI_NR], node_int[i, NI_NU], node_int[i, NI_SIDE] = nr, nu, 1
+        node_int[i, NI_LEFT] = node_int[i, NI_RIGHT] = -1
+        offsets[i] = cursor
+        cursor += -(-nr * nu // 64) * 64
+    node_int[0, NI_LEFT], node_int[0, NI_RIGHT], node_int[0, NI_HEIGHT] = 1, 2, 1
+    node_geo = np.array(geo, np.float64)
+    rng = np.random.default_rng(3)
+    arena_host = np.zeros(cursor, np.complex64)
+    for i in (1, 2):
+        n = 96 * 64
+        arena_host[offsets[i]:offsets[i] + n] = (rng.standard_normal(n) +
+                                                 1j * rng.standard_normal(n)).astype(np.complex64)
+    t_int, t_geo = DeviceArray.from_host(node_int), DeviceArray.from_host(node_geo)
+    t_off = DeviceArray.from_host(offsets)
+    k = 4 * np.pi * 9.6e9 / 299792458.0
+    out = {}
+    for name in ("tiled", "untiled"):
+        arena = DeviceArray.from_host(arena_host)
+        table = np.asarray(merge_block_table(node_int, [0], "split" if name == "tiled" else "x"),
+                           np.int32)
+        blocks = DeviceArray.from_host(table)
+        if name == "tiled":
+            _lib.call("bpmi_ffbp_merges_tiled", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
+                      _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.interp_code(interp),
+                      None)
+        else:
+            _lib.call("bpmi_ffbp_merges", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
+                      _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.MODE_SPLIT,
+                      _lib.interp_code(interp), None)
+        synchronize()
+        out[name] = arena.get()[:24].reshape(8, 3)
+    assert np.all(np.isfinite(out["tiled"]))
+    assert out["tiled"][3, 2] == 0
+    assert np.count_nonzero(out["tiled"]) >= 6
+    assert np.array_equal(out["tiled"], out["untiled"])
