@@ -47,8 +47,15 @@ PEAK_HBM_GBS = 8000.0
 # 1 GiB copy kernel (FETCH x 2.0, WRITE x 1.0); see
 # profiles/ffbp_c2_merge_counters.md.  Keyed by (workload, mode, stage).
 MEASURED_TRAFFIC_BYTES = {  # (workload, mode, kernel, cull)
-    ("c2_ffbp_full", "split", "batched_merge_regrid_split", "none"): 4.22e9,
-    ("c2_ffbp_full", "split", "batched_merge_regrid_split", "grid"): 3.63e9,
+    ("c2_ffbp_full", "split", "batched_merge_tiled_kernel<kaiser8>", "none"): 4.22e9,
+    ("c2_ffbp_full", "split", "batched_merge_tiled_kernel<kaiser8>", "grid"): 3.63e9,
+}
+# Schedule stage -> the HIP kernel its launches run (split mode, C2 settings).
+STAGE_KERNEL = {
+    "batched_bp_polar_split": "bp_split_kernel<lagrange4> (FFBP leaves)",
+    "batched_merge_regrid_split": "batched_merge_tiled_kernel<kaiser8>",
+    "eval_grid": "eval_grid_tiled_kernel<kaiser8>",
+    "ffbp_persistent_split": "ffbp_persistent_kernel<lagrange4, kaiser8>",
 }
 # Algorithmic operation counts (add/sub/mul = 1, FMA = 2, sqrt/sin/cos = 1).
 OPS_LEAF_LAGRANGE4 = 57      # per leaf pixel*pulse (4-point Lagrange)
@@ -85,6 +92,10 @@ def parse():
                         "grid's footprint in it ('grid') depends on (output bitwise unchanged)")
     p.add_argument("--no-unculled", action="store_true",
                    help="C2 with culling: skip timing the unculled step beside it")
+    p.add_argument("--rounds", type=int, default=15,
+                   help="rounds of the randomized interleaved timing protocol")
+    p.add_argument("--no-others", action="store_true",
+                   help="C2 default run: skip the summaries of the other workloads")
     p.add_argument("--persistent", action="store_true",
                    help="FFBP workloads: one persistent launch for leaves and all merge levels "
                         "(C5: and the evaluation)")
@@ -128,6 +139,46 @@ def step_timing(step_ms):
     return {"source": "HIP events around each timed step", "n": int(v.size),
             "median_ms": round(float(med), 4), "iqr_ms": round(float(q3 - q1), 4),
             "min_ms": round(float(v.min()), 4), "max_ms": round(float(v.max()), 4)}
+
+
+def interleaved_timing(variants, rounds=15, warmups=2, seed=20260917):
+    """The reference's timing protocol (fast_bp/resident_experiment.py:87-119):
+    ``warmups`` untimed calls of every variant, then ``rounds`` rounds that
+    each call every variant once in a seeded random order; each call is
+    bracketed by HIP events on the launch stream and by the host wall clock,
+    and synchronised.  Returns per-variant median/IQR of both and the orders."""
+    from bp_project_amd.device import Event, elapsed_ms
+    rng = np.random.default_rng(seed)
+    names = list(variants)
+    for name in names:
+        for _ in range(warmups):
+            variants[name]()
+    ev = {n: [] for n in names}
+    wall = {n: [] for n in names}
+    orders = []
+    a, b = Event(), Event()
+    for _ in range(rounds):
+        order = [names[i] for i in rng.permutation(len(names))]
+        orders.append(order)
+        for name in order:
+            t = time.perf_counter()
+            a.record()
+            variants[name]()
+            b.record()
+            b.synchronize()
+            wall[name].append(1000 * (time.perf_counter() - t))
+            ev[name].append(elapsed_ms(a, b))
+
+    def stats(v):
+        q1, med, q3 = np.percentile(np.asarray(v), [25, 50, 75])
+        return round(float(med), 4), round(float(q3 - q1), 4)
+    out = {"protocol": "seeded randomized interleaved rounds (resident_experiment.py:87-119)",
+           "rounds": rounds, "warmups": warmups, "seed": seed, "orders": orders}
+    for n in names:
+        (em, ei), (wm, wi) = stats(ev[n]), stats(wall[n])
+        out[n] = {"event_median_ms": em, "event_iqr_ms": ei, "wall_median_ms": wm,
+                  "wall_iqr_ms": wi}
+    return out
 
 
 def ffbp_workload(args, scene):
@@ -240,11 +291,12 @@ def run_ffbp(args, scene):
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": MEASURED_TRAFFIC_BYTES.get((args.workload, args.mode, dominant,
-                                                   args.cull)),
+            "traffic": MEASURED_TRAFFIC_BYTES.get((args.workload, args.mode,
+                                                   STAGE_KERNEL.get(dominant), args.cull)),
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                            "copy-kernel calibrated)",
-            "kernel": dominant,
+                            "copy-kernel calibrated; profiles/ffbp_c2_merge_counters.md)",
+            "kernel": STAGE_KERNEL.get(dominant, dominant),
+            "stage": dominant,
             "ops_per_unit": (OPS_LEAF_LAGRANGE4 if "bp_polar" in dominant else
                              ops_merge if "merge" in dominant else
                              f"{OPS_LEAF_LAGRANGE4} per leaf pixel*pulse + {ops_merge} per parent "
@@ -273,24 +325,20 @@ def run_ffbp(args, scene):
         result["cull"] = dict(prog.cull, computed_fraction=round(
             prog.cull["computed_pixels"] / prog.cull["node_pixels"], 4))
     if prog.cull is not None and not args.no_unculled:
-        # the same step without culling: its time, and the outputs compared bit for bit
+        # the same step without culling, timed against the culled step by
+        # the reference's randomized interleaved protocol; outputs compared
+        # bit for bit
         full_prog, full_cart, full_step, _ = make(args.mode)
-        for _ in range(args.warmup):
-            full_step()
-        synchronize()
-        start.record()
-        for _ in range(args.steps):
-            full_step()
-        end.record()
-        synchronize()
-        full_ms = elapsed_ms(start, end) / args.steps
+        inter = interleaved_timing({"culled": step, "unculled": full_step},
+                                   rounds=args.rounds, warmups=args.warmup)
         step()
         synchronize()
         same = bool(np.array_equal(cart.get(), full_cart.get()))
         full_prog.close()
-        result["cull"].update(unculled_ms_per_step=round(full_ms, 4),
+        result["timing"]["interleaved"] = inter
+        result["cull"].update(unculled_ms_per_step=inter["unculled"]["event_median_ms"],
                               output_bitwise_equal_unculled=same)
-        log("cull:", json.dumps(result["cull"]))
+        log("interleaved:", json.dumps({k: v for k, v in inter.items() if k != "orders"}))
         if not same:
             raise SystemExit("culled output differs from the unculled output")
     if not args.no_quality:
@@ -323,6 +371,9 @@ def run_ffbp(args, scene):
             "passes_nrmse_1e-4": bool(f["relative_complex_error"] <= 1e-4),
         }
         log("quality:", json.dumps(result["quality"]))
+        if not result["quality"]["passes_nrmse_1e-4"]:
+            raise SystemExit(f"quality gate failed: NRMSE {f['relative_complex_error']:.3e} "
+                             "> 1e-4 against the FP64 image; no throughput is reported")
     prog.close()
     return result
 
@@ -431,6 +482,9 @@ def run_direct(args, scene, name):
                     "passes_nrmse_1e-4": bool(f["relative_complex_error"] <= 1e-4)}
         result["quality"] = dict(reference="FP64 direct BP on the GPU", **assess(img))
         log("quality:", json.dumps(result["quality"]))
+        if args.mode != "fp32" and not result["quality"]["passes_nrmse_1e-4"]:
+            raise SystemExit(f"quality gate failed: {args.mode} NRMSE "
+                             f"{result['quality']['nrmse']:.3e} > 1e-4 against the FP64 image")
         if args.precision_study:
             study = {}
             for mode in ("fp32", "mixed", "dfloat", "split"):
@@ -452,7 +506,9 @@ def run_direct(args, scene, name):
 
 def run_autofocus(args, scene):
     """Config 4: 30 Adam iterations of entropy autofocus through the HIP adjoint,
-    against the per-step cost of FP64 BP + central-difference gradients."""
+    against plain FP64 BP with finite-difference gradients: the FD cost per
+    parameter measured on a block of pulses at full size, and FD autofocus run
+    to the adjoint's final entropy on a reduced scene of the same geometry."""
     import torch
     from bp_project_amd.autofocus import GridOperator, entropy, gradient_autofocus
     from bp_project_amd.device import Event, elapsed_ms, synchronize
@@ -461,26 +517,18 @@ def run_autofocus(args, scene):
                              lr_pos=5e-4)
     steps_ms = res["ms_per_iteration"][2:] or res["ms_per_iteration"]
     ms_step = float(np.median(steps_ms))
-    # FP64 BP with complex128 output: the image a finite-difference autofocus needs
     phase = torch.from_numpy(scene.phase).cuda()
     pos = torch.from_numpy(scene.pos).cuda()
-    op.image64(phase, pos)
-    synchronize()
-    a, b = Event(), Event()
-    a.record()
-    for _ in range(3):
-        op.image64(phase, pos)
-    b.record()
-    synchronize()
-    fp64_ms = elapsed_ms(a, b) / 3
     # the two HIP launches of a gradient step on their own: split forward BP, adjoint
     rng = np.random.default_rng(1)
     gimg = torch.from_numpy((rng.standard_normal((scene.grid.ny, scene.grid.nx)) + 1j *
                              rng.standard_normal((scene.grid.ny, scene.grid.nx)))
                             .astype(np.complex64)).cuda()
+    a, b = Event(), Event()
     kernel_ms = {}
     for name, fn in (("forward_bp_ms", lambda: op.image(phase, pos)),
-                     ("adjoint_ms", lambda: op.adjoint(phase, pos, gimg))):
+                     ("adjoint_ms", lambda: op.adjoint(phase, pos, gimg)),
+                     ("fp64_bp_image_ms", lambda: op.image64(phase, pos))):
         fn()
         synchronize()
         a.record()
@@ -489,14 +537,16 @@ def run_autofocus(args, scene):
         b.record()
         synchronize()
         kernel_ms[name] = round(elapsed_ms(a, b) / 5, 3)
-    # central differences: 2 images per parameter, 4 parameters (phase, xyz) per pulse
-    fd_step_ms = 2 * 4 * scene.pulses * fp64_ms
+    fd = fd_measured_block(op, scene, block=8)
+    fd_step_ms = fd["ms_per_parameter"] * 4 * scene.pulses
     with torch.no_grad():
         true_img = op.image(torch.from_numpy(scene.phase - scene.meta["phase_error"]).cuda(),
                             torch.from_numpy(scene.true_pos).cuda())
         true_entropy = float(entropy(true_img))
+    reduced = fd_equal_entropy(args)
     g = scene.grid
     pp = g.nx * g.ny * scene.pulses
+    achieved = pp * OPS_ADJOINT / (kernel_ms["adjoint_ms"] * 1e-3) / 1e12
     return {
         "ms_per_step": ms_step,
         "value": pp / (ms_step * 1e-3) / 1e9,
@@ -504,20 +554,20 @@ def run_autofocus(args, scene):
             "ms_per_gradient_step": round(ms_step, 3),
             "iterations": len(res["ms_per_iteration"]),
             "entropy_initial": res["entropy"][0], "entropy_final": res["best_entropy"],
-            "entropy_with_true_phase": true_entropy,
-            "fp64_bp_image_ms": round(fp64_ms, 3), **kernel_ms,
-            "fd_fp64_gradient_step_ms_extrapolated": round(fd_step_ms, 1),
-            "speedup_vs_fd_fp64": round(fd_step_ms / ms_step, 1),
-            "note": "FD baseline = 2 x 4 x pulses FP64 GPU BP images per gradient step "
-                    "(measured image time x count); adjoint gradients equal the FD ones "
-                    "(tests/test_gpu_autofocus.py), so both reach the same entropy"},
+            "entropy_with_true_phase": true_entropy, **kernel_ms,
+            "fd_measured_block": fd,
+            "fd_fp64_gradient_step_ms": round(fd_step_ms, 1),
+            "fd_fp64_gradient_step_source": (
+                f"measured: central differences over {fd['parameters']} parameters "
+                f"({fd['pulses']} pulses x phase, x, y, z; {fd['images']} FP64 images) took "
+                f"{fd['ms']:.1f} ms = {fd['ms_per_parameter']:.3f} ms per parameter; "
+                f"x {4 * scene.pulses} parameters of a full gradient"),
+            "speedup_vs_fd_fp64_per_step": round(fd_step_ms / ms_step, 1),
+            "equal_entropy_reduced_scene": reduced},
         "roofline": {
-            "bound": "valu", "achieved": round(pp * OPS_ADJOINT / (kernel_ms["adjoint_ms"] * 1e-3)
-                                               / 1e12, 3),
-            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(pp * OPS_ADJOINT / (kernel_ms["adjoint_ms"] * 1e-3) / 1e12
-                          / PEAK_FP32_TFLOPS, 4),
-            "traffic": None, "kernel": "bp_adjoint_split_kernel (+ adjoint_reduce_kernel)",
+            "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": "bp_adjoint_split_kernel (+ adjoint_reduce_kernel)",
             "ops_per_unit": OPS_ADJOINT, "unit_of_work": "pixel*pulse",
             "note": "dominant kernel of a gradient step (the forward BP is the SPLIT "
                     "kernel of C3); time = HIP events around one adjoint call"},
@@ -525,6 +575,96 @@ def run_autofocus(args, scene):
                    "output": [g.ny, g.nx], "pixel_m": g.dx, "mode": args.mode,
                    "optimizer": "Adam(phase lr 0.05, pos lr 5e-4), 30 iterations"},
     }
+
+
+def fd_measured_block(op, scene, block=8):
+    """Central-difference entropy gradient of ``block`` pulses (phase and
+    x, y, z each: 8 FP64-mode images per pulse) at full size, timed."""
+    from bp_project_amd.autofocus import finite_difference_gradient
+    from bp_project_amd.device import synchronize
+    pulses = np.random.default_rng(3).choice(scene.pulses, block, replace=False)
+    finite_difference_gradient(op, scene.phase, scene.pos, pulses[:1], h=1e-4, mode="fp64")
+    synchronize()
+    t = time.perf_counter()
+    finite_difference_gradient(op, scene.phase, scene.pos, pulses, h=1e-4, mode="fp64")
+    synchronize()
+    ms = 1000 * (time.perf_counter() - t)
+    return {"pulses": block, "parameters": 4 * block, "images": 8 * block, "ms": round(ms, 2),
+            "ms_per_parameter": round(ms / (4 * block), 4),
+            "timing": "host wall clock around the whole FD evaluation (images, entropies)"}
+
+
+def fd_equal_entropy(args, iterations=30):
+    """A reduced scene of the C4 geometry (256 pulses onto 64 x 64 at 0.5 m,
+    same phase random walk and along-track error): the adjoint autofocus runs
+    ``iterations`` Adam steps; FD autofocus (fd_autofocus: same optimiser,
+    FP64 images, 8 P images per step) runs until it reaches the adjoint's
+    final entropy (at most twice as many steps).  Both final states are
+    scored with the same FP64 image."""
+    import torch
+    from bp_project_amd.autofocus import (GridOperator, entropy, fd_autofocus,
+                                          gradient_autofocus)
+    from bp_project_amd.scenes import stripmap_scene
+    s = stripmap_scene(pulses=256, bins=1024, n_targets=8, grid_n=64, grid_spacing=0.5,
+                       seed=20260920, phase_walk_rms=0.5, along_track_error=0.02)
+    op = GridOperator.from_scene(s, mode=args.mode, interp="lagrange4")
+    kw = dict(lr_phase=0.05, lr_pos=5e-4)
+    gradient_autofocus(op, s.phase, s.pos, iterations=2, **kw)  # warm-up
+    adj = gradient_autofocus(op, s.phase, s.pos, iterations=iterations, **kw)
+    target = adj["entropy"][-1]
+    reach = next((i for i, e in enumerate(adj["entropy"]) if e <= target), iterations)
+    adj_ms = float(np.sum(adj["ms_per_iteration"][:reach]))
+    fd = fd_autofocus(op, s.phase, s.pos, iterations=2 * iterations, target=target, **kw)
+    fd_ms = float(np.sum(fd["ms_per_iteration"]))
+    fd_steps = len(fd["ms_per_iteration"])
+    reached = fd["entropy"][-1] <= target
+
+    def score(ph, ps):
+        with torch.no_grad():
+            return float(entropy(op.image64(torch.from_numpy(ph).cuda(),
+                                            torch.from_numpy(ps).cuda())))
+    return {"scene": "256 pulses x 1024 bins onto 64 x 64 at 0.5 m, 8 targets, phase walk "
+                     "0.5 rad RMS, along-track error 2 cm (seed 20260920)",
+            "target_entropy": target, "adjoint_steps_to_target": reach,
+            "adjoint_ms_to_target": round(adj_ms, 2),
+            "fd_steps": fd_steps, "fd_reached_target": bool(reached),
+            "fd_ms": round(fd_ms, 1), "fd_images": fd["images"],
+            "fd_ms_per_step": round(fd_ms / max(fd_steps, 1), 2),
+            "adjoint_ms_per_step": round(float(np.median(adj["ms_per_iteration"])), 3),
+            "final_entropy_fp64_image": {"adjoint": score(adj["phase"], adj["pos"]),
+                                         "fd": score(fd["phase"], fd["pos"])},
+            "speedup_to_equal_entropy": round(fd_ms / max(adj_ms, 1e-9), 1),
+            "note": "measured wall time of both loops (synchronised per step); the FD run "
+                    "stops at the first step whose entropy is at most the target"}
+
+
+def cpu_baseline_autofocus(scene, budget_s=10.0):
+    """The oracle's FP64 gradient step -- forward BP, entropy gradient
+    image, adjoint over every pulse -- on a subset of grid rows (the image
+    gradient of those rows; the work per row does not depend on its values)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    g = scene.grid
+
+    def run(rows):
+        pts = g.points()[: rows * g.nx]
+        t = time.perf_counter()
+        img = oracle.backproject_points(scene.data, scene.pos, pts, scene.r_start,
+                                        scene.data_dr, scene.k, scene.weight, scene.phase)
+        _, grad = oracle.entropy_and_grad(img)
+        oracle.backproject_points_adjoint(scene.data, scene.pos, pts, scene.r_start,
+                                          scene.data_dr, scene.k, scene.weight, scene.phase,
+                                          grad)
+        return len(pts), time.perf_counter() - t
+    n, dt = run(1)
+    rows = min(g.ny, max(1, int(budget_s / max(dt, 1e-3))))
+    n, dt = run(rows)
+    return {"value": round(n * scene.pulses / dt / 1e9, 4), "unit": "Gpix·pulse/s",
+            "cores": oracle.num_threads(), "kind": "port", "seconds": round(dt, 2),
+            "ms_per_gradient_step_extrapolated": round(1000 * dt * g.ny / rows, 1),
+            "sample": f"FP64 oracle forward BP + entropy gradient + adjoint of {rows} grid rows "
+                      f"({n} pixels) x {scene.pulses} pulses; rate in pixel*pulse/s of a "
+                      "gradient step (extrapolation to the full grid by rows)"}
 
 
 def run_multipatch(args):
@@ -567,8 +707,36 @@ def run_multipatch(args):
                                                                  merge_interp="kaiser8",
                                                                  tile_cols_log2=best, cull=cull)
     same = all(np.array_equal(x, y.get()) for x, y in zip(batch_out, seq_out))
+    for _, p_, _, _ in progs:
+        p_.close()
     work = prog.work
     seq_total = completion[-1]
+    roofline = None
+    if not args.persistent:
+        # stage times of the batch from events between its launches (eager
+        # execution of the same schedule the graph holds)
+        timer = StageTimer()
+        timer.mark("start")
+        reps = 3
+        for _ in range(reps):
+            prog.execute(mark=timer.mark)
+        synchronize()
+        stages = timer.durations()
+        merge_ms = float(np.sum(stages["merges"])) / reps
+        merge_px = int(sum(prog.merge_pixels))
+        achieved = merge_px * OPS_MERGE_KAISER8 / (merge_ms * 1e-3) / 1e12
+        roofline = {
+            "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": "batched_merge_tiled_kernel<kaiser8>",
+            "ops_per_unit": OPS_MERGE_KAISER8, "unit_of_work": "parent pixel",
+            "launches_per_step": len(prog.merge_tables),
+            "parent_pixels_per_batch": merge_px,
+            "stages_ms_per_batch": {k: round(float(np.sum(v)) / reps, 4)
+                                    for k, v in stages.items()},
+            "note": "dominant stage of the batch (merge launches of every height); events "
+                    "between the launches of eager executions"}
+    cpu = None if args.no_cpu else cpu_baseline_multipatch(scenes)
     return {
         "ms_per_step": ms_batch,
         "value": work / (ms_batch * 1e-3) / 1e9,
@@ -588,11 +756,44 @@ def run_multipatch(args):
             "batched_equals_sequential_bitwise": bool(same),
             "autotuned_leaf_tile_cols_log2": best, "autotune_ms": times,
             "launches_per_batch": prog.launch_count},
-        "roofline": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "config": {"workload": "c5_multipatch", "patches": len(scenes), "mode": args.mode,
                    "merge_interp": "kaiser8", "graph": True, "persistent": bool(args.persistent),
                    "cull": "grid" if cull else "none"},
     }
+
+
+def cpu_baseline_multipatch(scenes, budget_s=10.0):
+    """The oracle's FFBP + Cartesian evaluation patch after patch (the
+    sequential per-patch loop) over a seeded subset of the batch, until about
+    ``budget_s`` of CPU work; effective pixel*pulse/s."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    from bp_project_amd.scenes import polar_extent_for_grid
+    order = np.random.default_rng(20260921).permutation(len(scenes))
+    work = seconds = 0.0
+    done = []
+    for i in order:
+        s = scenes[int(i)]
+        t = time.perf_counter()
+        extent = polar_extent_for_grid(s.grid, s.pos.mean(axis=0), margin=2.0)
+        res, _ = oracle.form_ffbp(s.data, s.pos, s.r_start, s.data_dr, s.fc, extent, side=1,
+                                  leaf_size=16, range_spacing=0.5, angular_oversample=2.5,
+                                  weight=s.weight, phase=s.phase, policy="parity",
+                                  leaf_interp="lagrange4", merge_interp="kaiser8")
+        oracle.evaluate_image(res.image, res.origin, res.r0, res.dr, res.u0, res.du,
+                              s.grid.points(), s.k, interp="kaiser8")
+        seconds += time.perf_counter() - t
+        work += s.grid.nx * s.grid.ny * s.pulses
+        done.append(int(i))
+        if seconds >= budget_s:
+            break
+    return {"value": round(work / seconds / 1e9, 4), "unit": "Gpix·pulse/s",
+            "cores": oracle.num_threads(), "kind": "port", "seconds": round(seconds, 2),
+            "sample": f"FP64 oracle FFBP + evaluate_image of {len(done)} of the {len(scenes)} "
+                      f"patches (seeded order {done[:8]}...), one after another; effective "
+                      "output pixel*pulse/s"}
 
 
 def run_range_compress(args):
@@ -657,7 +858,7 @@ def cpu_baseline_range_compress():
                       f"x {samples} samples"}
 
 
-def cpu_baseline_direct(scene, name):
+def cpu_baseline_direct(scene, name, budget_s=10.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     g = scene.grid
@@ -670,13 +871,13 @@ def cpu_baseline_direct(scene, name):
                                   scene.k, scene.weight, scene.phase, interp=interp)
         return len(pts), time.perf_counter() - t
 
-    # a probe of 2 grid rows sizes a sample of ~10 s of CPU work (whole grids
-    # repeated when the grid is smaller than that)
+    # a probe of 2 grid rows sizes a sample of ~budget_s of CPU work (whole
+    # grids repeated when the grid is smaller than that)
     n, dt = run(2)
-    rows = min(g.ny, max(1, int(2 * 10.0 / max(dt, 1e-3))))
+    rows = min(g.ny, max(1, int(2 * budget_s / max(dt, 1e-3))))
     total_px = total_s = 0.0
     reps = 0
-    while reps == 0 or total_s < 8.0:
+    while reps == 0 or total_s < 0.8 * budget_s:
         n, dt = run(rows)
         total_px += n
         total_s += dt
@@ -711,10 +912,10 @@ def main():
         cpu = None if args.no_cpu else cpu_baseline_ffbp(args, scene)
     elif args.workload == "c4_autofocus":
         res = run_autofocus(args, scene)
-        cpu = None
+        cpu = None if args.no_cpu else cpu_baseline_autofocus(scene)
     elif args.workload == "c5_multipatch":
         res = run_multipatch(args)
-        cpu = None
+        cpu = res.get("cpu_baseline")
     elif args.workload == "range_compress":
         res = run_range_compress(args)
         cpu = None if args.no_cpu else cpu_baseline_range_compress()
@@ -739,7 +940,58 @@ def main():
         line["precision_study"] = res["precision_study"]
     if "stages" in res:
         line["stages_ms_per_step"] = {k: round(v["ms_per_step"], 4) for k, v in res["stages"].items()}
+    if args.workload == "c2_ffbp_full" and not args.no_others:
+        del scene, res
+        line["other_workloads"] = other_workloads(args)
     print(json.dumps(line), flush=True)
+
+
+def other_workloads(args):
+    """The other stripmap workloads, each measured in this same run with
+    fewer steps (C1 and C3 direct BP, C4 autofocus, C5 batched patches),
+    with their roofline fraction and a shorter CPU baseline (~4 s each):
+    compact records of what ``--workload NAME`` reports in full."""
+    from bp_project_amd.scenes import workload_scene
+    out = {}
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup = 5, 1
+    for name in ("c1_direct_small", "c3_direct_precision", "c4_autofocus", "c5_multipatch"):
+        t0 = time.perf_counter()
+        scene = None if name == "c5_multipatch" else workload_scene(name)
+        sub.workload = name
+        sub.precision_study = name == "c3_direct_precision"
+        if name == "c4_autofocus":
+            res = run_autofocus(sub, scene)
+            cpu = None if args.no_cpu else cpu_baseline_autofocus(scene, budget_s=4.0)
+        elif name == "c5_multipatch":
+            res = run_multipatch(sub)
+            cpu = res.get("cpu_baseline")
+        else:
+            res = run_direct(sub, scene, name)
+            cpu = None if args.no_cpu else cpu_baseline_direct(scene, name, budget_s=4.0)
+        rec = {"value": round(res["value"], 3), "unit": res.get("unit", "Gpix·pulse/s"),
+               "ms_per_step": round(res["ms_per_step"], 4), "steps": sub.steps,
+               "config": res["config"]}
+        if res.get("roofline"):
+            rec["roofline"] = {k: res["roofline"][k] for k in ("bound", "achieved", "peak",
+                                                               "unit", "frac", "kernel")}
+        if cpu:
+            rec["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind",
+                                                       "sample")}
+        for key in ("quality", "precision_study", "multipatch"):
+            if key in res:
+                rec[key] = res[key]
+        if "autofocus" in res:
+            af = res["autofocus"]
+            rec["autofocus"] = {k: af[k] for k in (
+                "ms_per_gradient_step", "entropy_initial", "entropy_final",
+                "fd_fp64_gradient_step_ms", "speedup_vs_fd_fp64_per_step",
+                "equal_entropy_reduced_scene")}
+        rec["seconds"] = round(time.perf_counter() - t0, 1)
+        log(f"{name}:", json.dumps(rec)[:600])
+        out[name] = rec
+        del scene, res
+    return out
 
 
 if __name__ == "__main__":

@@ -151,10 +151,12 @@ class MultiPatchProgram:
         self.merge_tables = []
         tiles = (None if self._masks is None else
                  {n: merge_tiles_of(m) for n, m in enumerate(self._masks)})
+        self.merge_pixels = []  # parent pixels each merge launch computes
         for h in range(1, int(heights.max()) + 1):
             nodes = np.flatnonzero(heights == h)
             mapping = merge_block_table(self.node_int, nodes, self.mode, tiles) or [(-1, 0)]
             self.merge_tables.append(DeviceArray.from_host(np.asarray(mapping, np.int32)))
+            self.merge_pixels.append(self._mapping_pixels(mapping))
         # Cartesian evaluation jobs (crop as a view into each root)
         jobs = np.zeros(len(self.patches), EVAL_JOB)
         self.outputs = []
@@ -192,6 +194,23 @@ class MultiPatchProgram:
         if self.persistent:
             self._build_persistent()
             self.launch_count = 1 if self.persistent is True else 2
+
+    def _mapping_pixels(self, mapping):
+        """Parent pixels covered by a merge block table (32 x 32 tiles clipped
+        to their node in split mode, else blocks of merge pixels)."""
+        from .batched_resident_backend import MERGE_BLOCK
+        total = 0
+        for n, t in mapping:
+            if n < 0:
+                continue
+            nr, nu = int(self.node_int[n, NI_NR]), int(self.node_int[n, NI_NU])
+            if self.mode == "split":
+                tu = -(-nu // 32)
+                r0, c0 = (t // tu) * 32, (t % tu) * 32
+                total += (min(r0 + 32, nr) - r0) * (min(c0 + 32, nu) - c0)
+            else:
+                total += min(MERGE_BLOCK, nr * nu - t)
+        return total
 
     def _required_masks(self):
         """Tile masks of every node (global index) that each patch's
@@ -280,7 +299,10 @@ class MultiPatchProgram:
         """(output pixels x pulses) summed over patches."""
         return sum(pp.scene.grid.nx * pp.scene.grid.ny * pp.scene.pulses for pp in self.patches)
 
-    def execute(self):
+    def execute(self, mark=None):
+        """Enqueue the batch on the current stream.  ``mark(name)``, when
+        given, is called after each launch ("leaves", "merges", "eval") so a
+        caller can record events between them."""
         mode = _lib.mode_code(self.mode)
         s = stream_handle()
         if self._persistent is not None:
@@ -289,10 +311,14 @@ class MultiPatchProgram:
             self._persistent.launch(s)
             return self.outputs
         self._launch_leaves(mode, s)
+        if mark:
+            mark("leaves")
         for table in self.merge_tables:
             enqueue_merges(self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
                            self.arena.ptr, table.ptr, table.shape[0], self.k, self.mode,
                            self.merge_interp, s)
+            if mark:
+                mark("merges")
         if self.mode == "split":
             _lib.call("bpmi_eval_grid_multi_tiled", _vp(self.t_jobs.ptr),
                       _vp(self.t_eval_blocks.ptr), self.n_eval_blocks,
@@ -300,6 +326,8 @@ class MultiPatchProgram:
         else:
             _lib.call("bpmi_eval_grid_multi", _vp(self.t_jobs.ptr), _vp(self.t_eval_blocks.ptr),
                       self.n_eval_blocks, mode, _lib.interp_code(self.merge_interp), s)
+        if mark:
+            mark("eval")
         return self.outputs
 
     def _launch_leaves(self, mode, s):
