@@ -35,6 +35,7 @@ SIGNATURES = {
                          POINTER(c_size_t)],
     "bpmi_runtime_version": [POINTER(c_int)],
     "bpmi_mem_info": [POINTER(c_size_t), POINTER(c_size_t)],
+    "bpmi_wall_clock_khz": [POINTER(c_int)],
     "bpmi_malloc": [POINTER(c_void_p), _sz],
     "bpmi_free": [_vp],
     "bpmi_host_alloc": [POINTER(c_void_p), _sz],

@@ -1801,8 +1801,12 @@ int bpmi_ffbp_persistent(const bpmi_persistent_schedule* sc, int leaf_interp, in
     pa.sync = (int*)(ws + PW_ARGS_BYTES);
     pa.nodes = sc->nodes;
     pa.stamps = (unsigned long long*)(ws + pw_stamps_offset(sc->nodes, sc->njobs));
-    // the device wall clock runs at 100 MHz
-    pa.spin_ticks = (unsigned long long)(sc->max_wait_s * 1.0e8);
+    // the wait bound in ticks of the device wall clock, at the rate the
+    // device reports for it
+    int khz = 0;
+    st = bpmi_wall_clock_khz(&khz);
+    if (st) return st;
+    pa.spin_ticks = (unsigned long long)(sc->max_wait_s * 1.0e3 * (double)khz);
     pa.k = sc->k;
     hipStream_t hs = (hipStream_t)stream;
     const int nsync = (int)pw_sync_ints(sc->nodes, sc->njobs);

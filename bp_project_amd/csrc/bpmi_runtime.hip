@@ -90,6 +90,18 @@ int bpmi_runtime_version(int* version) {
     return check_hip(hipRuntimeGetVersion(version), "hipRuntimeGetVersion");
 }
 
+int bpmi_wall_clock_khz(int* khz) {
+    if (!khz) return set_error(BPMI_ERR_INVALID, "khz is null");
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st) return st;
+    st = check_hip(hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev),
+                   "hipDeviceGetAttribute(wall clock rate)");
+    if (st) return st;
+    if (*khz <= 0) return set_error(BPMI_ERR_HIP, "device reports no wall clock rate");
+    return BPMI_OK;
+}
+
 int bpmi_mem_info(size_t* free_bytes, size_t* total_bytes) {
     return check_hip(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
 }

@@ -30,7 +30,7 @@ from .plan import NI_HEIGHT, NI_NR, NI_NU
 WORK_LEAF, WORK_MERGE, WORK_EVAL = 0, 1, 2
 EVAL_BLOCK = 256
 INTERP_NONE = -1  # BPMI_INTERP_NONE: a work list without leaf items
-TICKS_PER_MS = 1.0e5  # the device wall clock runs at 100 MHz
+
 
 
 def _split_tiles(nr, nu, tc):
@@ -201,7 +201,9 @@ class PersistentFFBP:
         stamps = np.zeros(1 + self.njobs, np.uint64)
         _lib.call("bpmi_persistent_stamps", _lib.c_void_p(self.workspace.ptr), self.nodes,
                   self.njobs, stamps.ctypes.data_as(_lib.c_void_p))
-        return (stamps[1:].astype(np.int64) - np.int64(stamps[0])) / TICKS_PER_MS
+        khz = ctypes.c_int()
+        _lib.call("bpmi_wall_clock_khz", ctypes.byref(khz))
+        return (stamps[1:].astype(np.int64) - np.int64(stamps[0])) / float(khz.value)
 
     def close(self):
         self._keep = []

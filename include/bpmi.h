@@ -61,6 +61,9 @@ int bpmi_device_info(int device, char* name, size_t name_len, char* arch, size_t
                      int* cu_count, int* clock_khz, size_t* total_mem);
 int bpmi_runtime_version(int* version);
 int bpmi_mem_info(size_t* free_bytes, size_t* total_bytes);
+/* Rate of the device wall clock (s_memrealtime, the clock of the persistent
+ * scheduler's bounded waits and completion stamps) of the current device, kHz. */
+int bpmi_wall_clock_khz(int* khz);
 int bpmi_malloc(void** ptr, size_t bytes);
 int bpmi_free(void* ptr);
 int bpmi_host_alloc(void** ptr, size_t bytes);

@@ -1,6 +1,8 @@
 """One-launch (persistent) FFBP on the GPU: bitwise equal to the level-by-level
 launches for a single plan and for many uneven patches, under graph replay
 too; a dependency wait that runs out is reported and the launch still ends."""
+import time
+
 import numpy as np
 import pytest
 
@@ -105,10 +107,16 @@ def test_persistent_wait_timeout_is_reported_and_the_launch_ends():
     tiles = pf.node_tiles_host.copy()
     tiles[leaf] += 1  # a completion count the leaf never reaches
     pf.node_tiles_dev.set(tiles)
-    pf.schedule.max_wait_s = 0.05
+    pf.schedule.max_wait_s = 0.5
+    t = time.perf_counter()
     prog.execute()
     synchronize()
+    elapsed = time.perf_counter() - t
     assert pf.aborted()
+    # the bound is in wall-clock time: the first wait that cannot complete
+    # gives up after 0.5 s at the device's reported clock rate
+    print(f"aborted launch took {elapsed:.2f} s for a 0.5 s wait bound")
+    assert 0.4 <= elapsed <= 5.0
     with pytest.raises(RuntimeError, match="aborted"):
         prog.completion_ms()
     pf.node_tiles_dev.set(pf.node_tiles_host)
