@@ -96,6 +96,10 @@ def parse():
                    help="rounds of the randomized interleaved timing protocol")
     p.add_argument("--no-others", action="store_true",
                    help="C2 default run: skip the summaries of the other workloads")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="C2: keep the default launch shapes instead of choosing the leaf pulse "
+                        "block depth and one merge shape per level with the autotuner before "
+                        "timing (C5 always tunes)")
     p.add_argument("--persistent", action="store_true",
                    help="FFBP workloads: one persistent launch for leaves and all merge levels "
                         "(C5: and the evaluation)")
@@ -181,6 +185,33 @@ def interleaved_timing(variants, rounds=15, warmups=2, seed=20260917):
     return out
 
 
+def autotune_summary(config, report, cached, defaults, per_stage=("merge_shapes",)):
+    """The tuner's choice, its per-candidate stage times (ms, medians of
+    event-timed runs) and, per knob group, the timed stages' total under the
+    default shape against the chosen one."""
+    gain = {}
+    for name, r in report.items():
+        times = r.get("times") or {}
+        d = json.dumps(defaults.get(name))
+        choice = r.get("choice", config.get(name))
+        if d not in times:
+            continue
+        if name in per_stage:
+            tuned = sum(times[json.dumps(c)][i] for i, c in enumerate(choice))
+        else:
+            tuned = sum(times[json.dumps(choice)])
+        gain[name] = {"default_ms": round(float(sum(times[d])), 4),
+                      "tuned_ms": round(float(tuned), 4)}
+    return {"config": config, "cached": bool(cached),
+            "search": "coordinate search, one knob group at a time with the others at their "
+                      "best; each launch stage timed by HIP events between launches",
+            "candidates_ms": {name: {"stages": r.get("stages"),
+                                     "times": {k: [round(float(x), 4) for x in v]
+                                               for k, v in (r.get("times") or {}).items()}}
+                              for name, r in report.items()},
+            "default_vs_tuned": gain}
+
+
 def ffbp_workload(args, scene):
     from bp_project_amd import _lib
     from bp_project_amd.batched_resident_backend import BatchedResidentProgram
@@ -215,7 +246,6 @@ def ffbp_workload(args, scene):
         # is a strided view (row length = root nu) instead of a copy.
         root = prog.node_image(plan.root.index)
         crop_ptr = root.ptr + 8 * (plan.crop_r[0] * plan.root.nu + plan.crop_u[0])
-        schedule = [e for e in prog._schedule if e[0] != "crop_float2"]
 
         def evaluate():
             _lib.call("bpmi_eval_grid", vp(crop_ptr), geo.ctypes.data_as(vp), plan.nr, plan.nu,
@@ -223,7 +253,10 @@ def ffbp_workload(args, scene):
                       vp(cart.ptr), stream_handle())
 
         def step(timer=None):
-            for name, launcher, grid_, block, a in schedule:
+            # the schedule is read at each call: set_launch_shapes() edits it
+            for name, launcher, grid_, block, a in prog._schedule:
+                if name == "crop_float2":
+                    continue
                 launcher(grid_, block, a)
                 if timer:
                     timer.mark(name)
@@ -242,6 +275,16 @@ def run_ffbp(args, scene):
     plan, backend, dev, make = ffbp_workload(args, scene)
     g = scene.grid
     prog, cart, step, _ = make(args.mode, args.cull)
+    tuned = None
+    if not args.no_autotune and args.mode == "split" and not args.persistent:
+        from types import SimpleNamespace
+        t = time.perf_counter()
+        cfg, report, cached = prog.autotune(
+            execute=lambda mark: step(None if mark is None else SimpleNamespace(mark=mark)))
+        tuned = autotune_summary(cfg, report, cached,
+                                 {"pulse_block": 32, "merge_shapes": [3, 2176]})
+        log(f"autotuned in {time.perf_counter() - t:.1f} s:", json.dumps(tuned["config"]),
+            json.dumps(tuned["default_vs_tuned"]))
     for _ in range(args.warmup):
         step()
     synchronize()
@@ -321,6 +364,8 @@ def run_ffbp(args, scene):
             "cull": args.cull,
         },
     }
+    if tuned is not None:
+        result["autotune"] = tuned
     if prog.cull is not None:
         result["cull"] = dict(prog.cull, computed_fraction=round(
             prog.cull["computed_pixels"] / prog.cull["node_pixels"], 4))
@@ -329,6 +374,8 @@ def run_ffbp(args, scene):
         # the reference's randomized interleaved protocol; outputs compared
         # bit for bit
         full_prog, full_cart, full_step, _ = make(args.mode)
+        if tuned is not None:
+            full_prog.set_launch_shapes(prog.pulse_block, prog.merge_shapes)
         inter = interleaved_timing({"culled": step, "unculled": full_step},
                                    rounds=args.rounds, warmups=args.warmup)
         step()
@@ -677,8 +724,10 @@ def run_multipatch(args):
     cull = args.cull != "none" and args.mode == "split"
     prog = MultiPatchProgram(scenes, mode=args.mode, merge_interp="kaiser8",
                              persistent=args.persistent, cull=cull)
-    best, times = prog.autotune()
-    log(f"autotuned leaf tile width 2^{best}: {times}")
+    t = time.perf_counter()
+    tuned_cfg, tune_report, tune_cached = prog.autotune()
+    best, pulse_block = tuned_cfg["leaf"]
+    log(f"autotuned in {time.perf_counter() - t:.1f} s: {json.dumps(tuned_cfg)}")
     prog.capture_graph()
     for _ in range(args.warmup):
         prog.replay()
@@ -705,7 +754,8 @@ def run_multipatch(args):
         lat_p99 = float(np.percentile(lat, 99))
     seq_out, isolated, completion, progs = sequential_reference(scenes, mode=args.mode,
                                                                  merge_interp="kaiser8",
-                                                                 tile_cols_log2=best, cull=cull)
+                                                                 tile_cols_log2=best, cull=cull,
+                                                                 pulse_block=pulse_block)
     same = all(np.array_equal(x, y.get()) for x, y in zip(batch_out, seq_out))
     for _, p_, _, _ in progs:
         p_.close()
@@ -754,7 +804,8 @@ def run_multipatch(args):
             "sequential_isolated_p99_ms": round(float(np.percentile(isolated, 99)), 3),
             "sequential_gpix_pulse_per_s": round(work / (seq_total * 1e-3) / 1e9, 2),
             "batched_equals_sequential_bitwise": bool(same),
-            "autotuned_leaf_tile_cols_log2": best, "autotune_ms": times,
+            "autotune": autotune_summary(tuned_cfg, tune_report, tune_cached,
+                                         {"leaf": [6, 32], "merge_shapes": [3, 2176]}),
             "launches_per_batch": prog.launch_count},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -933,7 +984,7 @@ def main():
     }
     if "quality" in res:
         line["quality"] = res["quality"]
-    for extra in ("timing", "autofocus", "multipatch", "cull"):
+    for extra in ("timing", "autofocus", "multipatch", "cull", "autotune"):
         if extra in res:
             line[extra] = res[extra]
     if "precision_study" in res:

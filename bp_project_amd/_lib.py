@@ -21,6 +21,13 @@ MODE_CODES = {"fp64": MODE_FP64, "fp32": MODE_FP32, "mixed": MODE_MIXED,
 INTERP_CODES = {"lagrange4": 0, "linear": 1, "kaiser8": 2}
 ACCUMULATE, OUT_C128 = 1, 2
 KAISER_TABLE_LEN = 4 * 2048 + 2
+# Launch shapes accepted by bpmi_ffbp_leaves_multi_ex (split-kernel pulse
+# block depth) and bpmi_ffbp_merges_tiled_ex ((waves per SIMD hint, window
+# cap)); the first of each is the shape of the entry points without _ex.
+PULSE_BLOCKS = (32, 16, 64)
+MERGE_SHAPES = ((3, 2176), (5, 2000), (5, 1664), (7, 1088))
+DEFAULT_PULSE_BLOCK = PULSE_BLOCKS[0]
+DEFAULT_MERGE_SHAPE = MERGE_SHAPES[0]
 
 _vp, _i, _d, _ll, _sz = c_void_p, c_int, c_double, c_longlong, c_size_t
 
@@ -84,11 +91,13 @@ SIGNATURES = {
     "bpmi_split_tiles": [_i, _i, POINTER(c_int)],
     "bpmi_split_tiles_ex": [_i, _i, _i, POINTER(c_int)],
     "bpmi_ffbp_merges_tiled": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _vp],
+    "bpmi_ffbp_merges_tiled_ex": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _i, _i, _vp],
     "bpmi_merge_tiles": [_i, _i, POINTER(c_int)],
     "bpmi_ffbp_required_tiles": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "bpmi_eval_required_tiles": [_vp, _i, _i, _i, _i, _i, _i, _d, _d, _i, _d, _d, _i, _i, _vp,
                                  _vp],
     "bpmi_ffbp_leaves_multi": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp],
+    "bpmi_ffbp_leaves_multi_ex": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp],
     "bpmi_eval_grid_multi": [_vp, _vp, _i, _i, _i, _vp],
     "bpmi_eval_grid_multi_tiled": [_vp, _vp, _i, _i, _vp],
     "bpmi_eval_tiles": [_i, _i, POINTER(c_int)],

@@ -26,7 +26,7 @@ from . import _lib
 from .device import DeviceArray, synchronize
 from .gpu_backend import leaf_tile_width
 from .plan import NI_NR, NI_NU, flatten_plan
-from .resident_backend import ResidentProgram
+from .resident_backend import ResidentProgram, _Launcher
 
 _vp = _lib.c_void_p
 
@@ -78,10 +78,38 @@ def merge_block_table(node_int, nodes, mode, tiles=None):
             for s in range(0, int(node_int[n, NI_NR]) * int(node_int[n, NI_NU]), MERGE_BLOCK)]
 
 
+def check_launch_shapes(mode, pulse_block, merge_shapes, merge_heights):
+    """Validate a split-mode launch configuration; returns (pulse_block,
+    one merge shape per height), defaults filled in."""
+    if mode != "split" and (pulse_block is not None or merge_shapes is not None):
+        raise ValueError("launch shapes apply to split mode only")
+    pb = _lib.DEFAULT_PULSE_BLOCK if pulse_block is None else int(pulse_block)
+    if pb not in _lib.PULSE_BLOCKS:
+        raise ValueError(f"pulse_block must be one of {sorted(_lib.PULSE_BLOCKS)}")
+    if merge_shapes is None:
+        shapes = [_lib.DEFAULT_MERGE_SHAPE] * merge_heights
+    elif len(merge_shapes) == 2 and all(isinstance(v, (int, np.integer)) for v in merge_shapes):
+        shapes = [tuple(int(v) for v in merge_shapes)] * merge_heights
+    else:
+        shapes = [tuple(int(v) for v in sh) for sh in merge_shapes]
+        if len(shapes) != merge_heights:
+            raise ValueError(f"merge_shapes needs one shape per merge height ({merge_heights})")
+    for sh in shapes:
+        if sh not in _lib.MERGE_SHAPES:
+            raise ValueError(f"merge shape must be one of {list(_lib.MERGE_SHAPES)}")
+    return pb, shapes
+
+
 def enqueue_merges(node_int_ptr, node_geo_ptr, offsets_ptr, arena_ptr, blocks_ptr, nblocks, k,
-                   mode, merge_interp, stream):
-    """One merge launch over a block table built by ``merge_block_table``."""
+                   mode, merge_interp, stream, shape=None):
+    """One merge launch over a block table built by ``merge_block_table``;
+    ``shape`` = (waves, window_cap) of the split-mode tiled merge."""
     if mode == "split":
+        if shape is not None and tuple(shape) != _lib.DEFAULT_MERGE_SHAPE:
+            _lib.call("bpmi_ffbp_merges_tiled_ex", _vp(node_int_ptr), _vp(node_geo_ptr),
+                      _vp(offsets_ptr), _vp(arena_ptr), _vp(blocks_ptr), int(nblocks), float(k),
+                      _lib.interp_code(merge_interp), int(shape[0]), int(shape[1]), stream)
+            return
         _lib.call("bpmi_ffbp_merges_tiled", _vp(node_int_ptr), _vp(node_geo_ptr),
                   _vp(offsets_ptr), _vp(arena_ptr), _vp(blocks_ptr), int(nblocks), float(k),
                   _lib.interp_code(merge_interp), stream)
@@ -95,7 +123,11 @@ class BatchedResidentProgram(ResidentProgram):
     """ResidentProgram-compatible FFBP batched across each tree level."""
 
     def __init__(self, backend, plan, device, mode, range_start, data_dr, k, chunk_pulses=64,
-                 leaf_tile_cols_log2=None, persistent=False, cull=None):
+                 leaf_tile_cols_log2=None, persistent=False, cull=None, pulse_block=None,
+                 merge_shapes=None):
+        if persistent and (pulse_block is not None or merge_shapes is not None):
+            raise ValueError("launch shapes apply to the level launches, not the persistent "
+                             "schedule")
         if persistent and mode != "split":
             raise ValueError("the persistent schedule runs in split precision only")
         if persistent not in (False, True, "merges"):
@@ -107,6 +139,8 @@ class BatchedResidentProgram(ResidentProgram):
         self.computed_pixels = None  # {node: pixels computed} when culled
         self._table_buffers = []
         self._leaf_tc_request = leaf_tile_cols_log2
+        self._shape_request = (pulse_block, merge_shapes)
+        self.pulse_block = self.merge_shapes = None
         self._batch_stages = []
         self._persistent_request = persistent
         self.persistent = None
@@ -142,6 +176,9 @@ class BatchedResidentProgram(ResidentProgram):
         mode = _lib.mode_code(self.mode)
         tiles = computed = None
         self._cull_tiles = None
+        self._leaf_slot, self._merge_slots = None, []
+        if self.mode == "split" or self._shape_request != (None, None):
+            check_launch_shapes(self.mode, *self._shape_request, len(flat.levels) - 1)
         if self.mode == "split":
             tc = self._leaf_tc_request
             if tc is None:
@@ -168,11 +205,13 @@ class BatchedResidentProgram(ResidentProgram):
                     src["samples"] = d.data.shape[1]
                     sources = self._table(src.view(np.uint8))
                     blocks = self._table(np.asarray(mapping, dtype=np.int32))
-                    self._enqueue("bpmi_ffbp_leaves_multi", len(mapping) * MERGE_BLOCK, (
+                    leaf_args = (
                         _vp(sources.ptr), _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
                         _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr),
-                        len(mapping), mode, self._interp(), tc),
-                        name=f"batched_bp_polar_{self.mode}")
+                        len(mapping), mode, self._interp(), tc)
+                    self._enqueue("bpmi_ffbp_leaves_multi", len(mapping) * MERGE_BLOCK,
+                                  leaf_args, name=f"batched_bp_polar_{self.mode}")
+                    self._leaf_slot = (len(self._schedule) - 1, leaf_args)
                 else:
                     mapping = [(n, s) for n in nodes
                                for s in range(0, int(flat.node_int[n, NI_NR]) *
@@ -192,11 +231,13 @@ class BatchedResidentProgram(ResidentProgram):
                     mapping = [(-1, 0)]
                 blocks = self._table(np.asarray(mapping, dtype=np.int32))
                 if self.mode == "split":
-                    self._enqueue("bpmi_ffbp_merges_tiled", len(mapping) * MERGE_BLOCK, (
+                    merge_args = (
                         _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
                         _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr),
-                        len(mapping), self.k, self._merge_interp()),
-                        name=f"batched_merge_regrid_{self.mode}")
+                        len(mapping), self.k, self._merge_interp())
+                    self._enqueue("bpmi_ffbp_merges_tiled", len(mapping) * MERGE_BLOCK,
+                                  merge_args, name=f"batched_merge_regrid_{self.mode}")
+                    self._merge_slots.append((len(self._schedule) - 1, merge_args))
                 else:
                     self._enqueue("bpmi_ffbp_merges", len(mapping) * MERGE_BLOCK, (
                         _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
@@ -215,10 +256,95 @@ class BatchedResidentProgram(ResidentProgram):
                 "block_table_bytes": int(blocks.nbytes)})
         if self._persistent_request:
             self._replace_with_persistent(flat, sources)
+        elif self.mode == "split":
+            self.set_launch_shapes(*self._shape_request)
         root = flat.offsets[self.plan.root.index]
         # the root image is a view into the arena
         return DeviceArray((self.plan.root.nr * self.plan.root.nu,), np.complex64,
                            arena._alloc, arena._offset + int(root) * 8)
+
+    def set_launch_shapes(self, pulse_block=None, merge_shapes=None):
+        """Set the split-mode launch shapes of the built schedule in place:
+        the leaf kernel's pulse block depth and one (waves, window_cap)
+        merge shape per subtree height (a single pair applies to every
+        height; None = the defaults).  A captured graph is dropped (the
+        caller first waits for work that still uses it)."""
+        if self.persistent is not None or self._leaf_slot is None:
+            raise ValueError("launch shapes apply to split-mode level launches")
+        pb, shapes = check_launch_shapes(self.mode, pulse_block, merge_shapes,
+                                         len(self._merge_slots))
+        index, args = self._leaf_slot
+        if pb == _lib.DEFAULT_PULSE_BLOCK:
+            self._set_entry(index, "bpmi_ffbp_leaves_multi", args)
+        else:
+            self._set_entry(index, "bpmi_ffbp_leaves_multi_ex", args + (pb,))
+        for (index, args), shape in zip(self._merge_slots, shapes):
+            if shape == _lib.DEFAULT_MERGE_SHAPE:
+                self._set_entry(index, "bpmi_ffbp_merges_tiled", args)
+            else:
+                self._set_entry(index, "bpmi_ffbp_merges_tiled_ex", args + tuple(shape))
+        self.pulse_block, self.merge_shapes = pb, shapes
+        if self._graph is not None:
+            self._graph.close()
+            self._graph = None
+        if self._graph_stream is not None:
+            self._graph_stream.close()
+            self._graph_stream = None
+
+    def _set_entry(self, index, cname, args):
+        label, _, grid, block, _ = self._schedule[index]
+        launcher = _Launcher(cname)
+        self._schedule[index] = (label, launcher, grid, block, args)
+        self.operations[index].update({"kernel": launcher, "args": args})
+
+    def launch_stage_names(self):
+        """Stage names of the schedule's launches as ``autotune.stage_times``
+        reports them when every launch is marked with its label."""
+        from .autotune import stage_keys
+        return stage_keys([e[0] for e in self._schedule])
+
+    def autotune(self, execute=None, force=False, pulse_blocks=None, merge_shapes=None,
+                 warmup=1, repeats=3):
+        """Choose the leaf pulse block depth and one merge shape per subtree
+        height by timing (autotune.tune_config); the leaf tile width stays
+        as built.  ``execute(mark)`` runs one step and calls ``mark(label)``
+        after each launch (default: the schedule, marked with its labels;
+        a caller may add launches such as an evaluation).  Returns (config,
+        report, cached) and leaves the schedule on the chosen shapes."""
+        from .autotune import Group, make_key, stage_keys, stage_times, tune_config
+        if self.persistent is not None or self._leaf_slot is None:
+            raise ValueError("launch shapes apply to split-mode level launches")
+        if execute is None:
+            def execute(mark):
+                for label, launcher, grid, block, args in self._schedule:
+                    launcher(grid, block, args)
+                    if mark:
+                        mark(label)
+        names = stage_keys([e[0] for e in self._schedule])
+        leaf_stage = names[self._leaf_slot[0]]
+        merge_stages = [names[i] for i, _ in self._merge_slots]
+        groups = [Group("pulse_block", list(pulse_blocks or _lib.PULSE_BLOCKS), [leaf_stage]),
+                  Group("merge_shapes", [list(s) for s in (merge_shapes or _lib.MERGE_SHAPES)],
+                        merge_stages, per_stage=True)]
+        work = int(self.plan.counts["leaf_pixel_pulse_products"])
+        key = make_key("ffbp_levels", self.mode,
+                       f"{self.backend.interp}+{self.backend.merge_interp}", work,
+                       extra=f"h{len(merge_stages)}/tc{self.leaf_tile_cols_log2}"
+                             f"/{'cull' if self.cull else 'full'}")
+
+        def measure(config):
+            self.set_launch_shapes(config["pulse_block"], config["merge_shapes"])
+            synchronize()
+            return stage_times(execute, warmup, repeats)
+        base = {"pulse_block": self.pulse_block,
+                "merge_shapes": [list(s) for s in self.merge_shapes]}
+        if not groups[0].valid(base["pulse_block"]):
+            base["pulse_block"] = groups[0].candidates[0]
+        if not groups[1].valid(base["merge_shapes"]):
+            base["merge_shapes"] = [groups[1].candidates[0]] * len(merge_stages)
+        config, report, cached = tune_config(key, groups, measure, base, force=force)
+        self.set_launch_shapes(config["pulse_block"], config["merge_shapes"])
+        return config, report, cached
 
     def _cull(self, flat, arena, tile_cols_log2):
         """Required tiles of every node for the requested output region

@@ -380,6 +380,17 @@ extern "C" int bpmi_ffbp_leaves_multi(const bpmi_node_source* sources, const int
                                       const double* node_geo, const long long* offsets,
                                       void* arena, const int* blocks, int nblocks, int mode,
                                       int interp, int tile_cols_log2, void* stream) {
+    return bpmi_ffbp_leaves_multi_ex(sources, node_int, node_geo, offsets, arena, blocks, nblocks,
+                                     mode, interp, tile_cols_log2, SP_NB, stream);
+}
+
+extern "C" int bpmi_ffbp_leaves_multi_ex(const bpmi_node_source* sources, const int* node_int,
+                                         const double* node_geo, const long long* offsets,
+                                         void* arena, const int* blocks, int nblocks, int mode,
+                                         int interp, int tile_cols_log2, int pulse_block,
+                                         void* stream) {
+    if (pulse_block != 16 && pulse_block != 32 && pulse_block != 64)
+        return set_error(BPMI_ERR_INVALID, "pulse_block must be 16, 32 or 64");
     if (nblocks <= 0) return BPMI_OK;
     if (!sources || !node_int || !node_geo || !offsets || !arena || !blocks)
         return set_error(BPMI_ERR_INVALID, "null table");
@@ -397,12 +408,22 @@ extern "C" int bpmi_ffbp_leaves_multi(const bpmi_node_source* sources, const int
         if (interp != BPMI_INTERP_LAGRANGE4 && interp != BPMI_INTERP_LINEAR)
             return set_error(BPMI_ERR_UNSUPPORTED, "split mode supports lagrange4 and linear interpolation");
         SrcArgs s{};
-        if (interp == BPMI_INTERP_LAGRANGE4)
-            hipLaunchKernelGGL((bp_split_kernel<INTERP_LAGRANGE4, SRC_POLAR, true>), dim3(nblocks),
-                               dim3(SP_THREADS), 0, hs, a, s, tile_cols_log2, 0, lt);
+        auto launch = [&]<int NB>() {
+            if (interp == BPMI_INTERP_LAGRANGE4)
+                hipLaunchKernelGGL((bp_split_kernel<INTERP_LAGRANGE4, SRC_POLAR, true, NB>),
+                                   dim3(nblocks), dim3(SP_THREADS), 0, hs, a, s, tile_cols_log2,
+                                   0, lt);
+            else
+                hipLaunchKernelGGL((bp_split_kernel<INTERP_LINEAR, SRC_POLAR, true, NB>),
+                                   dim3(nblocks), dim3(SP_THREADS), 0, hs, a, s, tile_cols_log2,
+                                   0, lt);
+        };
+        if (pulse_block == 16)
+            launch.template operator()<16>();
+        else if (pulse_block == 64)
+            launch.template operator()<64>();
         else
-            hipLaunchKernelGGL((bp_split_kernel<INTERP_LINEAR, SRC_POLAR, true>), dim3(nblocks),
-                               dim3(SP_THREADS), 0, hs, a, s, tile_cols_log2, 0, lt);
+            launch.template operator()<32>();
         return check_launch("bp_split_kernel(multi-patch leaves)");
     }
     const float* kt = kaiser_table_device();

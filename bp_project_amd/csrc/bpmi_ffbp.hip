@@ -592,12 +592,16 @@ __global__ __launch_bounds__(256) void batched_merge_fast_kernel(
 // the vector cache.  A pixel whose stencil is not inside the window, or a
 // window above MT_CAP samples, reads global memory with identical
 // arithmetic, so results never depend on the path taken.
-// Launch bounds (256, 3) cap the kernel at 128 VGPRs: with ~35 KiB of LDS per
-// workgroup, 4 waves per SIMD is the occupancy either way, and tighter
-// register budgets (e.g. 67 VGPRs under (256, 4)) measured 14% slower.
+// The default launch shape: launch bounds (256, 3) and a window cap of
+// MT_CAP samples (~35 KiB of LDS per workgroup, so 4 waves per SIMD).  Other
+// shapes -- a smaller cap (less LDS, more resident workgroups, more tiles on
+// the global-memory path) and other occupancy hints -- are instantiated for
+// the autotuner; results do not depend on the shape (same arithmetic on
+// either path).
 constexpr int MT_ROWS = 32, MT_COLS = 32, MT_THREADS = 256;
 constexpr int MT_PPT = MT_ROWS * MT_COLS / MT_THREADS;
 constexpr int MT_CAP = 2176;
+constexpr int MT_WAVES = 3;
 
 
 struct Window {
@@ -689,32 +693,35 @@ __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& 
     return stencil_global<INTERP, COH>(f, c, kp);
 }
 
-// LDS of one merge tile: both child windows, the corner stencil origins,
-// the children's reciprocal spacings and a copy of the Kaiser weight
-// polynomials (read as vector operands, they free the 36 scalar registers
-// whose pressure otherwise spills scalars into vector lanes in the pixel loop).
-struct MergeLds {
-    float2 sh[2 * MT_CAP];
+// LDS of one merge tile: both child windows (CAP samples each), the corner
+// stencil origins, the children's reciprocal spacings and a copy of the
+// Kaiser weight polynomials (read as vector operands, they free the 36 scalar
+// registers whose pressure otherwise spills scalars into vector lanes in the
+// pixel loop).
+template <int CAP>
+struct MergeLdsCap {
+    float2 sh[2 * CAP];
     double2 row[MT_ROWS];  // per tile row: slant range r, ground range sqrt(r^2 - z^2)
     int corner[4][4];      // corner j: ir_a, iu_a, ir_b, iu_b
     double inv[4];         // 1/dr, 1/du of both children
     float kpoly[4 * KPOLY_STRIDE];
 };
+using MergeLds = MergeLdsCap<MT_CAP>;
 
 // One 32 x 32 tile of parent node ``node`` (all MT_THREADS threads call it;
 // every thread reaches both barriers).  kp_src: the Kaiser weight
 // polynomials (c_kaiser_poly).  A caller that runs several tiles in a row
 // must place a barrier between them.
-template <int INTERP, bool COH = false>
+template <int INTERP, bool COH = false, int CAP = MT_CAP>
 __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
                                            const double* __restrict__ node_geo,
                                            const long long* __restrict__ offsets, float2* arena,
                                            int node, int tile, double k, const float* kp_src,
-                                           MergeLds& lds) {
+                                           MergeLdsCap<CAP>& lds) {
     // The second window's base offset is opaque to the compiler, so the
     // stencil reads address it from a register rather than through
     // per-instruction offset additions (ds_read2 offsets reach 2 KiB).
-    int second = MT_CAP;
+    int second = CAP;
     asm volatile("" : "+s"(second));
     float2* const sha = lds.sh;
     float2* const shb = lds.sh + second;
@@ -784,8 +791,8 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         ira[j] = uniform_i32(lds.corner[j][0]); iua[j] = uniform_i32(lds.corner[j][1]);
         irb[j] = uniform_i32(lds.corner[j][2]); iub[j] = uniform_i32(lds.corner[j][3]);
     }
-    const Window wa = corner_window<INTERP>(ira, iua, fa.nr, fa.nu);
-    const Window wb = corner_window<INTERP>(irb, iub, fb.nr, fb.nu);
+    const Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
+    const Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
     if (wa.lds) stage_window<COH>(fa, wa, sha, wave, lane);
     if (wb.lds) stage_window<COH>(fb, wb, shb, wave, lane);
     __syncthreads();
@@ -809,16 +816,16 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     }
 }
 
-template <int INTERP>
-__global__ __launch_bounds__(MT_THREADS, 3) void batched_merge_tiled_kernel(
+template <int INTERP, int WAVES = MT_WAVES, int CAP = MT_CAP>
+__global__ __launch_bounds__(MT_THREADS, WAVES) void batched_merge_tiled_kernel(
         const int* __restrict__ node_int, const double* __restrict__ node_geo,
         const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
         double k) {
-    __shared__ MergeLds lds;
+    __shared__ MergeLdsCap<CAP> lds;
     const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
     if (node < 0) return;  // padding entry of a die-ordered table (uniform)
-    merge_tile<INTERP>(node_int, node_geo, offsets, arena, node, tile, k, &c_kaiser_poly[0][0],
-                       lds);
+    merge_tile<INTERP, false, CAP>(node_int, node_geo, offsets, arena, node, tile, k,
+                                   &c_kaiser_poly[0][0], lds);
 }
 
 // ---- required-tile marking ----------------------------------------------------
@@ -1559,6 +1566,34 @@ static void dispatch2(int mode, int interp, F&& f) {
 
 using namespace bpmi;
 
+// The merge launch shapes: (occupancy hint in waves per SIMD, window cap)
+// pairs.  The cap sets the LDS per workgroup (16 B per cap sample plus 752
+// B): 2176 -> 35568 B (4 workgroups per CU), 2000 -> 32752 B and 1664 ->
+// 27376 B (5), 1088 -> 18160 B (8); the hint caps the registers (Kaiser-8
+// keeps its 96 VGPRs up to 5 waves and spills beyond; Lagrange-4 needs 69,
+// which 7 waves allow), and the smaller of the two limits is the occupancy.
+// Measured on the 8192-pulse scene, window sizes (samples per child) are
+// ~1025-1092 at the lower levels, up to 1344 at the last but one, and
+// 1804 (median) to 2142 at the root: (5, 2000) keeps 85% of the root's
+// windows in LDS at 5 waves per SIMD.
+template <int INTERP>
+static bool launch_merge_shape(int waves, int cap, int nblocks, hipStream_t hs,
+                               const int* node_int, const double* node_geo,
+                               const long long* offsets, float2* ar, const int* blocks, double k) {
+#define BPMI_MERGE_SHAPE(W, C)                                                                  \
+    if (waves == W && cap == C) {                                                               \
+        hipLaunchKernelGGL((batched_merge_tiled_kernel<INTERP, W, C>), dim3(nblocks),           \
+                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k); \
+        return true;                                                                            \
+    }
+    BPMI_MERGE_SHAPE(3, 2176)
+    BPMI_MERGE_SHAPE(5, 2000)
+    BPMI_MERGE_SHAPE(5, 1664)
+    BPMI_MERGE_SHAPE(7, 1088)
+#undef BPMI_MERGE_SHAPE
+    return false;
+}
+
 extern "C" {
 
 int bpmi_set_kaiser_table(const double* host_table, int len) {
@@ -1688,19 +1723,32 @@ int bpmi_merge_tiles(int nr, int nu, int* tiles) {
 int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo, const long long* offsets,
                            void* arena, const int* blocks, int nblocks, double k, int interp,
                            void* stream) {
+    return bpmi_ffbp_merges_tiled_ex(node_int, node_geo, offsets, arena, blocks, nblocks, k,
+                                     interp, MT_WAVES, MT_CAP, stream);
+}
+
+int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
+                              const long long* offsets, void* arena, const int* blocks,
+                              int nblocks, double k, int interp, int waves, int window_cap,
+                              void* stream) {
     int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
     if (st) return st;
+    if (!((waves == 3 && window_cap == 2176) || (waves == 5 && window_cap == 2000) ||
+          (waves == 5 && window_cap == 1664) || (waves == 7 && window_cap == 1088)))
+        return set_error(BPMI_ERR_INVALID,
+                         "merge shape (waves, window_cap) must be one of (3, 2176), (5, 2000), "
+                         "(5, 1664), (7, 1088)");
     if (nblocks <= 0) return BPMI_OK;
     if (!node_int || !node_geo || !offsets || !arena || !blocks)
         return set_error(BPMI_ERR_INVALID, "null table");
     float2* ar = (float2*)arena;
     hipStream_t hs = (hipStream_t)stream;
     if (interp == BPMI_INTERP_KAISER8)
-        hipLaunchKernelGGL(batched_merge_tiled_kernel<INTERP_KAISER8>, dim3(nblocks),
-                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
+        launch_merge_shape<INTERP_KAISER8>(waves, window_cap, nblocks, hs, node_int, node_geo,
+                                           offsets, ar, blocks, k);
     else
-        hipLaunchKernelGGL(batched_merge_tiled_kernel<INTERP_LAGRANGE4>, dim3(nblocks),
-                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k);
+        launch_merge_shape<INTERP_LAGRANGE4>(waves, window_cap, nblocks, hs, node_int, node_geo,
+                                             offsets, ar, blocks, k);
     return check_launch("batched_merge_tiled_kernel");
 }
 

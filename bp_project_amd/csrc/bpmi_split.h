@@ -539,12 +539,16 @@ __device__ __forceinline__ void stage_pulse_block(const BpArgs& a, const PulsePa
     }
 }
 
-// LDS of one split tile: staged window coefficients of a pulse block, the
-// block's per-pulse parameters, the tile reference and a reduction scratch.
-template <int INTERP>
+// LDS of one split tile: staged window coefficients of a pulse block of NB
+// pulses, the block's per-pulse parameters, the tile reference and a
+// reduction scratch.  NB is a launch-shape choice (16, 32 or 64: one wave's
+// ballot covers a block); deeper blocks amortise the two barriers per block
+// over more pulses, shallower ones need less LDS per workgroup.
+template <int INTERP, int NB = SP_NB>
 struct SplitLds {
-    float4 coef[SP_NB * SP_W * SlotSize<INTERP>::F4];
-    PulseParams prm[SP_NB];
+    static_assert(NB >= 1 && NB <= 64, "a pulse block is at most one wave of flags");
+    float4 coef[NB * SP_W * SlotSize<INTERP>::F4];
+    PulseParams prm[NB];
     TileRef ref;
     double red[4][8];
 };
@@ -555,9 +559,9 @@ struct SplitLds {
 // the first barrier here is reached after LDS writes of the tile setup.
 // COH: the output is read by other workgroups of the same launch (device-
 // scope stores, see ld_c64/st_c64).
-template <int INTERP, int SRC, bool ORIGIN_REF, bool COH = false>
+template <int INTERP, int SRC, bool ORIGIN_REF, bool COH = false, int NB = SP_NB>
 __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, int tile,
-                                              SplitLds<INTERP>& lds) {
+                                              SplitLds<INTERP, NB>& lds) {
     constexpr int F4 = SlotSize<INTERP>::F4;
     float4* const coef = lds.coef;
     PulseParams* const prm = lds.prm;
@@ -643,8 +647,8 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
     const int vlo = InterpShape<INTERP>::FIRST < 0 ? -InterpShape<INTERP>::FIRST : 0;
     const int vhi = S - (InterpShape<INTERP>::TAPS + InterpShape<INTERP>::FIRST);  // last valid i
 
-    for (int nb0 = a.pulse_begin; nb0 < a.pulse_end; nb0 += SP_NB) {
-        const int nb = min(SP_NB, a.pulse_end - nb0);
+    for (int nb0 = a.pulse_begin; nb0 < a.pulse_end; nb0 += NB) {
+        const int nb = min(NB, a.pulse_end - nb0);
         // ---- per-pulse parameters (FP64, one thread per pulse) ------------
         if (t < nb) prm[t] = split_pulse_params<ORIGIN_REF>(a, ref, nb0 + t, kt, inv_ddr, vlo, vhi);
         __syncthreads();
@@ -782,10 +786,10 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
     }
 }
 
-template <int INTERP, int SRC, bool ORIGIN_REF>
+template <int INTERP, int SRC, bool ORIGIN_REF, int NB = SP_NB>
 __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs s, int tc_log2,
                                                               int tile_base, LeafTables lt) {
-    __shared__ SplitLds<INTERP> lds;
+    __shared__ SplitLds<INTERP, NB> lds;
     int tile = blockIdx.x + tile_base;
     if (lt.blocks) {
         // batched leaves: (node, tile) from the block table
@@ -799,7 +803,7 @@ __global__ __launch_bounds__(SP_THREADS) void bp_split_kernel(BpArgs a, SrcArgs 
         a.out = (float2*)a.partial + (long long)blockIdx.y * src_count(s, SRC);
         a.flags &= ~BPMI_ACCUMULATE;
     }
-    bp_split_tile<INTERP, SRC, ORIGIN_REF>(a, s, tc_log2, tile, lds);
+    bp_split_tile<INTERP, SRC, ORIGIN_REF, false, NB>(a, s, tc_log2, tile, lds);
 }
 
 }  // namespace bpmi

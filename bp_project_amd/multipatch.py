@@ -5,7 +5,9 @@ merges of every patch run one launch per subtree height, and every patch's
 Cartesian output is evaluated in one launch (split mode: LDS-tiled, 64 x 4
 pixel tiles); the whole batch is captured
 in one HIP graph, so a batch is one graph launch with no host round trips.
-The split-mode leaf tile width is chosen by the autotuner.  With
+In split mode the launch shapes (leaf tile width, the leaf kernel's pulse
+block depth, one merge shape per subtree height) are chosen by the
+autotuner.  With
 ``cull=True`` only the node tiles that each patch's Cartesian output depends
 on are launched (culling.py); the outputs are bitwise unchanged.
 
@@ -20,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Event, Graph, Stream, current_stream, elapsed_ms, stream_handle, synchronize
-from .batched_resident_backend import enqueue_merges, merge_block_table
+from .batched_resident_backend import check_launch_shapes, enqueue_merges, merge_block_table
 from .culling import merge_tiles_of, split_tiles_of
 from .gpu_backend import build_ffbp_plan, ensure_kaiser_table
 from .plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, flatten_plan
@@ -86,9 +88,13 @@ class MultiPatchProgram:
 
     def __init__(self, scenes, mode="split", leaf_size=16, range_spacing=0.5,
                  angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
-                 tile_cols_log2=6, persistent=False, cull=False):
+                 tile_cols_log2=6, persistent=False, cull=False, pulse_block=None,
+                 merge_shapes=None):
         if not scenes:
             raise ValueError("no patches")
+        if persistent and (pulse_block is not None or merge_shapes is not None):
+            raise ValueError("launch shapes apply to the level launches, not the persistent "
+                             "schedule")
         if cull and (mode != "split" or persistent == "merges"):
             raise ValueError("culling applies to split-mode schedules with leaves")
         ks = {round(s.k, 9) for s in scenes}
@@ -146,8 +152,12 @@ class MultiPatchProgram:
         self.t_offsets = DeviceArray.from_host(offsets)
         self.t_sources = DeviceArray.from_host(np.concatenate(sources).view(np.uint8))
         self._masks = self._required_masks() if cull else None
-        self.set_tile_width(tile_cols_log2)
         heights = self.node_int[:, NI_HEIGHT]
+        self.pulse_block, self.merge_shapes = (
+            check_launch_shapes(mode, pulse_block, merge_shapes, int(heights.max()))
+            if mode == "split" or pulse_block is not None or merge_shapes is not None
+            else (None, None))
+        self.set_tile_width(tile_cols_log2)
         self.merge_tables = []
         tiles = (None if self._masks is None else
                  {n: merge_tiles_of(m) for n, m in enumerate(self._masks)})
@@ -313,10 +323,11 @@ class MultiPatchProgram:
         self._launch_leaves(mode, s)
         if mark:
             mark("leaves")
-        for table in self.merge_tables:
+        for h, table in enumerate(self.merge_tables):
             enqueue_merges(self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
                            self.arena.ptr, table.ptr, table.shape[0], self.k, self.mode,
-                           self.merge_interp, s)
+                           self.merge_interp, s,
+                           shape=self.merge_shapes[h] if self.merge_shapes else None)
             if mark:
                 mark("merges")
         if self.mode == "split":
@@ -331,10 +342,13 @@ class MultiPatchProgram:
         return self.outputs
 
     def _launch_leaves(self, mode, s):
-        _lib.call("bpmi_ffbp_leaves_multi", _vp(self.t_sources.ptr), _vp(self.t_node_int.ptr),
-                  _vp(self.t_node_geo.ptr), _vp(self.t_offsets.ptr), _vp(self.arena.ptr),
-                  _vp(self.leaf_blocks.ptr), self.n_leaf_blocks, mode,
-                  _lib.interp_code(self.leaf_interp), self.tile_cols_log2, s)
+        args = (_vp(self.t_sources.ptr), _vp(self.t_node_int.ptr), _vp(self.t_node_geo.ptr),
+                _vp(self.t_offsets.ptr), _vp(self.arena.ptr), _vp(self.leaf_blocks.ptr),
+                self.n_leaf_blocks, mode, _lib.interp_code(self.leaf_interp), self.tile_cols_log2)
+        if self.pulse_block not in (None, _lib.DEFAULT_PULSE_BLOCK):
+            _lib.call("bpmi_ffbp_leaves_multi_ex", *args, self.pulse_block, s)
+        else:
+            _lib.call("bpmi_ffbp_leaves_multi", *args, s)
 
     def capture_graph(self):
         if self._graph is None:
@@ -352,24 +366,78 @@ class MultiPatchProgram:
         self._graph.launch()
         return self.outputs
 
-    def autotune(self, candidates=(4, 5, 6), force=False):
-        """Pick the split-mode leaf tile width (log2 columns) by timing."""
-        from .autotune import make_key, tune
-        if self.mode != "split":
-            return self.tile_cols_log2, {}
-        work = int(sum(pp.plan.counts["leaf_pixel_pulse_products"] for pp in self.patches))
-        kernel = "ffbp_persistent_multi" if self._persistent is not None else "ffbp_leaves_multi"
-        key = make_key(kernel, self.mode, self.leaf_interp, work)
+    def launch_config(self):
+        """The split-mode launch shapes in use: {"leaf": [tile_cols_log2,
+        pulse_block], "merge_shapes": [[waves, window_cap] per height]}."""
+        return {"leaf": [self.tile_cols_log2, self.pulse_block],
+                "merge_shapes": [list(sh) for sh in self.merge_shapes]}
 
-        def run(tc):
-            # the rebuild happens in the untimed warm-up call of a candidate
-            if tc != self.tile_cols_log2:
-                self.set_tile_width(tc)
-                synchronize()
-            self.execute()
-        best, times = tune(key, list(candidates), run, force=force)
-        self.set_tile_width(best)
-        return best, times
+    def set_launch_config(self, config):
+        """Apply a ``launch_config()``-shaped dict (level launches, split
+        mode); rebuilds the leaf table when the tile width changes and drops
+        a captured graph."""
+        if self.mode != "split" or self._persistent is not None:
+            raise ValueError("launch configurations apply to split-mode level launches")
+        tc, pb = (int(v) for v in config["leaf"])
+        pb, shapes = check_launch_shapes(self.mode, pb, config["merge_shapes"],
+                                         len(self.merge_tables))
+        changed = (pb, shapes) != (self.pulse_block, self.merge_shapes)
+        self.pulse_block, self.merge_shapes = pb, shapes
+        if tc != self.tile_cols_log2:
+            self.set_tile_width(tc)
+        elif changed and self._graph is not None:
+            self._graph.close()
+            self._graph = None
+
+    def autotune(self, force=False, tile_widths=(4, 5, 6), pulse_blocks=None,
+                 merge_shapes=None, warmup=1, repeats=3):
+        """Choose the split-mode launch shapes by timing (autotune.tune_config).
+
+        Two knob groups, each timed per launch stage with the other at its
+        current best: the leaf launch's (tile width, pulse block depth)
+        pairs, then the merge shapes, one chosen per subtree height from the
+        merge launch of that height.  Returns (config, report, cached) and
+        leaves the program on the chosen config.  A persistent schedule
+        tunes its leaf tile width only (one launch: whole-batch time).
+        """
+        from .autotune import Group, make_key, tune, tune_config
+        if self.mode != "split":
+            return None, {}, False
+        work = int(sum(pp.plan.counts["leaf_pixel_pulse_products"] for pp in self.patches))
+        if self._persistent is not None:
+            key = make_key("ffbp_persistent_multi", self.mode, self.leaf_interp, work)
+
+            def run(tc):
+                # the rebuild happens in the untimed warm-up call of a candidate
+                if tc != self.tile_cols_log2:
+                    self.set_tile_width(tc)
+                    synchronize()
+                self.execute()
+            best, times = tune(key, list(tile_widths), run, force=force)
+            self.set_tile_width(best)
+            return {"leaf": [best, None]}, {"leaf": {"times": times}}, False
+        pulse_blocks = tuple(pulse_blocks or _lib.PULSE_BLOCKS)
+        merge_shapes = tuple(merge_shapes or _lib.MERGE_SHAPES)
+        heights = len(self.merge_tables)
+        merge_stages = [f"merges{h}" for h in range(1, heights + 1)] if heights > 1 else ["merges"]
+        groups = [Group("leaf", [[tc, pb] for tc in tile_widths for pb in pulse_blocks],
+                        ["leaves"]),
+                  Group("merge_shapes", [list(sh) for sh in merge_shapes], merge_stages,
+                        per_stage=True)]
+        key = make_key("multipatch_levels", self.mode,
+                       f"{self.leaf_interp}+{self.merge_interp}", work, extra=f"h{heights}")
+
+        def measure(config):
+            from .autotune import stage_times
+            self.set_launch_config(config)
+            synchronize()
+            return stage_times(self.execute, warmup, repeats)
+        base = self.launch_config()
+        if not groups[0].valid(base["leaf"]):
+            base["leaf"] = groups[0].candidates[0]
+        config, report, cached = tune_config(key, groups, measure, base, force=force)
+        self.set_launch_config(config)
+        return config, report, cached
 
     def close(self):
         if self._graph is not None:
@@ -385,10 +453,11 @@ class MultiPatchProgram:
 
 def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
                          angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
-                         tile_cols_log2=6, cull=False):
+                         tile_cols_log2=6, cull=False, pulse_block=None):
     """Per-patch programs (the same kernels, one patch at a time): returns the
     outputs, each patch's isolated event time and its completion time when
-    the patches are launched back to back."""
+    the patches are launched back to back.  ``pulse_block`` is the leaf
+    kernel's pulse block depth (the merge shapes never change a result)."""
     from .batched_resident_backend import BatchedResidentProgram
     from .gpu_backend import HipBaseline, geo_array
     backend = HipBaseline(interp=leaf_interp, merge_interp=merge_interp)
@@ -398,7 +467,7 @@ def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
         dev = backend.upload(s.data, s.pos, s.weight, s.phase, pp.plan)
         prog = BatchedResidentProgram(backend, pp.plan, dev, mode, s.r_start, s.data_dr, s.k,
                                       leaf_tile_cols_log2=tile_cols_log2,
-                                      cull=s.grid if cull else None)
+                                      cull=s.grid if cull else None, pulse_block=pulse_block)
         out = DeviceArray.empty(s.grid.shape)
         progs.append((pp, prog, out, geo_array(pp.plan)))
 

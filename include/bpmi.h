@@ -267,6 +267,18 @@ int bpmi_split_tiles(int nr, int nu, int* tiles);
 int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo,
                            const long long* offsets, void* arena, const int* blocks,
                            int nblocks, double k, int interp, void* stream);
+/* bpmi_ffbp_merges_tiled with an explicit launch shape, for the autotuner:
+ * (waves, window_cap) = the occupancy hint of the launch bounds and the
+ * per-child LDS window cap in samples, one of (3, 2176) (the default shape of
+ * bpmi_ffbp_merges_tiled), (5, 2000), (5, 1664), (7, 1088); anything else
+ * is BPMI_ERR_INVALID.  Every shape computes the same values
+ * (a window above the cap is read from global memory with the same
+ * arithmetic).  The reference has no counterpart: it fixed its launch shapes
+ * by hand sweeps (resident_experiment.py:25-31). */
+int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
+                              const long long* offsets, void* arena, const int* blocks,
+                              int nblocks, double k, int interp, int waves, int window_cap,
+                              void* stream);
 int bpmi_merge_tiles(int nr, int nu, int* tiles);
 
 /* Required-tile marking for culled schedules (no reference counterpart: the
@@ -314,6 +326,17 @@ int bpmi_ffbp_leaves_multi(const bpmi_node_source* sources, const int* node_int,
                            const double* node_geo, const long long* offsets, void* arena,
                            const int* blocks, int nblocks, int mode, int interp,
                            int tile_cols_log2, void* stream);
+/* bpmi_ffbp_leaves_multi with the split kernel's pulse-block depth (16, 32
+ * or 64 pulses staged in LDS per block; 32 is the default of
+ * bpmi_ffbp_leaves_multi), for the autotuner; other values are
+ * BPMI_ERR_INVALID, other modes ignore it.  The depth regroups the pulses a
+ * tile accumulates per path, so outputs agree to FP32 rounding (bitwise when
+ * every pulse of a tile takes the same path).  No counterpart in the
+ * reference (hand-swept launch shapes, resident_experiment.py:25-31). */
+int bpmi_ffbp_leaves_multi_ex(const bpmi_node_source* sources, const int* node_int,
+                              const double* node_geo, const long long* offsets, void* arena,
+                              const int* blocks, int nblocks, int mode, int interp,
+                              int tile_cols_log2, int pulse_block, void* stream);
 int bpmi_split_tiles_ex(int nr, int nu, int tile_cols_log2, int* tiles);
 
 /* One Cartesian evaluation job per patch (64-bit fields first). */

@@ -1,6 +1,7 @@
 """One C2-sized FFBP for hardware-counter collection.
 
     python profile_merges.py [--pulses 8192] [--no-untiled] [--cull grid]
+                             [--merge-shapes W:C[,W:C...]]
 
 Runs the split-precision FFBP of the 8192-pulse x 8192-bin stripmap workload
 (leaf 16, factor 2, 8-tap Kaiser-sinc merges) once through the level-batched
@@ -40,7 +41,15 @@ def main(argv=None):
     ap.add_argument("--pulses", type=int, default=8192)
     ap.add_argument("--no-untiled", action="store_true")
     ap.add_argument("--cull", default="grid", choices=["none", "crop", "grid"])
+    ap.add_argument("--merge-shapes", default=None,
+                    help="W:C (every level) or one W:C per level, comma-separated: the "
+                         "merge launch shapes (occupancy hint, window cap)")
     a = ap.parse_args(argv)
+    shapes = None
+    if a.merge_shapes:
+        shapes = [tuple(int(v) for v in item.split(":")) for item in a.merge_shapes.split(",")]
+        if len(shapes) == 1:
+            shapes = shapes[0]
     t = time.perf_counter()
     s = workload_scene("c2_ffbp_full", pulses=a.pulses, bins=a.pulses)
     extent = polar_extent_for_grid(s.grid, s.pos.mean(axis=0), margin=2.0)
@@ -50,7 +59,8 @@ def main(argv=None):
     dev = backend.upload(s.data, s.pos, s.weight, s.phase, plan)
     cull = {"none": None, "crop": "crop", "grid": s.grid}[a.cull]
     prog = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
-                                  cull=cull)
+                                  cull=cull, merge_shapes=shapes)
+    print(f"merge shapes: {prog.merge_shapes}", flush=True)
     if prog.cull:
         print(f"cull: {prog.cull}", flush=True)
     print(f"setup {time.perf_counter() - t:.1f} s", flush=True)

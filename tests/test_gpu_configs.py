@@ -121,6 +121,23 @@ def test_c2_ffbp_full_vs_oracle(oracle):
     step()
     synchronize()
     assert np.array_equal(cart.get(), split), "two consecutive steps differ"
+    # The autotuner's launch shapes at full size, where the upper levels'
+    # windows (up to ~2050 samples) exceed the smaller caps and take the
+    # global-memory path: every merge shape is bitwise equal to the default;
+    # the other leaf pulse block depths agree to FP32 rounding.
+    from bp_project_amd import _lib
+    for shape in _lib.MERGE_SHAPES[1:]:
+        prog.set_launch_shapes(merge_shapes=shape)
+        step()
+        synchronize()
+        assert np.array_equal(cart.get(), split), shape
+    for pb in (16, 64):
+        prog.set_launch_shapes(pulse_block=pb)
+        step()
+        synchronize()
+        d = float(np.linalg.norm(cart.get() - split) / np.linalg.norm(split))
+        print(f"C2 pulse block {pb}: relative difference {d:.2e}", flush=True)
+        assert d <= 1e-6, (pb, d)
     prog.close()
     prog, cart, step, _ = make("fp64")
     step()

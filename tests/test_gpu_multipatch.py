@@ -14,7 +14,7 @@ def scenes():
 
 
 @pytest.mark.parametrize("mode", ["split", "mixed"])
-def test_multipatch_equals_sequential(scenes, mode):
+def test_multipatch_equals_sequential(scenes, mode, tmp_path, monkeypatch):
     from bp_project_amd.device import synchronize
     from bp_project_amd.multipatch import MultiPatchProgram, sequential_reference
     prog = MultiPatchProgram(scenes, mode=mode, merge_interp="kaiser8")
@@ -32,12 +32,14 @@ def test_multipatch_equals_sequential(scenes, mode):
         for a, o in zip(outs, prog.outputs):
             assert np.array_equal(a, o.get())
     if mode == "split":
-        best, times = prog.autotune(force=True)
-        assert best in (4, 5, 6)
+        monkeypatch.setenv("BPMI_TUNE_CACHE", str(tmp_path / "tune.json"))
+        config, _, _ = prog.autotune(force=True)
+        assert config["leaf"][0] in (4, 5, 6)
         prog.execute()
         synchronize()
         for a, o in zip(outs, prog.outputs):
-            # tile width changes only FP32 rounding of the leaf residuals
+            # leaf tile width and pulse block depth change only FP32 rounding
+            # of the leaf residuals; merge shapes change nothing
             assert np.linalg.norm(o.get() - a) <= 1e-5 * np.linalg.norm(a)
     prog.close()
 
