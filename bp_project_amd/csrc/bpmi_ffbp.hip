@@ -1568,30 +1568,44 @@ using namespace bpmi;
 
 // The merge launch shapes: (occupancy hint in waves per SIMD, window cap)
 // pairs.  The cap sets the LDS per workgroup (16 B per cap sample plus 752
-// B): 2176 -> 35568 B (4 workgroups per CU), 2000 -> 32752 B and 1664 ->
-// 27376 B (5), 1088 -> 18160 B (8); the hint caps the registers (Kaiser-8
+// B): 2176 -> 35568 B (4 workgroups per CU), 1664 -> 27376 B and 1920 ->
+// 31472 B (5), 1088 -> 18160 B (8); the hint caps the registers (Kaiser-8
 // keeps its 96 VGPRs up to 5 waves and spills beyond; Lagrange-4 needs 69,
 // which 7 waves allow), and the smaller of the two limits is the occupancy.
-// Measured on the 8192-pulse scene, window sizes (samples per child) are
-// ~1025-1092 at the lower levels, up to 1344 at the last but one, and
-// 1804 (median) to 2142 at the root: (5, 2000) keeps 85% of the root's
-// windows in LDS at 5 waves per SIMD.
+// On the 8192-pulse scene the windows (samples per child) are ~1025-1092 at
+// the lower levels, up to 1344 at the last but one and 1804 (median) to 2142
+// at the root, so the 5-wave shapes win below the root (~10% per launch) and
+// the default shape at the root, where a tile whose window exceeds the cap
+// reads global memory (a 32752 B workgroup, cap 2000, measured like 4
+// waves: it does not fit 5 per CU).
+#define BPMI_MERGE_SHAPE_LIST(X) X(3, 2176) X(5, 1664) X(5, 1920) X(7, 1088)
+
+static bool merge_shape_known(int waves, int cap) {
+#define BPMI_MERGE_SHAPE(W, C) \
+    if (waves == W && cap == C) return true;
+    BPMI_MERGE_SHAPE_LIST(BPMI_MERGE_SHAPE)
+#undef BPMI_MERGE_SHAPE
+    return false;
+}
+
+static const char* merge_shape_names() {
+#define BPMI_MERGE_SHAPE(W, C) " (" #W ", " #C ")"
+    return BPMI_MERGE_SHAPE_LIST(BPMI_MERGE_SHAPE);
+#undef BPMI_MERGE_SHAPE
+}
+
 template <int INTERP>
-static bool launch_merge_shape(int waves, int cap, int nblocks, hipStream_t hs,
+static void launch_merge_shape(int waves, int cap, int nblocks, hipStream_t hs,
                                const int* node_int, const double* node_geo,
                                const long long* offsets, float2* ar, const int* blocks, double k) {
 #define BPMI_MERGE_SHAPE(W, C)                                                                  \
     if (waves == W && cap == C) {                                                               \
         hipLaunchKernelGGL((batched_merge_tiled_kernel<INTERP, W, C>), dim3(nblocks),           \
                            dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k); \
-        return true;                                                                            \
+        return;                                                                                 \
     }
-    BPMI_MERGE_SHAPE(3, 2176)
-    BPMI_MERGE_SHAPE(5, 2000)
-    BPMI_MERGE_SHAPE(5, 1664)
-    BPMI_MERGE_SHAPE(7, 1088)
+    BPMI_MERGE_SHAPE_LIST(BPMI_MERGE_SHAPE)
 #undef BPMI_MERGE_SHAPE
-    return false;
 }
 
 extern "C" {
@@ -1733,11 +1747,9 @@ int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
                               void* stream) {
     int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
     if (st) return st;
-    if (!((waves == 3 && window_cap == 2176) || (waves == 5 && window_cap == 2000) ||
-          (waves == 5 && window_cap == 1664) || (waves == 7 && window_cap == 1088)))
-        return set_error(BPMI_ERR_INVALID,
-                         "merge shape (waves, window_cap) must be one of (3, 2176), (5, 2000), "
-                         "(5, 1664), (7, 1088)");
+    if (!merge_shape_known(waves, window_cap))
+        return set_error(BPMI_ERR_INVALID, "merge shape (waves, window_cap) must be one of%s",
+                         merge_shape_names());
     if (nblocks <= 0) return BPMI_OK;
     if (!node_int || !node_geo || !offsets || !arena || !blocks)
         return set_error(BPMI_ERR_INVALID, "null table");

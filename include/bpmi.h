@@ -270,7 +270,7 @@ int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo,
 /* bpmi_ffbp_merges_tiled with an explicit launch shape, for the autotuner:
  * (waves, window_cap) = the occupancy hint of the launch bounds and the
  * per-child LDS window cap in samples, one of (3, 2176) (the default shape of
- * bpmi_ffbp_merges_tiled), (5, 2000), (5, 1664), (7, 1088); anything else
+ * bpmi_ffbp_merges_tiled), (5, 1664), (5, 1920), (7, 1088); anything else
  * is BPMI_ERR_INVALID.  Every shape computes the same values
  * (a window above the cap is read from global memory with the same
  * arithmetic).  The reference has no counterpart: it fixed its launch shapes

@@ -148,8 +148,8 @@ def test_launch_shape_validation():
     assert pb == _lib.DEFAULT_PULSE_BLOCK == 32
     assert shapes == [_lib.DEFAULT_MERGE_SHAPE] * 3 == [(3, 2176)] * 3
     assert check_launch_shapes("split", 64, (5, 1664), 2) == (64, [(5, 1664)] * 2)
-    assert check_launch_shapes("split", 16, [[7, 1088], [5, 2000]], 2)[1] == [(7, 1088),
-                                                                               (5, 2000)]
+    assert check_launch_shapes("split", 16, [[7, 1088], [5, 1920]], 2)[1] == [(7, 1088),
+                                                                               (5, 1920)]
     with pytest.raises(ValueError, match="pulse_block must be one of"):
         check_launch_shapes("split", 48, None, 2)
     with pytest.raises(ValueError, match="merge shape must be one of"):
@@ -160,3 +160,25 @@ def test_launch_shape_validation():
         check_launch_shapes("mixed", 16, None, 2)
     assert set(_lib.PULSE_BLOCKS) == {16, 32, 64}
     assert len(set(_lib.MERGE_SHAPES)) == len(_lib.MERGE_SHAPES) == 4
+
+
+def test_c_abi_shape_lists_match_the_python_constants():
+    """The library validates launch shapes before touching a device; its
+    messages list exactly the shapes the Python side offers the tuner."""
+    import ctypes
+    lib = _lib.load()
+    one = ctypes.c_void_p(1)
+    # lagrange4 merges need no Kaiser table, so the shape check is the first
+    # to fail
+    st = lib.bpmi_ffbp_merges_tiled_ex(one, one, one, one, one, 1, 1.0,
+                                       _lib.INTERP_CODES["lagrange4"], 4, 2176, None)
+    assert st == _lib.ERR_INVALID
+    msg = _lib.last_error()
+    listed = [tuple(int(v) for v in m) for m in
+              __import__("re").findall(r"\((\d+), (\d+)\)", msg.split("one of", 1)[1])]
+    assert listed == list(_lib.MERGE_SHAPES)
+    st = lib.bpmi_ffbp_leaves_multi_ex(one, one, one, one, one, one, 1, _lib.MODE_SPLIT,
+                                       _lib.INTERP_CODES["lagrange4"], 6, 48, None)
+    assert st == _lib.ERR_INVALID
+    assert _lib.last_error() == "pulse_block must be 16, 32 or 64"
+    assert sorted(_lib.PULSE_BLOCKS) == [16, 32, 64]

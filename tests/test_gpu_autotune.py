@@ -141,7 +141,7 @@ def test_shape_arguments_rejected_at_the_boundary():
     from bp_project_amd import _lib
     lib = _lib.load()
     one = ctypes.c_void_p(1)
-    st = lib.bpmi_ffbp_merges_tiled_ex(one, one, one, one, one, 1, 1.0, 2, 4, 2176, None)
+    st = lib.bpmi_ffbp_merges_tiled_ex(one, one, one, one, one, 1, 1.0, 0, 4, 2176, None)
     assert st == _lib.ERR_INVALID and "merge shape" in _lib.last_error()
     st = lib.bpmi_ffbp_leaves_multi_ex(one, one, one, one, one, one, 1, 4, 0, 6, 48, None)
     assert st == _lib.ERR_INVALID and "pulse_block must be 16, 32 or 64" in _lib.last_error()
