@@ -366,6 +366,11 @@ def run_ffbp(args, scene):
     }
     if tuned is not None:
         result["autotune"] = tuned
+        if "merge" in dominant:
+            # the merge launches run the autotuned <interp, waves, cap> instance per level
+            result["roofline"]["kernel_instances"] = [
+                f"batched_merge_tiled_kernel<{args.merge_interp}, {w}, {c}>"
+                for w, c in prog.merge_shapes]
     if prog.cull is not None:
         result["cull"] = dict(prog.cull, computed_fraction=round(
             prog.cull["computed_pixels"] / prog.cull["node_pixels"], 4))
