@@ -185,6 +185,95 @@ def interleaved_timing(variants, rounds=15, warmups=2, seed=20260917):
     return out
 
 
+_PEAKS = None
+
+
+def measured_peaks(blocks=2048, iters=2048, copy_bytes=1 << 30, repeats=5):
+    """The peaks the roofline divides by, re-measured on this GPU (HIP events,
+    median of ``repeats`` after a warm-up): the FP32 FMA issue rate of
+    ``fma_peak_kernel`` (vector FMAs, packed FP32 not used, as in every
+    kernel of this build) and a device copy of ``copy_bytes`` (``bpmi_crop``
+    of a complex64 image: that many bytes read and written).  Cached per
+    process."""
+    global _PEAKS
+    if _PEAKS is not None:
+        return _PEAKS
+    from bp_project_amd import _lib
+    from bp_project_amd.device import DeviceArray, Event, elapsed_ms, stream_handle, synchronize
+    vp = _lib.c_void_p
+    out = DeviceArray.empty((blocks * 256,), np.float32)
+    rows, cols = copy_bytes // (8 * 8192), 8192
+    src, dst = DeviceArray.empty((rows, cols)), DeviceArray.empty((rows, cols))
+    src.fill_zero()
+
+    def fma():
+        _lib.call("bpmi_probe_fma_peak", vp(out.ptr), blocks, iters, stream_handle())
+
+    def copy():
+        _lib.call("bpmi_crop", vp(src.ptr), cols, 0, 0, rows, cols, vp(dst.ptr), stream_handle())
+
+    def timed(fn):
+        fn()
+        synchronize()
+        ms = []
+        for _ in range(repeats):
+            a, b = Event(), Event()
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ms.append(elapsed_ms(a, b))
+        return float(np.median(ms))
+    t_fma, t_copy = timed(fma), timed(copy)
+    flops = blocks * 256 * iters * 16 * 8 * 2
+    _PEAKS = {"fp32_fma_tflops": round(flops / (t_fma * 1e-3) / 1e12, 2),
+              "copy_gbs": round(2 * rows * cols * 8 / (t_copy * 1e-3) / 1e9, 1),
+              "method": f"fma_peak_kernel ({blocks} x 256 threads, 8 independent v_fma_f32 "
+                        f"chains, {flops:.3g} FLOPs) and a {copy_bytes >> 20} MiB device copy "
+                        "(read + write), HIP events, median of 5"}
+    del out, src, dst
+    return _PEAKS
+
+
+def attach_measured(roof):
+    """Add the re-measured peaks to a roofline record and the achieved rate's
+    fraction of them (FP32 FMA rate for TFLOP/s records, copy bandwidth for
+    GB/s records)."""
+    if not roof:
+        return roof
+    pk = measured_peaks()
+    roof.setdefault("measured_peak", pk)
+    if roof.get("unit") == "TFLOP/s":
+        roof["frac_of_measured_fp32"] = round(roof["achieved"] / pk["fp32_fma_tflops"], 4)
+    elif roof.get("unit") == "GB/s":
+        roof["frac_of_measured_copy"] = round(roof["achieved"] / pk["copy_gbs"], 4)
+    return roof
+
+
+def step_attribution(stages, steps, parts):
+    """The FFBP step's roofline attributed per stage: for each stage the time
+    its algorithmic ops need at the FP32 vector peak and its bytes need at
+    the HBM peak, the larger of the two, summed over the stages, against the
+    measured step.  ``parts``: {stage: (ops per step, bytes per step)}."""
+    out, ideal = {}, 0.0
+    for name, (ops, nbytes) in parts.items():
+        if name not in stages:
+            continue
+        t_ops = ops / (PEAK_FP32_TFLOPS * 1e12) * 1e3
+        t_bytes = nbytes / (PEAK_HBM_GBS * 1e9) * 1e3
+        ms = stages[name]["ms_per_step"]
+        out[name] = {"ops": float(ops), "bytes": float(nbytes), "bound_ms": round(max(t_ops, t_bytes), 4),
+                     "bound": "valu" if t_ops >= t_bytes else "hbm", "measured_ms": round(ms, 4),
+                     "frac": round(max(t_ops, t_bytes) / ms, 4)}
+        ideal += max(t_ops, t_bytes)
+    measured = sum(stages[n]["ms_per_step"] for n in out)
+    return {"stages": out, "bound_ms": round(ideal, 4), "measured_ms": round(measured, 4),
+            "frac": round(ideal / measured, 4) if measured else None,
+            "method": "sum over stages of max(ops / FP32 peak, bytes / HBM peak) over the "
+                      "measured stage times (merges: measured HBM bytes; leaves, evaluation: "
+                      "compulsory bytes)"}
+
+
 def autotune_summary(config, report, cached, defaults, per_stage=("merge_shapes",)):
     """The tuner's choice, its per-candidate stage times (ms, medians of
     event-timed runs) and, per knob group, the timed stages' total under the
@@ -364,6 +453,20 @@ def run_ffbp(args, scene):
             "cull": args.cull,
         },
     }
+    attach_measured(result["roofline"])
+    leaf_px = int(sum(nodes_px[i] for i in flat.levels[0]))
+    merge_traffic = MEASURED_TRAFFIC_BYTES.get((args.workload, args.mode,
+                                                STAGE_KERNEL.get(f"batched_merge_regrid_{args.mode}"),
+                                                args.cull))
+    result["roofline"]["step"] = step_attribution(per_stage, args.steps, {
+        f"batched_bp_polar_{args.mode}": (
+            leaf_pp * OPS_LEAF_LAGRANGE4, 8 * leaf_px + 8 * scene.pulses * scene.bins),
+        f"batched_merge_regrid_{args.mode}": (
+            merge_px * ops_merge,
+            (merge_traffic * (len(flat.levels) - 1)) if merge_traffic else 16 * merge_px),
+        "eval_grid": (g.nx * g.ny * OPS_EVAL_KAISER8,
+                      8 * g.nx * g.ny + 8 * int(nodes_px[plan.root.index])),
+    })
     if tuned is not None:
         result["autotune"] = tuned
         if "merge" in dominant:
@@ -985,7 +1088,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(res["ms_per_step"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": res["config"], "roofline": res["roofline"], "cpu_baseline": cpu,
+        "config": res["config"], "roofline": attach_measured(res["roofline"]), "cpu_baseline": cpu,
     }
     if "quality" in res:
         line["quality"] = res["quality"]
@@ -1029,8 +1132,10 @@ def other_workloads(args):
                "ms_per_step": round(res["ms_per_step"], 4), "steps": sub.steps,
                "config": res["config"]}
         if res.get("roofline"):
-            rec["roofline"] = {k: res["roofline"][k] for k in ("bound", "achieved", "peak",
-                                                               "unit", "frac", "kernel")}
+            roof = attach_measured(res["roofline"])
+            rec["roofline"] = {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac",
+                                                    "frac_of_measured_fp32", "kernel")
+                               if k in roof}
         if cpu:
             rec["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind",
                                                        "sample")}

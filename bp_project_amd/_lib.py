@@ -112,6 +112,7 @@ SIGNATURES = {
     "bpmi_bp_points_adjoint": [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _ll, _d, _d, _d, _i, _vp, _vp,
                                _sz, _vp, _vp, _vp],
     "bpmi_probe_sincos_turns": [_vp, _vp, _vp, _ll, _vp],
+    "bpmi_probe_fma_peak": [_vp, _i, _i, _vp],
     "bpmi_range_compress_workspace_bytes": [_i, _i, _d, _d, _i, _i, POINTER(c_size_t)],
     "bpmi_range_compress": [_vp, _i, _i, _ll, _d, _d, _d, _i, _i, _i, _vp, _vp, _sz, _vp],
     "bpmi_decode_iq16": [_vp, _ll, _ll, _ll, _i, _vp, _ll, _vp, _vp],

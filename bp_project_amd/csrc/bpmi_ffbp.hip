@@ -1207,6 +1207,27 @@ __global__ void sincos_probe_kernel(const float* t, float* s, float* c, long lon
     sincos_turns(t[p], &s[p], &c[p]);
 }
 
+// FP32 FMA issue-rate probe: each thread runs FMA_CHAINS independent chains
+// x <- x * m + c, 16 steps per iteration, and stores their sum (so nothing
+// is folded away).  FLOPs per launch: blocks * 256 * iters * 16 * chains * 2.
+constexpr int FMA_CHAINS = 8;
+__global__ __launch_bounds__(256) void fma_peak_kernel(float* out, int iters, float m, float c) {
+    float a[FMA_CHAINS];
+#pragma unroll
+    for (int j = 0; j < FMA_CHAINS; ++j) a[j] = (float)(threadIdx.x + j) * 1e-3f;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+#pragma unroll
+            for (int j = 0; j < FMA_CHAINS; ++j) a[j] = fmaf(a[j], m, c);
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < FMA_CHAINS; ++j) s += a[j];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 // ---- persistent FFBP: one launch for leaves, every merge level and the
 // Cartesian evaluation ---------------------------------------------------------
 // Workgroups claim work items in list order from one ticket counter.  The
@@ -2109,6 +2130,14 @@ int bpmi_probe_sincos_turns(const float* t, float* s, float* c, long long n, voi
     hipLaunchKernelGGL(sincos_probe_kernel, dim3(grid_1d(n, 256)), dim3(256), 0,
                        (hipStream_t)stream, t, s, c, n);
     return check_launch("sincos_probe_kernel");
+}
+
+int bpmi_probe_fma_peak(float* out, int blocks, int iters, void* stream) {
+    if (!out) return set_error(BPMI_ERR_INVALID, "null argument");
+    if (blocks <= 0 || iters <= 0) return set_error(BPMI_ERR_INVALID, "blocks and iters must be positive");
+    hipLaunchKernelGGL(fma_peak_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out,
+                       iters, 0.999f, 1e-3f);
+    return check_launch("fma_peak_kernel");
 }
 
 }  // extern "C"

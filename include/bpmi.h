@@ -474,6 +474,11 @@ int bpmi_decode_iq16(const void* frames, long long nframes, long long frame_byte
 int bpmi_set_kaiser_table(const double* host_table, int len);
 /* sin/cos(2*pi*t) with the hardware transcendental path (device arrays). */
 int bpmi_probe_sincos_turns(const float* t, float* s, float* c, long long n, void* stream);
+/* FP32 FMA issue-rate probe (measures the vector FP32 peak this build can
+ * reach; packed FP32 is not used): ``blocks`` workgroups of 256 threads, each
+ * thread 8 independent FMA chains of iters x 16 steps, one float stored per
+ * thread into out[blocks * 256].  FLOPs = blocks * 256 * iters * 256. */
+int bpmi_probe_fma_peak(float* out, int blocks, int iters, void* stream);
 
 #ifdef __cplusplus
 }
