@@ -18,6 +18,8 @@ kernels: results are bitwise equal to running each patch's
 """
 from __future__ import annotations
 
+import json
+
 import numpy as np
 
 from . import _lib
@@ -415,7 +417,11 @@ class MultiPatchProgram:
                 self.execute()
             best, times = tune(key, list(tile_widths), run, force=force)
             self.set_tile_width(best)
-            return {"leaf": [best, None]}, {"leaf": {"times": times}}, False
+            # the report in tune_config's shape: one stage (the whole launch)
+            report = {"leaf": {"stages": ["batch"], "choice": [best, None],
+                               "times": {json.dumps([int(k), None]): [float(v)]
+                                         for k, v in times.items()}}}
+            return {"leaf": [best, None]}, report, False
         pulse_blocks = tuple(pulse_blocks or _lib.PULSE_BLOCKS)
         merge_shapes = tuple(merge_shapes or _lib.MERGE_SHAPES)
         heights = len(self.merge_tables)

@@ -182,3 +182,21 @@ def test_c_abi_shape_lists_match_the_python_constants():
     assert st == _lib.ERR_INVALID
     assert _lib.last_error() == "pulse_block must be 16, 32 or 64"
     assert sorted(_lib.PULSE_BLOCKS) == [16, 32, 64]
+
+
+def test_bench_autotune_summary_handles_both_report_shapes():
+    """The bench's tuner record accepts the coordinate-search report and the
+    single-knob report of a persistent program (one whole-launch stage)."""
+    import bench
+    calls = []
+    cfg, report = at.coordinate_search(groups(), cost_model(calls),
+                                       {"leaf": [6, 32], "merges": ["A", "A"]})
+    out = bench.autotune_summary(cfg, report, False, {"leaf": [6, 32], "merges": "A"},
+                                 per_stage=("merges",))
+    assert out["default_vs_tuned"]["merges"] == {"default_ms": 5.0, "tuned_ms": 3.5}
+    assert out["default_vs_tuned"]["leaf"]["tuned_ms"] <= out["default_vs_tuned"]["leaf"]["default_ms"]
+    persistent = {"leaf": {"stages": ["batch"], "choice": [5, None],
+                           "times": {json.dumps([4, None]): [3.0], json.dumps([5, None]): [2.0]}}}
+    out = bench.autotune_summary({"leaf": [5, None]}, persistent, False, {"leaf": [6, 32]})
+    assert out["candidates_ms"]["leaf"]["times"] == {"[4, null]": [3.0], "[5, null]": [2.0]}
+    assert out["default_vs_tuned"] == {}
