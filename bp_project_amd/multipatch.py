@@ -431,7 +431,8 @@ class MultiPatchProgram:
                   Group("merge_shapes", [list(sh) for sh in merge_shapes], merge_stages,
                         per_stage=True)]
         key = make_key("multipatch_levels", self.mode,
-                       f"{self.leaf_interp}+{self.merge_interp}", work, extra=f"h{heights}")
+                       f"{self.leaf_interp}+{self.merge_interp}", work,
+                       extra=f"h{heights}/{'cull' if self._masks is not None else 'full'}")
 
         def measure(config):
             from .autotune import stage_times
