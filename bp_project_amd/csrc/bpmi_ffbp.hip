@@ -648,11 +648,33 @@ __device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w, in
 }
 
 // Copy the window into LDS: wave v takes rows v, v + 4, .., lane j columns
-// j, j + 64, ..; per-row steps are uniform pointer increments.
+// j, j + 64, ..; per-row steps are uniform pointer increments.  Windows up
+// to 64 columns (the usual case) load STAGE_ROWS rows before storing any, so
+// a wave waits for one batch of loads instead of once per row.
+constexpr int STAGE_ROWS = 4;
+
+// A complex64 load from an image in global memory (never LDS).  The address
+// space cast keeps it a global load where the pointer was read from a table
+// in memory: as a flat load it could alias LDS, and the compiler would order
+// it behind every LDS access of the wave.
+typedef __attribute__((address_space(1))) const unsigned long long global_u64;
+template <bool COH = false>
+__device__ __forceinline__ float2 ld_image_c64(const float2* p) {
+    if constexpr (COH) {
+        return ld_c64<true>(p);
+    } else {
+        // loaded as one 64-bit word: a float2 copy would go through a
+        // generic reference and lose the address space again
+        const unsigned long long u = *(const global_u64*)p;
+        return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
+    }
+}
+
 template <bool COH = false>
 __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, float2* dst,
                                              int wave, int lane) {
     constexpr int WAVES = MT_THREADS / 64;
+    wave = uniform_i32(wave);  // the row tests below are then scalar branches
     if (wave >= w.h) return;
     const float2* src = f.img + (long long)(w.r0 + wave) * f.ld + w.c0;
     const long long src_step = (long long)WAVES * f.ld;
@@ -660,16 +682,26 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
     const int dst_step = WAVES * w.w;
     if (w.w <= 64) {
         if (lane < w.w) {
-            for (int i = wave; i < w.h; i += WAVES) {
-                d[lane] = ld_c64<COH>(src + lane);
-                src += src_step;
-                d += dst_step;
+            for (int i = wave; i < w.h; i += STAGE_ROWS * WAVES) {
+                // rows i + b * WAVES of this batch: nb of them lie in the
+                // window; the loads of the others repeat its last row (no
+                // branch around a load), only the nb rows are stored
+                const int nb = min(STAGE_ROWS, (w.h - i + WAVES - 1) / WAVES);
+                float2 v[STAGE_ROWS];
+#pragma unroll
+                for (int b = 0; b < STAGE_ROWS; ++b)
+                    v[b] = ld_image_c64<COH>(src + min(b, nb - 1) * src_step + lane);
+#pragma unroll
+                for (int b = 0; b < STAGE_ROWS; ++b)
+                    if (b < nb) d[b * dst_step + lane] = v[b];
+                src += STAGE_ROWS * src_step;
+                d += STAGE_ROWS * dst_step;
             }
         }
         return;
     }
     for (int i = wave; i < w.h; i += WAVES) {
-        for (int j = lane; j < w.w; j += 64) d[j] = ld_c64<COH>(src + j);
+        for (int j = lane; j < w.w; j += 64) d[j] = ld_image_c64<COH>(src + j);
         src += src_step;
         d += dst_step;
     }
