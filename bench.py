@@ -103,6 +103,9 @@ def parse():
     p.add_argument("--persistent", action="store_true",
                    help="FFBP workloads: one persistent launch for leaves and all merge levels "
                         "(C5: and the evaluation)")
+    p.add_argument("--latency-groups", default="0.25,0.5,0.75",
+                   help="C5: also serve the batch in latency groups cut at these fractions "
+                        "of the patches ordered by work ('none': skip)")
     return p.parse_args()
 
 
@@ -894,6 +897,10 @@ def run_multipatch(args):
                                     for k, v in stages.items()},
             "note": "dominant stage of the batch (merge launches of every height); events "
                     "between the launches of eager executions"}
+    groups = None
+    if not args.persistent and args.mode == "split" and args.latency_groups != "none":
+        groups = latency_groups(scenes, batch_out, (best, pulse_block), cull, args,
+                                tuple(float(f) for f in args.latency_groups.split(",")))
     cpu = None if args.no_cpu else cpu_baseline_multipatch(scenes)
     return {
         "ms_per_step": ms_batch,
@@ -914,13 +921,57 @@ def run_multipatch(args):
             "batched_equals_sequential_bitwise": bool(same),
             "autotune": autotune_summary(tuned_cfg, tune_report, tune_cached,
                                          {"leaf": [6, 32], "merge_shapes": [3, 2176]}),
-            "launches_per_batch": prog.launch_count},
+            "launches_per_batch": prog.launch_count,
+            "latency_groups": groups},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "config": {"workload": "c5_multipatch", "patches": len(scenes), "mode": args.mode,
                    "merge_interp": "kaiser8", "graph": True, "persistent": bool(args.persistent),
                    "cull": "grid" if cull else "none"},
     }
+
+
+def latency_groups(scenes, batch_out, leaf, cull, args, fractions=(0.25, 0.5, 0.75)):
+    """The same batch served in latency groups (GroupedMultiPatch: patches by
+    work, cut at ``fractions`` of the count, one group after another in one
+    HIP graph) with the main batch's leaf shapes, so the outputs are
+    the main batch's bit for bit; per-patch completion from events after
+    each group of eager executions."""
+    from bp_project_amd.device import Event, elapsed_ms, synchronize
+    from bp_project_amd.multipatch import GroupedMultiPatch
+    gp = GroupedMultiPatch(scenes, fractions=fractions, mode=args.mode, merge_interp="kaiser8",
+                           cull=cull, tile_cols_log2=int(leaf[0]), pulse_block=leaf[1])
+    try:
+        tuned = gp.autotune(leaf=leaf)
+        gp.capture_graph()
+        for _ in range(max(1, args.warmup)):
+            gp.replay()
+        synchronize()
+        a, b = Event(), Event()
+        a.record()
+        for _ in range(args.steps):
+            gp.replay()
+        b.record()
+        synchronize()
+        ms = elapsed_ms(a, b) / args.steps
+        lat = np.concatenate([gp.completion_ms() for _ in range(3)])
+        same = all(np.array_equal(o.get(), x) for o, x in zip(gp.outputs, batch_out))
+        work = gp.work
+        rec = {"fractions": list(fractions), "group_sizes": [len(g) for g in gp.groups],
+               "graph_ms": round(ms, 3), "gpix_pulse_per_s": round(work / (ms * 1e-3) / 1e9, 2),
+               "latency_p50_ms": round(float(np.percentile(lat, 50)), 3),
+               "latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "group_completion_ms": [round(float(np.median(lat[np.asarray(g)])), 3)
+                                       for g in gp.groups],
+               "merge_shapes": [cfg["merge_shapes"] for cfg, _, _ in tuned],
+               "launches_per_batch": gp.launch_count,
+               "equals_batch_bitwise": bool(same),
+               "latency_source": "events after each group's launches, eager executions"}
+    finally:
+        gp.close()
+    log(f"latency groups {rec['group_sizes']}: {rec['graph_ms']} ms per batch, "
+        f"p50 {rec['latency_p50_ms']} ms, p99 {rec['latency_p99_ms']} ms")
+    return rec
 
 
 def cpu_baseline_multipatch(scenes, budget_s=10.0):

@@ -458,6 +458,120 @@ class MultiPatchProgram:
         self.arena = None
 
 
+def group_by_work(work, fractions):
+    """Indices 0..len(work)-1 ordered by work (ties by index) and cut at the
+    given fractions of the count (rounded; empty groups dropped)."""
+    n = len(work)
+    order = sorted(range(n), key=lambda i: (work[i], i))
+    cuts = sorted({int(round(f * n)) for f in fractions if 0.0 < f < 1.0})
+    bounds = [0] + [c for c in cuts if 0 < c < n] + [n]
+    return [order[a:b] for a, b in zip(bounds, bounds[1:])]
+
+
+class GroupedMultiPatch:
+    """A batch served in latency groups.
+
+    The patches are sorted by work (output pixels x pulses) and split at the
+    given fractions of the patch count into consecutive groups, each a
+    MultiPatchProgram; the groups run one after another (one HIP graph holds
+    them all).  A patch completes with its group's evaluation launch, so the
+    small patches of the first groups complete after a fraction of the batch
+    time, at the price of a less filled GPU while those groups run.  Outputs
+    are in the order of ``scenes`` and are the same arrays as one
+    MultiPatchProgram of all patches computes with the same leaf shapes (a
+    patch's images do not depend on the other patches of its launch).
+    Keyword arguments go to every group's MultiPatchProgram.
+    """
+
+    def __init__(self, scenes, fractions=(0.5, 0.75), **kwargs):
+        if not scenes:
+            raise ValueError("no patches")
+        n = len(scenes)
+        self.groups = group_by_work([s.grid.nx * s.grid.ny * s.pulses for s in scenes], fractions)
+        self.programs = [MultiPatchProgram([scenes[i] for i in g], **kwargs) for g in self.groups]
+        self.outputs = [None] * n
+        for g, prog in zip(self.groups, self.programs):
+            for i, o in zip(g, prog.outputs):
+                self.outputs[i] = o
+        self.launch_count = sum(p.launch_count for p in self.programs)
+        self._graph = None
+        self._stream = None
+
+    @property
+    def work(self):
+        return sum(p.work for p in self.programs)
+
+    def execute(self, group_done=None):
+        """Enqueue every group on the current stream; ``group_done(g)``, when
+        given, is called after group g's launches."""
+        for g, prog in enumerate(self.programs):
+            prog.execute()
+            if group_done:
+                group_done(g)
+        return self.outputs
+
+    def autotune(self, force=False, leaf=None):
+        """Tune each group's launch shapes (MultiPatchProgram.autotune);
+        ``leaf`` = (tile width, pulse block) fixes the leaf shapes, e.g. to
+        those of a batch whose outputs these must equal bit for bit.  Returns
+        the groups' (config, report, cached)."""
+        out = []
+        for prog in self.programs:
+            if leaf is None:
+                out.append(prog.autotune(force=force))
+            else:
+                pb = _lib.DEFAULT_PULSE_BLOCK if leaf[1] is None else int(leaf[1])
+                out.append(prog.autotune(force=force, tile_widths=(int(leaf[0]),),
+                                         pulse_blocks=(pb,)))
+        if self._graph is not None:
+            self._graph.close()
+            self._graph = None
+        return out
+
+    def completion_ms(self):
+        """One eager execution: each patch's completion time (ms after the
+        batch started, HIP events after each group), in ``scenes`` order."""
+        start = Event()
+        ends = []
+
+        def group_done(g):
+            e = Event()
+            e.record()
+            ends.append(e)
+        start.record()
+        self.execute(group_done)
+        ends[-1].synchronize()
+        done = np.zeros(len(self.outputs))
+        for g, e in zip(self.groups, ends):
+            done[g] = elapsed_ms(start, e)
+        return done
+
+    def capture_graph(self):
+        if self._graph is None:
+            self._stream = Stream()
+            current_stream().synchronize()
+            with self._stream:
+                self.execute()
+            self._stream.synchronize()
+            self._graph = Graph.capture(self._stream, self.execute)
+        return True
+
+    def replay(self):
+        if self._graph is None:
+            raise RuntimeError("capture_graph() must succeed before replay()")
+        self._graph.launch()
+        return self.outputs
+
+    def close(self):
+        if self._graph is not None:
+            self._graph.close()
+        self._graph = None
+        for prog in self.programs:
+            prog.close()
+        self.programs = []
+        self.outputs = []
+
+
 def sequential_reference(scenes, mode="split", leaf_size=16, range_spacing=0.5,
                          angular_oversample=2.5, leaf_interp="lagrange4", merge_interp="kaiser8",
                          tile_cols_log2=6, cull=False, pulse_block=None):

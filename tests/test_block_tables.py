@@ -23,3 +23,16 @@ def test_xcd_order_gives_each_die_a_contiguous_run():
     firsts = [out[x][0] for x in range(8) if out[x][0] >= 0]
     assert firsts == sorted(firsts)
     assert np.diff(firsts).min() >= 7
+
+
+def test_group_by_work():
+    """Latency groups: indices ordered by work (ties by index), cut at
+    rounded fractions of the count, every index once, no empty group."""
+    from bp_project_amd.multipatch import group_by_work
+    work = [50, 10, 30, 10, 90, 70, 20, 60]
+    g = group_by_work(work, (0.5, 0.75))
+    assert g == [[1, 3, 6, 2], [0, 7], [5, 4]]
+    assert group_by_work(work, ()) == [[1, 3, 6, 2, 0, 7, 5, 4]]
+    assert group_by_work(work, (0.0, 1.0, 0.01)) == [[1, 3, 6, 2, 0, 7, 5, 4]]
+    assert group_by_work([5], (0.5,)) == [[0]]
+    assert [len(x) for x in group_by_work(list(range(64)), (0.5, 0.75))] == [32, 16, 16]

@@ -88,6 +88,40 @@ def test_culled_multipatch_equals_unculled(scenes):
     culled.close()
 
 
+def test_grouped_multipatch_equals_batch(scenes, tmp_path, monkeypatch):
+    """Latency groups (patches by work, run group after group): outputs
+    bitwise equal to one batch of all patches with the same leaf shapes,
+    eagerly and under graph replay; the first group completes first."""
+    from bp_project_amd.device import synchronize
+    from bp_project_amd.multipatch import GroupedMultiPatch, MultiPatchProgram
+    monkeypatch.setenv("BPMI_TUNE_CACHE", str(tmp_path / "tune.json"))
+    batch = MultiPatchProgram(scenes, mode="split", merge_interp="kaiser8", tile_cols_log2=5,
+                              cull=True)
+    batch.execute()
+    synchronize()
+    ref = [o.get() for o in batch.outputs]
+    batch.close()
+    gp = GroupedMultiPatch(scenes, fractions=(0.5,), mode="split", merge_interp="kaiser8",
+                           tile_cols_log2=5, cull=True)
+    assert sorted(i for g in gp.groups for i in g) == list(range(len(scenes)))
+    assert [len(g) for g in gp.groups] == [3, 3]
+    work = [s.grid.nx * s.grid.ny * s.pulses for s in scenes]
+    assert max(work[i] for i in gp.groups[0]) <= min(work[i] for i in gp.groups[1])
+    gp.autotune(force=True, leaf=(5, None))
+    done = gp.completion_ms()
+    assert all(np.array_equal(a, o.get()) for a, o in zip(ref, gp.outputs))
+    assert max(done[gp.groups[0]]) < min(done[gp.groups[1]])
+    assert gp.capture_graph()
+    for _ in range(3):
+        for o in gp.outputs:
+            o.fill_zero()
+        synchronize()
+        gp.replay()
+        synchronize()
+        assert all(np.array_equal(a, o.get()) for a, o in zip(ref, gp.outputs))
+    gp.close()
+
+
 def test_multipatch_patch_vs_fp64_oracle(oracle, scenes):
     """One patch through the batch against the oracle's FP64 FFBP + evaluation."""
     from bp_project_amd.device import synchronize
