@@ -43,6 +43,15 @@ __device__ __forceinline__ float fract32(float x) { return x - floorf(x); }
 __device__ __forceinline__ double hw_fract64(double x) { return __builtin_amdgcn_fract(x); }
 __device__ __forceinline__ float hw_fract32(float x) { return __builtin_amdgcn_fractf(x); }
 
+// FP64 -> int32 rounded toward zero, saturating: v_cvt_i32_f64 gives INT_MAX
+// above and INT_MIN below the int range and 0 for NaN.  Written as the
+// instruction because a C++ conversion of an out-of-range value is undefined.
+__device__ __forceinline__ int cvt_i32_sat(double x) {
+    int i;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+
 // sin/cos of 2*pi*t with the hardware transcendental unit (input in turns).
 __device__ __forceinline__ void sincos_turns(float t, float* s, float* c) {
     *s = __builtin_amdgcn_sinf(t);

@@ -334,14 +334,15 @@ __global__ __launch_bounds__(256) void contrib_kernel(const float2* __restrict__
 // Fast (split-mode) regrid: FP64 geometry without IEEE square roots or
 // divisions (hardware reciprocal square root / reciprocal + Newton steps),
 // the destination pixel evaluated once per parent pixel, and Kaiser-sinc
-// weights from per-tap polynomials (degree 8 in s = 2t - 1, fitted by the
-// library to the tabulated kernel at upload; max weight error ~2.5e-7).
+// weights from per-tap polynomials (degree 7 in s = 2t - 1, fitted by the
+// library to the tabulated kernel at upload; max weight error 1.3e-6, and
+// the eight weights sum to 1 up to FP32 rounding).
 //
 // The 8-tap kernel is mirror-symmetric, w_{7-a}(s) = w_a(-s), so each pair
-// (a, 7 - a) shares an even part E_a(s^2) (5 coefficients) and an odd part
+// (a, 7 - a) shares an even part E_a(s^2) (4 coefficients) and an odd part
 // s O_a(s^2) (4 coefficients): w_a = E + s O, w_{7-a} = E - s O.
 // ---------------------------------------------------------------------------
-constexpr int KPOLY_EVEN = 5, KPOLY_ODD = 4, KPOLY_STRIDE = KPOLY_EVEN + KPOLY_ODD;
+constexpr int KPOLY_EVEN = 4, KPOLY_ODD = 4, KPOLY_STRIDE = KPOLY_EVEN + KPOLY_ODD;
 __constant__ float c_kaiser_poly[4][KPOLY_STRIDE];
 
 // kp: the coefficient table (c_kaiser_poly, or a workgroup's LDS copy of it).
@@ -423,6 +424,33 @@ __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double
     c.ok = defined && InterpShape<INTERP>::valid(c.ir, f.nr) && InterpShape<INTERP>::valid(c.iu, f.nu);
     c.tr = (float)(qr - fr);
     c.tu = (float)(qu - fu);
+    c.rot = cis_turns64(kt * (rr - r));
+    return c;
+}
+
+// The tiled pixel loops' form of child_tap (3 instead of 6 VALU per axis):
+// the fraction is v_fract_f64 (q - floor(q), the same value) and the index
+// the saturating truncation of q, which equals floor(q) for q >= 0; for a
+// negative, NaN (origin) or out-of-range q it is 0, INT_MIN or INT_MAX, and
+// no stencil with such an index is valid (FIRST >= 1 taps before it), so the
+// sample is 0 as with child_tap.  ok is not computed: the LDS window path
+// needs no validity test and tile_sample's global-memory path tests it.
+template <int INTERP>
+__device__ __forceinline__ ChildTap child_tap_px(const FastSrc& f, double x, double y, double r,
+                                                 double kt) {
+    static_assert(FastShape<INTERP>::FIRST >= 1, "a truncated negative index must be invalid");
+    const double dx = x - f.ox, dy = y - f.oy;
+    const double s2 = dx * dx + dy * dy;
+    const double uu = dy * rsqrt64(s2);
+    const double rr2 = s2 + f.oz2;
+    const double rr = rr2 * rsqrt64(rr2);
+    const double qr = (rr - f.r0) * f.inv_dr, qu = (uu - f.u0) * f.inv_du;
+    ChildTap c;
+    c.ir = cvt_i32_sat(qr);
+    c.iu = cvt_i32_sat(qu);
+    c.tr = (float)hw_fract64(qr);
+    c.tu = (float)hw_fract64(qu);
+    c.ok = false;
     c.rot = cis_turns64(kt * (rr - r));
     return c;
 }
@@ -606,8 +634,18 @@ constexpr int MT_WAVES = 3;
 
 struct Window {
     int r0, c0, h, w;  // rows [r0, r0 + h), columns [c0, c0 + w) of the child
+    int pitch;         // LDS row pitch in samples (w, or NARROW_PITCH)
     bool lds;
 };
+
+// Narrow windows (at most NARROW_COLS columns: every merge level of the
+// 8192-pulse stripmap plan below the root, whose windows are 41-42 x 25-32)
+// are staged with a fixed row pitch of NARROW_PITCH samples, so that all 64
+// taps of a stencil lie at compile-time offsets of one LDS address (7 x 33 +
+// 7 = 238 units of 8 bytes, within ds_read2_b64's 8-bit offsets) and the
+// per-row address additions disappear; the odd pitch keeps the two parent
+// rows of a wave, one child row apart, off a common bank.
+constexpr int NARROW_COLS = 32, NARROW_PITCH = 33;
 
 // Window from the corner stencil origins ir[4], iu[4] (INT_MAX if undefined).
 template <int INTERP, int CAP = MT_CAP>
@@ -632,19 +670,32 @@ __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, in
     w.c0 = c0;
     w.h = w.lds ? h : 0;
     w.w = w.lds ? wd : 0;
+    w.pitch = w.w;
     return w;
+}
+
+// Switches both windows of a merge tile to the narrow layout when both are
+// staged, at most NARROW_COLS wide and fit CAP samples at NARROW_PITCH.
+template <int CAP>
+__device__ __forceinline__ bool narrow_windows(Window& a, Window& b) {
+    const bool n = a.lds && b.lds && a.w <= NARROW_COLS && b.w <= NARROW_COLS &&
+                   a.h * NARROW_PITCH <= CAP && b.h * NARROW_PITCH <= CAP;
+    if (n) a.pitch = b.pitch = NARROW_PITCH;
+    return n;
 }
 
 // Stencil offsets (dr, du) of c in the window and whether the whole stencil
 // lies in it: one unsigned compare per axis (w.h, w.w >= TAPS when w.lds;
-// an undefined origin, INT_MAX, lands far outside).
+// an undefined origin, INT_MAX or INT_MIN, lands far outside: the offsets
+// are formed in unsigned arithmetic, which wraps).
 template <int INTERP>
 __device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w, int* dr, int* du) {
     constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
-    *dr = c.ir - (FIRST + w.r0);
-    *du = c.iu - (FIRST + w.c0);
-    return w.lds && (unsigned)*dr <= (unsigned)(w.h - TAPS) &&
-           (unsigned)*du <= (unsigned)(w.w - TAPS);
+    const unsigned ur = (unsigned)c.ir - (unsigned)(FIRST + w.r0);
+    const unsigned uc = (unsigned)c.iu - (unsigned)(FIRST + w.c0);
+    *dr = (int)ur;
+    *du = (int)uc;
+    return w.lds && ur <= (unsigned)(w.h - TAPS) && uc <= (unsigned)(w.w - TAPS);
 }
 
 // Copy the window into LDS: wave v takes rows v, v + 4, .., lane j columns
@@ -678,8 +729,8 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
     if (wave >= w.h) return;
     const float2* src = f.img + (long long)(w.r0 + wave) * f.ld + w.c0;
     const long long src_step = (long long)WAVES * f.ld;
-    float2* d = dst + wave * w.w;
-    const int dst_step = WAVES * w.w;
+    float2* d = dst + wave * w.pitch;
+    const int dst_step = WAVES * w.pitch;
     if (w.w <= 64) {
         if (lane < w.w) {
             for (int i = wave; i < w.h; i += STAGE_ROWS * WAVES) {
@@ -740,26 +791,29 @@ __device__ __forceinline__ void stage_two_windows(const FastSrc& fa, const Windo
 #pragma unroll
         for (int b = 0; b < STAGE_ROWS; ++b) {
             const int r = i + b * WAVES;
-            *(r < wa.h && lane < wa.w ? da + r * wa.w + lane : dump + lane) = va[b];
-            *(r < wb.h && lane < wb.w ? db + r * wb.w + lane : dump + lane) = vb[b];
+            *(r < wa.h && lane < wa.w ? da + r * wa.pitch + lane : dump + lane) = va[b];
+            *(r < wb.h && lane < wb.w ? db + r * wb.pitch + lane : dump + lane) = vb[b];
         }
     }
 }
 
-template <int INTERP, bool COH = false>
+// PITCH: the window's LDS row pitch when it is known at compile time (the
+// narrow layout), else 0 (w.pitch).
+template <int INTERP, bool COH = false, int PITCH = 0>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
                                               const float2* sh, const float* kp) {
-    // The window is clipped to the image, so a stencil inside it is valid
-    // (c.ok): only the global-memory path needs the validity test, which is
-    // restated here so that c.ok is not computed on the window path.
+    // The window is clipped to the image, so a stencil inside it is valid:
+    // only the global-memory path needs the validity test, which is restated
+    // here for the indices of child_tap (INT_MAX on both axes when undefined)
+    // and of child_tap_px (truncated, saturated on each axis).
     int dr, du;
     if (in_window<INTERP>(c, w, &dr, &du)) {
-        const float2* base = sh + dr * w.w + du;
-        const int pitch = w.w;
+        const int pitch = PITCH > 0 ? PITCH : w.pitch;
+        const float2* base = sh + dr * pitch + du;
         return stencil<INTERP>(c, [&](int a) { return base + a * pitch; }, kp);
     }
-    const bool ok = c.ir != 0x7fffffff && InterpShape<INTERP>::valid(c.ir, f.nr) &&
-                    InterpShape<INTERP>::valid(c.iu, f.nu);
+    const bool ok = c.ir != 0x7fffffff && c.iu != 0x7fffffff &&
+                    InterpShape<INTERP>::valid(c.ir, f.nr) && InterpShape<INTERP>::valid(c.iu, f.nu);
     if (!ok) return make_float2(0.f, 0.f);
     return stencil_global<INTERP, COH>(f, c, kp);
 }
@@ -863,12 +917,16 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         ira[j] = uniform_i32(lds.corner[j][0]); iua[j] = uniform_i32(lds.corner[j][1]);
         irb[j] = uniform_i32(lds.corner[j][2]); iub[j] = uniform_i32(lds.corner[j][3]);
     }
-    const Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
-    const Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
+    Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
+    Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
+    const bool narrow = narrow_windows<CAP>(wa, wb);
     stage_two_windows<COH>(fa, wa, sha, fb, wb, shb, lds.dump, wave, lane);
     __syncthreads();
-    if (col < nu) {
-        const DestColumn dc = dest_column(g, side, col);
+    if (col >= nu) return;
+    const DestColumn dc = dest_column(g, side, col);
+    // The pixel loop, with the windows' LDS pitch fixed (narrow layout) or
+    // read from the window: the same arithmetic either way.
+    auto pixels = [&]<int PITCH>() {
 #pragma unroll 1
         for (int j = 0; j < MT_PPT; ++j) {
             const int row = row0 + j * ROW_STEP;
@@ -877,14 +935,18 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
             const double2 rg = lds.row[t / MT_COLS + j * ROW_STEP];
             const double rr = rg.x;
             const double x = g[0] + rg.y * dc.scu, y = g[1] + rg.y * dc.u;
-            const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
-            const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-            const float2 va = tile_sample<INTERP, COH>(fa, ca, wa, sha, kpoly);
-            const float2 vb = tile_sample<INTERP, COH>(fb, cb, wb, shb, kpoly);
+            const ChildTap ca = child_tap_px<INTERP>(fa, x, y, rr, kt);
+            const ChildTap cb = child_tap_px<INTERP>(fb, x, y, rr, kt);
+            const float2 va = tile_sample<INTERP, COH, PITCH>(fa, ca, wa, sha, kpoly);
+            const float2 vb = tile_sample<INTERP, COH, PITCH>(fb, cb, wb, shb, kpoly);
             st_c64<COH>(out + (long long)row * nu + col,
                         make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y)));
         }
-    }
+    };
+    if (narrow)
+        pixels.template operator()<NARROW_PITCH>();
+    else
+        pixels.template operator()<0>();
 }
 
 template <int INTERP, int WAVES = MT_WAVES, int CAP = MT_CAP>
@@ -1117,7 +1179,7 @@ __device__ __forceinline__ void eval_tile(const FastSrc& f, double x0, double dx
     __syncthreads();
     const int ix = tx * ET_X + (t % ET_X), iy = ty * ET_Y + t / ET_X;
     if (ix < nx && iy < ny) {
-        const ChildTap c = child_tap<INTERP>(f, x0 + ix * dx, y0 + iy * dy, 0.0, kt);
+        const ChildTap c = child_tap_px<INTERP>(f, x0 + ix * dx, y0 + iy * dy, 0.0, kt);
         out[(long long)iy * nx + ix] = tile_sample<INTERP>(f, c, w, lds.sh, lds.kpoly);
     }
 }
@@ -1148,13 +1210,19 @@ __global__ __launch_bounds__(MT_THREADS) void eval_grid_multi_tiled_kernel(
                       j.nx, j.y0, j.dy, j.ny, j.k * INV_TWO_PI, tx, ty, (float2*)j.out, lds);
 }
 
-// Chebyshev interpolation (9 nodes) of the normalised 8-tap Kaiser weights
+// Chebyshev interpolation (8 nodes) of the normalised 8-tap Kaiser weights
 // w_a(t), a = 0..3, t in [0,1), read from the FP64 table exactly as the FP64
 // paths do, converted to a power series in s = 2t - 1 and split into even
-// coefficients (s^0, s^2, .., s^8) and odd coefficients (s^1, .., s^7).
-// Taps 4..7 are the mirror images w_{7-a}(s) = w_a(-s).
+// coefficients (s^0, s^2, s^4, s^6) and odd coefficients (s^1, .., s^7).
+// Taps 4..7 are the mirror images w_{7-a}(s) = w_a(-s).  The weights are
+// normalised at every node, and interpolation on one node set is linear, so
+// the eight polynomials sum to exactly 1 before their coefficients are
+// rounded to FP32: the error is no gain error but a zero-sum ripple of at
+// most 1.3e-6 per weight (degree 8 would give 1.9e-7 at one more FMA per tap
+// pair and axis; degree 6, 1.3e-5).
 static void fit_kaiser_poly(const double* table, float out[4][KPOLY_STRIDE]) {
-    constexpr int N = 9;
+    constexpr int N = 2 * KPOLY_ODD;
+    static_assert(KPOLY_EVEN == KPOLY_ODD, "degree 2 * KPOLY_ODD - 1: N = 2 * KPOLY_ODD nodes");
     double f[4][N];
     double sn[N];
     for (int j = 0; j < N; ++j) {
