@@ -120,15 +120,30 @@ __device__ __forceinline__ const BPMI_CONST_AS T* as_const(const T* p) {
     return (const BPMI_CONST_AS T*)p;
 }
 
-// complex64 loads/stores of memory that other workgroups of the same launch
-// write or read: COH = true uses device-scope (sc1) accesses, which are
-// coherent across the accelerator dies without cache-wide write-backs or
-// invalidations; COH = false is an ordinary access.
+// complex64 loads/stores of node images that other workgroups of the same
+// launch write or read (COH = true; COH = false is an ordinary access).
+// Stores are device-scope (sc1): they are written through to memory, where
+// every die sees them, and a wave waits for them (wait_vector_memory) before
+// its item is counted complete.  Loads are ordinary L2-cached loads of the
+// global address space, which is coherent here because of how node images
+// are used within one launch: a node's image is read only after its
+// completion count is seen, no workgroup reads it before that (producers
+// read only the children of the node they write, and images never share a
+// cache line: arena slots are 512-byte aligned), and the launch starts with
+// no stale line cached, so the first read of a line on any die misses its
+// caches and fetches the completed data.  (Device-scope loads bypass the
+// die's L2, and neighbouring tiles' windows overlap: measured on the
+// 64-patch batch, they made the one-launch FFBP 20% slower.)
+//
+// Both address global memory explicitly: through a generic pointer they
+// would be flat accesses, which the compiler orders behind every LDS access
+// of the wave (a flat access may touch LDS), so a window staged from them
+// would wait out each load's round trip.
+typedef __attribute__((address_space(1))) unsigned long long global_u64_rw;
 template <bool COH>
 __device__ __forceinline__ float2 ld_c64(const float2* p) {
     if constexpr (COH) {
-        const unsigned long long v = __hip_atomic_load(
-            (unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long v = *(const global_u64_rw*)p;
         return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
     } else {
         return *p;
@@ -139,7 +154,7 @@ __device__ __forceinline__ void st_c64(float2* p, float2 v) {
     if constexpr (COH) {
         const unsigned long long u = (unsigned long long)__float_as_uint(v.x) |
                                      ((unsigned long long)__float_as_uint(v.y) << 32);
-        __hip_atomic_store((unsigned long long*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store((global_u64_rw*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         *p = v;
     }

@@ -1375,14 +1375,15 @@ __global__ __launch_bounds__(256) void fma_peak_kernel(float* out, int iters, fl
 // by running workgroups: the schedule cannot deadlock, whatever the number
 // of resident workgroups.  A merge tile waits until both children have
 // counted all their tiles complete; an evaluation block waits for its root.
-// Node images pass between items through device-scope (sc1) loads and
-// stores, which are coherent across the accelerator dies without the
-// cache-wide L2 write-back / invalidate of an agent-scope fence (measured
-// 5x slower on the 8192-pulse FFBP): every wave waits for its stores to
-// complete before the item's completion is counted, and a consumer issues
-// its device-scope loads only after it has seen the count.  Every wait is bounded by a wall-clock limit: a wait that runs out
-// sets the abort flag, after which every remaining item is skipped and all
-// workgroups drain, so the launch always ends; the host reads the flag.
+// Node images pass between items through device-scope (sc1) stores, which
+// write through to memory without the cache-wide L2 write-back / invalidate
+// of an agent-scope fence (measured 5x slower on the 8192-pulse FFBP), and
+// ordinary loads (ld_c64<true>: why they are coherent here): every wave waits
+// for its stores to complete before the item's completion is counted, and a
+// consumer reads a node only after it has seen the node's count.  Every wait
+// is bounded by a wall-clock limit: a wait that runs out sets the abort flag,
+// after which every remaining item is skipped and all workgroups drain, so
+// the launch always ends; the host reads the flag.
 enum { PW_LEAF = 0, PW_MERGE = 1, PW_EVAL = 2 };
 enum { PS_TICKET = 0, PS_ABORT = 1, PS_NODES = 2 };
 
@@ -1516,8 +1517,11 @@ __device__ __forceinline__ bool persistent_wait(PersistentLoop& st, int4 w) {
     }
 }
 
+#ifndef BPMI_PERSISTENT_WAVES
+#define BPMI_PERSISTENT_WAVES 4
+#endif
 template <int LEAF_INTERP, int MERGE_INTERP>
-__global__ __launch_bounds__(MT_THREADS, 4) void ffbp_persistent_kernel(
+__global__ __launch_bounds__(MT_THREADS, BPMI_PERSISTENT_WAVES) void ffbp_persistent_kernel(
         const PersistentArgs* tab_arg, const int4* items_arg, int nitems_arg, int* sync_arg,
         unsigned long long* stamps_arg) {
     static_assert(SP_THREADS == MT_THREADS, "one workgroup shape for every item kind");

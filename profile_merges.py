@@ -1,14 +1,15 @@
 """One C2-sized FFBP for hardware-counter collection.
 
     python profile_merges.py [--pulses 8192] [--no-untiled] [--cull grid]
-                             [--merge-shapes W:C[,W:C...]]
+                             [--merge-shapes W:C[,W:C...]] [--persistent]
 
 Runs the split-precision FFBP of the 8192-pulse x 8192-bin stripmap workload
 (leaf 16, factor 2, 8-tap Kaiser-sinc merges) once through the level-batched
 program (LDS-tiled merges; ``--cull`` none, crop or grid selects the tiles as
 the C2 step does, default grid: those the 4096 x 4096 Cartesian output needs),
 then every merge level again through the
-untiled merge kernel on the same node images, then one 1 GiB copy kernel
+untiled merge kernel on the same node images, with ``--persistent`` the same
+FFBP once more as one persistent launch (same culling), then one 1 GiB copy kernel
 (`bpmi_crop` of a 16384 x 8192 complex64 image: 1 GiB read, 1 GiB written)
 whose counters calibrate FETCH_SIZE / WRITE_SIZE.  Meant
 to run under `rocprofv3 --pmc ...`; prints one line per phase.
@@ -44,6 +45,8 @@ def main(argv=None):
     ap.add_argument("--merge-shapes", default=None,
                     help="W:C (every level) or one W:C per level, comma-separated: the "
                          "merge launch shapes (occupancy hint, window cap)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="also run the FFBP as one persistent launch (bpmi_ffbp_persistent)")
     a = ap.parse_args(argv)
     shapes = None
     if a.merge_shapes:
@@ -80,6 +83,15 @@ def main(argv=None):
                       _lib.MODE_SPLIT, _lib.interp_code("kaiser8"), None)
             synchronize()
         print("untiled merges done", flush=True)
+    if a.persistent:
+        pers = BatchedResidentProgram(backend, plan, dev, "split", s.r_start, s.data_dr, s.k,
+                                      cull=cull, persistent=True)
+        pers.execute()
+        synchronize()
+        same = np.array_equal(pers.output.get(), prog.output.get())
+        print(f"persistent FFBP done ({pers.persistent.nitems} items), "
+              f"output bitwise equal to the level launches: {same}", flush=True)
+        pers.close()
     rows, cols = COPY_BYTES // (8 * 8192), 8192
     src, dst = DeviceArray.empty((rows, cols)), DeviceArray.empty((rows, cols))
     src.fill_zero()
