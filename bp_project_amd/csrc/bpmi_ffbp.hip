@@ -1517,11 +1517,12 @@ __device__ __forceinline__ bool persistent_wait(PersistentLoop& st, int4 w) {
     }
 }
 
-#ifndef BPMI_PERSISTENT_WAVES
-#define BPMI_PERSISTENT_WAVES 4
-#endif
+// 4 waves per SIMD: the leaf and merge bodies share 128 VGPRs (3 waves, with
+// no spill, measured slower; 5 waves spill heavily).
+constexpr int PW_WAVES = 4;
+
 template <int LEAF_INTERP, int MERGE_INTERP>
-__global__ __launch_bounds__(MT_THREADS, BPMI_PERSISTENT_WAVES) void ffbp_persistent_kernel(
+__global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
         const PersistentArgs* tab_arg, const int4* items_arg, int nitems_arg, int* sync_arg,
         unsigned long long* stamps_arg) {
     static_assert(SP_THREADS == MT_THREADS, "one workgroup shape for every item kind");
@@ -1542,6 +1543,15 @@ __global__ __launch_bounds__(MT_THREADS, BPMI_PERSISTENT_WAVES) void ffbp_persis
         st.ready0 = st.ready1 = -2;
         atomicMin(stamps_arg, wall_clock64());
     }
+    // Per item: thread 0 has checked the item's dependencies (st.go) before
+    // a barrier; every wave runs the body; a barrier; thread 0 counts the
+    // item (pending), moves on and checks the next item's dependencies; a
+    // barrier.  A wave waits for its stores (wait_vector_memory) only where
+    // thread 0 may publish counts next: before the last item of a chunk
+    // (the chunk's counts are flushed) and before an item whose successor
+    // has another node or kind (the pending count is flushed when the
+    // counter changes, and only such a successor can have to wait, which
+    // flushes too); items of one node keep their stores in flight.
     for (;;) {
         // claim the next chunk; an aborted launch claims nothing more
         if (t == 0) {
@@ -1549,18 +1559,20 @@ __global__ __launch_bounds__(MT_THREADS, BPMI_PERSISTENT_WAVES) void ffbp_persis
             const int first = abort ? st.nitems : atomicAdd(st.sync + PS_TICKET, PW_CHUNK);
             st.item = first;
             st.last = min(first + PW_CHUNK, st.nitems);
+            if (first < st.nitems) {
+                const BPMI_CONST_AS int* wi = as_const((const int*)st.items) + 4 * first;
+                st.go = persistent_wait(st, make_int4(wi[0], wi[1], wi[2], 0)) ? 1 : 0;
+            }
         }
         __syncthreads();
         if (uniform_i32(lds_reload(st.item)) >= uniform_i32(lds_reload(st.nitems))) break;
         for (;;) {
             const int item = uniform_i32(lds_reload(st.item));
-            if (item >= uniform_i32(lds_reload(st.last))) break;
+            const int last = uniform_i32(lds_reload(st.last));
             const BPMI_CONST_AS PersistentArgs* tab = as_const(uniform_ptr(lds_reload(st.tab)));
             const BPMI_CONST_AS int* wv =
                 as_const((const int*)uniform_ptr(lds_reload(st.items))) + 4 * item;
             const int4 w = make_int4(uniform_i32(wv[0]), uniform_i32(wv[1]), uniform_i32(wv[2]), 0);
-            if (t == 0) st.go = persistent_wait(st, w) ? 1 : 0;
-            __syncthreads();
             const bool go = uniform_i32(lds_reload(st.go)) != 0;
             if (go) {
                 const int* node_int = uniform_ptr(tab->node_int);
@@ -1605,30 +1617,31 @@ __global__ __launch_bounds__(MT_THREADS, BPMI_PERSISTENT_WAVES) void ffbp_persis
                             uniform_f64(jp->k) * INV_TWO_PI);
                     }
                 }
-                // the item's device-scope stores have completed in every wave
-                // before its completion is counted
-                wait_vector_memory();
             }
+            // the successor, and whether this item's stores must complete in
+            // every wave before the barrier (see above)
+            const int next = item + 1;
+            int4 wn = make_int4(-1, -1, -1, 0);
+            if (next < last)
+                wn = make_int4(uniform_i32(wv[4]), uniform_i32(wv[5]), uniform_i32(wv[6]), 0);
+            if (go && (next >= last || wn.x != w.x || wn.y != w.y)) wait_vector_memory();
             __syncthreads();
             if (t == 0) {
-                if (lds_reload(st.go)) {
-                    // the item again, from the loop state (nothing of it was kept live)
-                    const BPMI_CONST_AS int* wi =
-                        as_const((const int*)lds_reload(st.items)) + 4 * lds_reload(st.item);
-                    const int idx =
-                        wi[0] == PW_EVAL ? as_const(lds_reload(st.tab))->nodes + wi[1] : wi[1];
+                if (go) {
+                    const int idx = w.x == PW_EVAL ? as_const(st.tab)->nodes + w.y : w.y;
                     if (idx != st.pend_idx) {
                         flush_pending(st);
                         st.pend_idx = idx;
                     }
                     st.pend_cnt += 1;
                 }
-                st.item = lds_reload(st.item) + 1;
+                st.item = next;
+                if (next < last) st.go = persistent_wait(st, wn) ? 1 : 0;
             }
             __syncthreads();
+            if (next >= last) break;
         }
-        // every wave has read the chunk's end before thread 0 claims the next
-        __syncthreads();
+        // the chunk's last item has drained its stores in every wave
         if (t == 0) flush_pending(st);
     }
 }
