@@ -1529,7 +1529,9 @@ __global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
     __shared__ union {
         SplitLds<LEAF_INTERP> split;
         MergeLds merge;
+        EvalLds eval;
     } lds;
+    static_assert(sizeof(EvalLds) <= sizeof(MergeLds), "the evaluation tile adds no LDS");
     __shared__ PersistentLoop st;
     const int t = threadIdx.x;
     if (t == 0) {
@@ -1600,22 +1602,22 @@ __global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
                                                    uniform_f64(tab->k), &c_kaiser_poly[0][0],
                                                    lds.merge);
                 } else {
+                    // one 64 x 4-pixel tile of the job's grid, as the level
+                    // programs' tiled evaluation computes it
                     const BPMI_CONST_AS bpmi_eval_job* jp = as_const(tab->jobs) + w.y;
                     const int nx = uniform_i32(jp->nx), ny = uniform_i32(jp->ny);
-                    const long long p = (long long)w.z + t;
-                    if (p < (long long)nx * ny) {
-                        double geo[7];
+                    const int tiles_x = (nx + ET_X - 1) / ET_X;
+                    const int ty = w.z / tiles_x, tx = w.z - ty * tiles_x;
+                    double geo[7];
 #pragma unroll
-                        for (int j = 0; j < 7; ++j) geo[j] = uniform_f64(jp->geo[j]);
-                        const FastSrc f = make_fast_src(uniform_ptr((const float2*)jp->img), geo,
-                                                        uniform_i32(jp->nr), uniform_i32(jp->nu),
-                                                        uniform_i32(jp->ld));
-                        const int iy = (int)(p / nx), ix = (int)(p - (long long)iy * nx);
-                        uniform_ptr((float2*)jp->out)[p] = eval_fast<MERGE_INTERP, true>(
-                            f, uniform_f64(jp->x0) + ix * uniform_f64(jp->dx),
-                            uniform_f64(jp->y0) + iy * uniform_f64(jp->dy),
-                            uniform_f64(jp->k) * INV_TWO_PI);
-                    }
+                    for (int j = 0; j < 7; ++j) geo[j] = uniform_f64(jp->geo[j]);
+                    const FastSrc f = make_fast_src(uniform_ptr((const float2*)jp->img), geo,
+                                                    uniform_i32(jp->nr), uniform_i32(jp->nu),
+                                                    uniform_i32(jp->ld));
+                    eval_tile<MERGE_INTERP>(f, uniform_f64(jp->x0), uniform_f64(jp->dx), nx,
+                                            uniform_f64(jp->y0), uniform_f64(jp->dy), ny,
+                                            uniform_f64(jp->k) * INV_TWO_PI, tx, ty,
+                                            uniform_ptr((float2*)jp->out), lds.eval);
                 }
             }
             // the successor, and whether this item's stores must complete in

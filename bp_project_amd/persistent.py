@@ -3,11 +3,12 @@
 The level-by-level programs (``BatchedResidentProgram``, ``MultiPatchProgram``)
 launch the leaves, one merge level per subtree height and the Cartesian
 evaluation as separate kernels.  Here the same items -- leaf split tiles,
-32 x 32 merge tiles, 256-pixel evaluation blocks -- go into one list that a
+32 x 32 merge tiles, 64 x 4-pixel evaluation tiles (the LDS-tiled evaluation
+of the level programs) -- go into one list that a
 single persistent launch drains: workgroups claim items in list order, a
 merge tile waits until both of its children are complete, an evaluation
 block until its root is.  The list is height-major (all leaves, then every
-height-1 merge, ...), and the evaluation blocks of a patch follow the merges
+height-1 merge, ...), and the evaluation tiles of a patch follow the merges
 of its root's height, so small patches finish early while the GPU keeps
 working on the upper levels of large ones.  Within a height the nodes are
 taken in ``priority`` order (default: as given).
@@ -28,7 +29,6 @@ from .device import DeviceArray, stream_handle
 from .plan import NI_HEIGHT, NI_NR, NI_NU
 
 WORK_LEAF, WORK_MERGE, WORK_EVAL = 0, 1, 2
-EVAL_BLOCK = 256
 INTERP_NONE = -1  # BPMI_INTERP_NONE: a work list without leaf items
 
 
@@ -45,11 +45,18 @@ def _merge_tiles(nr, nu):
     return t.value
 
 
+def _eval_tiles(nx, ny):
+    t = ctypes.c_int()
+    _lib.call("bpmi_eval_tiles", int(nx), int(ny), ctypes.byref(t))
+    return t.value
+
+
 def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True, tiles=None):
     """Work list of a forest of FFBP trees.
 
     node_int: int32 [nodes, 8] node table (a, b, nr, nu, side, left, right,
-    height); jobs: sequence of (root node, output pixels) per evaluation job;
+    height); jobs: sequence of (root node, nx, ny) per evaluation job (an
+    nx x ny Cartesian grid, evaluated in bpmi_eval_tiles() tiles);
     leaves=False leaves the leaf tiles out (they are computed before the
     launch; their completion targets are then 0); tiles: optional {node:
     tile indices} of a culled schedule (culling.tile_selection), every node's
@@ -74,17 +81,17 @@ def work_items(node_int, tile_cols_log2, jobs=(), priority=None, leaves=True, ti
         if heights[n] == 0 and not leaves:
             tile_list[n] = []
         node_tiles[n] = len(tile_list[n])
-    job_blocks = np.array([-(-int(px) // EVAL_BLOCK) for _, px in jobs], np.int32)
-    job_height = [int(heights[root]) for root, _ in jobs]
+    job_blocks = np.array([_eval_tiles(nx, ny) for _, nx, ny in jobs], np.int32)
+    job_height = [int(heights[root]) for root, _, _ in jobs]
     items = []
     for h in range(int(heights.max()) + 1 if nodes else 0):
         level = sorted(np.flatnonzero(heights == h), key=lambda n: (prio[n], n))
         kind = WORK_LEAF if h == 0 else WORK_MERGE
         for n in level:
             items.extend((kind, int(n), t, 0) for t in tile_list[int(n)])
-        for j, (root, _) in enumerate(jobs):
+        for j in range(len(jobs)):
             if job_height[j] == h:
-                items.extend((WORK_EVAL, j, b * EVAL_BLOCK, 0) for b in range(int(job_blocks[j])))
+                items.extend((WORK_EVAL, j, b, 0) for b in range(int(job_blocks[j])))
     return np.asarray(items, np.int32).reshape(-1, 4), node_tiles, job_blocks
 
 
@@ -134,9 +141,10 @@ class PersistentFFBP:
         self.nodes = len(node_int_host)
         self.njobs = 0 if jobs is None else len(jobs)
         self.leaves = bool(leaves)
-        job_px = [] if jobs is None else [int(j["nx"]) * int(j["ny"]) for j in jobs]
+        grids = [] if jobs is None else [(int(j["nx"]), int(j["ny"])) for j in jobs]
         items, node_tiles, job_blocks = work_items(node_int_host, tile_cols_log2,
-                                                   list(zip(job_roots, job_px)), priority,
+                                                   [(r, nx, ny) for r, (nx, ny) in
+                                                    zip(job_roots, grids)], priority,
                                                    leaves=self.leaves, tiles=tiles)
         if not check_order(items, node_int_host, job_roots, leaves=self.leaves):
             raise RuntimeError("persistent work list is not topologically ordered")

@@ -388,8 +388,9 @@ typedef struct bpmi_persistent_schedule {
     /* Cartesian evaluation jobs (may be empty) */
     const bpmi_eval_job* jobs;
     int njobs;
-    /* work list: int32 [nitems, 4] = (kind, node or job, tile or first
-     * pixel, 0), every item after the items it depends on */
+    /* work list: int32 [nitems, 4] = (kind, node or job, tile, 0), every
+     * item after the items it depends on; an evaluation item is one
+     * bpmi_eval_tiles() tile (64 x 4 pixels) of its job's grid */
     const int* items;
     int nitems;
     const int* node_tiles;          /* int32 [nodes]: items per node */

@@ -47,7 +47,7 @@ def _text(node_int, items, node_tiles, jobs=(), job_blocks=()):
     lines = [f"{len(node_int)} {len(items)} {len(jobs)}"]
     lines += [f"{int(n[NI_LEFT])} {int(n[NI_RIGHT])} {int(t)}" for n, t in zip(node_int, node_tiles)]
     lines += [f"{int(k)} {int(i)}" for k, i, *_ in items]
-    lines += [f"{int(r)} {int(b)}" for (r, _), b in zip(jobs, job_blocks)]
+    lines += [f"{int(r)} {int(b)}" for (r, _, _), b in zip(jobs, job_blocks)]
     return "\n".join(lines) + "\n"
 
 
@@ -72,7 +72,7 @@ def forest():
         for col in (NI_LEFT, NI_RIGHT):
             ni[:, col] = np.where(ni[:, col] >= 0, ni[:, col] + base, -1)
         node_int.append(ni)
-        jobs.append((base + plan.root.index, grid.nx * grid.ny))
+        jobs.append((base + plan.root.index, grid.nx, grid.ny))
         base += len(ni)
     return np.concatenate(node_int), jobs
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from bp_project_amd import _lib
-from bp_project_amd.persistent import (EVAL_BLOCK, WORK_EVAL, WORK_LEAF, WORK_MERGE,
+from bp_project_amd.persistent import (WORK_EVAL, WORK_LEAF, WORK_MERGE,
                                        check_order, work_items)
 from bp_project_amd.plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, flatten_plan
 
@@ -26,7 +26,7 @@ def test_work_list_order_and_counts():
     plan, flat = _plan()
     ni = flat.node_int
     root = plan.root.index
-    jobs = [(root, 48 * 48)]
+    jobs = [(root, 48, 48)]
     items, node_tiles, job_blocks = work_items(ni, 5, jobs)
     assert check_order(items, ni, [root])
     # every tile of every node exactly once, evaluation blocks last
@@ -37,9 +37,9 @@ def test_work_list_order_and_counts():
     t = ctypes.c_int()
     _lib.call("bpmi_merge_tiles", int(ni[root, NI_NR]), int(ni[root, NI_NU]), ctypes.byref(t))
     assert node_tiles[root] == t.value
-    assert job_blocks.tolist() == [-(-48 * 48 // EVAL_BLOCK)]
+    assert job_blocks.tolist() == [1 * 12]  # 64 x 4-pixel tiles of a 48 x 48 grid
     ev = items[items[:, 0] == WORK_EVAL]
-    assert ev[:, 2].tolist() == list(range(0, 48 * 48, EVAL_BLOCK))
+    assert ev[:, 2].tolist() == list(range(12))
     assert np.flatnonzero(items[:, 0] == WORK_EVAL).min() == len(items) - len(ev)
     # heights never decrease along the list
     h = [ni[i[1], NI_HEIGHT] if i[0] != WORK_EVAL else ni[root, NI_HEIGHT] for i in items]
@@ -62,7 +62,7 @@ def test_check_order_rejects_a_parent_before_its_child():
 
 def test_priority_moves_a_forest_s_small_tree_first():
     """Two trees in one table: with the smaller tree's nodes prioritised, its
-    evaluation blocks come right after its root's height."""
+    evaluation tiles come right after its root's height."""
     _, big = _plan(pulses=512)
     _, small = _plan(pulses=128)
     ni_b, ni_s = big.node_int.copy(), small.node_int.copy()
@@ -73,7 +73,7 @@ def test_priority_moves_a_forest_s_small_tree_first():
     root_b = int(np.flatnonzero(ni_b[:, NI_HEIGHT] == ni_b[:, NI_HEIGHT].max())[0])
     root_s = base + int(np.flatnonzero(ni_s[:, NI_HEIGHT] == ni_s[:, NI_HEIGHT].max())[0])
     prio = np.r_[np.full(len(ni_b), 1), np.full(len(ni_s), 0)]
-    items, _, _ = work_items(ni, 6, [(root_b, 100), (root_s, 100)], priority=prio)
+    items, _, _ = work_items(ni, 6, [(root_b, 10, 10), (root_s, 10, 10)], priority=prio)
     assert check_order(items, ni, [root_b, root_s])
     first_eval = items[np.flatnonzero(items[:, 0] == WORK_EVAL)[0]]
     assert first_eval[1] == 1  # the small tree's job
