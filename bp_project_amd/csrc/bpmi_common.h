@@ -140,11 +140,17 @@ __device__ __forceinline__ const BPMI_CONST_AS T* as_const(const T* p) {
 // of the wave (a flat access may touch LDS), so a window staged from them
 // would wait out each load's round trip.
 typedef __attribute__((address_space(1))) unsigned long long global_u64_rw;
+// One 64-bit load of a complex64 sample in global memory (never LDS),
+// addressed as global memory whatever the pointer's origin (a float2 copy
+// would go through a generic reference and lose the address space again).
+__device__ __forceinline__ float2 ld_global_c64(const float2* p) {
+    const unsigned long long v = *(const global_u64_rw*)p;
+    return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+}
 template <bool COH>
 __device__ __forceinline__ float2 ld_c64(const float2* p) {
     if constexpr (COH) {
-        const unsigned long long v = *(const global_u64_rw*)p;
-        return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+        return ld_global_c64(p);
     } else {
         return *p;
     }

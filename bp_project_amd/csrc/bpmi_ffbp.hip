@@ -457,8 +457,8 @@ __device__ __forceinline__ ChildTap child_tap_px(const FastSrc& f, double x, dou
 
 // Separable stencil sum times the phase rotation.  row(a) points at tap
 // (a, 0) of the stencil (row ir - FIRST + a, column iu - FIRST), in global
-// memory or in an LDS copy: the arithmetic is identical either way.
-template <int INTERP, typename Row, bool COH = false>
+// memory (GLOBAL) or in an LDS copy: the arithmetic is identical either way.
+template <int INTERP, typename Row, bool GLOBAL = false>
 __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
                                           const float* kp = &c_kaiser_poly[0][0]) {
     constexpr int TAPS = FastShape<INTERP>::TAPS;
@@ -477,7 +477,7 @@ __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
         float sx = 0.f, sy = 0.f;
 #pragma unroll
         for (int b = 0; b < TAPS; ++b) {
-            const float2 v = ld_c64<COH>(rp + b);
+            const float2 v = GLOBAL ? ld_global_c64(rp + b) : rp[b];
             sx += wu[b] * v.x;
             sy += wu[b] * v.y;
         }
@@ -487,14 +487,14 @@ __device__ __forceinline__ float2 stencil(const ChildTap& c, Row row,
     return make_float2(z.x * c.rot.x - z.y * c.rot.y, z.x * c.rot.y + z.y * c.rot.x);
 }
 
-template <int INTERP, bool COH = false>
+template <int INTERP>
 __device__ __forceinline__ float2 stencil_global(const FastSrc& f, const ChildTap& c,
                                                  const float* kp = &c_kaiser_poly[0][0]) {
     constexpr int FIRST = FastShape<INTERP>::FIRST;
     const float2* base = f.img + (long long)(c.ir - FIRST) * f.ld + (c.iu - FIRST);
     const long long ld = f.ld;
     auto row = [&](int a) { return base + a * ld; };
-    return stencil<INTERP, decltype(row), COH>(c, row, kp);
+    return stencil<INTERP, decltype(row), true>(c, row, kp);
 }
 
 // Destination polar pixel p of node geometry g (fast_bp/ffbp.py:62-67).
@@ -533,17 +533,17 @@ __device__ __forceinline__ void dest_pixel_fast(const double* g, int side, long 
 }
 
 // Child regrid at destination point (x, y) with destination slant range r.
-template <int INTERP, bool COH = false>
+template <int INTERP>
 __device__ __forceinline__ float2 regrid_fast(const FastSrc& f, double x, double y, double r,
                                               double kt) {
     const ChildTap c = child_tap<INTERP>(f, x, y, r, kt);
-    return c.ok ? stencil_global<INTERP, COH>(f, c) : make_float2(0.f, 0.f);
+    return c.ok ? stencil_global<INTERP>(f, c) : make_float2(0.f, 0.f);
 }
 
 // Polar image -> point (x, y), remodulated by exp(+j k r) (ffbp.py:189-198).
-template <int INTERP, bool COH = false>
+template <int INTERP>
 __device__ __forceinline__ float2 eval_fast(const FastSrc& f, double x, double y, double kt) {
-    return regrid_fast<INTERP, COH>(f, x, y, 0.0, kt);
+    return regrid_fast<INTERP>(f, x, y, 0.0, kt);
 }
 
 template <int INTERP>
@@ -704,24 +704,9 @@ __device__ __forceinline__ bool in_window(const ChildTap& c, const Window& w, in
 // a wave waits for one batch of loads instead of once per row.
 constexpr int STAGE_ROWS = 4;
 
-// A complex64 load from an image in global memory (never LDS).  The address
-// space cast keeps it a global load where the pointer was read from a table
-// in memory: as a flat load it could alias LDS, and the compiler would order
-// it behind every LDS access of the wave.
-typedef __attribute__((address_space(1))) const unsigned long long global_u64;
-template <bool COH = false>
-__device__ __forceinline__ float2 ld_image_c64(const float2* p) {
-    if constexpr (COH) {
-        return ld_c64<true>(p);
-    } else {
-        // loaded as one 64-bit word: a float2 copy would go through a
-        // generic reference and lose the address space again
-        const unsigned long long u = *(const global_u64*)p;
-        return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
-    }
-}
-
-template <bool COH = false>
+// Window loads are global loads (ld_global_c64) also where the image pointer
+// was read from a table in memory: as flat loads they could alias LDS, and
+// the compiler would order them behind every LDS access of the wave.
 __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, float2* dst,
                                              int wave, int lane) {
     constexpr int WAVES = MT_THREADS / 64;
@@ -741,7 +726,7 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
                 float2 v[STAGE_ROWS];
 #pragma unroll
                 for (int b = 0; b < STAGE_ROWS; ++b)
-                    v[b] = ld_image_c64<COH>(src + min(b, nb - 1) * src_step + lane);
+                    v[b] = ld_global_c64(src + min(b, nb - 1) * src_step + lane);
 #pragma unroll
                 for (int b = 0; b < STAGE_ROWS; ++b)
                     if (b < nb) d[b * dst_step + lane] = v[b];
@@ -752,7 +737,7 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
         return;
     }
     for (int i = wave; i < w.h; i += WAVES) {
-        for (int j = lane; j < w.w; j += 64) d[j] = ld_image_c64<COH>(src + j);
+        for (int j = lane; j < w.w; j += 64) d[j] = ld_global_c64(src + j);
         src += src_step;
         d += dst_step;
     }
@@ -766,14 +751,13 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
 // lane) instead of the window: every store is unconditional, so the
 // compiler moves no load into a branch where it would be waited for alone.
 // Windows wider than 64 columns take stage_window.
-template <bool COH = false>
 __device__ __forceinline__ void stage_two_windows(const FastSrc& fa, const Window& wa, float2* da,
                                                   const FastSrc& fb, const Window& wb, float2* db,
                                                   float2* dump, int wave, int lane) {
     constexpr int WAVES = MT_THREADS / 64;
     if (!(wa.lds && wb.lds && wa.w <= 64 && wb.w <= 64)) {
-        if (wa.lds) stage_window<COH>(fa, wa, da, wave, lane);
-        if (wb.lds) stage_window<COH>(fb, wb, db, wave, lane);
+        if (wa.lds) stage_window(fa, wa, da, wave, lane);
+        if (wb.lds) stage_window(fb, wb, db, wave, lane);
         return;
     }
     wave = uniform_i32(wave);
@@ -785,8 +769,8 @@ __device__ __forceinline__ void stage_two_windows(const FastSrc& fa, const Windo
 #pragma unroll
         for (int b = 0; b < STAGE_ROWS; ++b) {
             const int ra = min(i + b * WAVES, wa.h - 1), rb = min(i + b * WAVES, wb.h - 1);
-            va[b] = ld_image_c64<COH>(sa + (long long)ra * fa.ld);
-            vb[b] = ld_image_c64<COH>(sb + (long long)rb * fb.ld);
+            va[b] = ld_global_c64(sa + (long long)ra * fa.ld);
+            vb[b] = ld_global_c64(sb + (long long)rb * fb.ld);
         }
 #pragma unroll
         for (int b = 0; b < STAGE_ROWS; ++b) {
@@ -799,7 +783,7 @@ __device__ __forceinline__ void stage_two_windows(const FastSrc& fa, const Windo
 
 // PITCH: the window's LDS row pitch when it is known at compile time (the
 // narrow layout), else 0 (w.pitch).
-template <int INTERP, bool COH = false, int PITCH = 0>
+template <int INTERP, int PITCH = 0>
 __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& c, const Window& w,
                                               const float2* sh, const float* kp) {
     // The window is clipped to the image, so a stencil inside it is valid:
@@ -815,7 +799,7 @@ __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& 
     const bool ok = c.ir != 0x7fffffff && c.iu != 0x7fffffff &&
                     InterpShape<INTERP>::valid(c.ir, f.nr) && InterpShape<INTERP>::valid(c.iu, f.nu);
     if (!ok) return make_float2(0.f, 0.f);
-    return stencil_global<INTERP, COH>(f, c, kp);
+    return stencil_global<INTERP>(f, c, kp);
 }
 
 // LDS of one merge tile: both child windows (CAP samples each), the corner
@@ -920,7 +904,7 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
     Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
     const bool narrow = narrow_windows<CAP>(wa, wb);
-    stage_two_windows<COH>(fa, wa, sha, fb, wb, shb, lds.dump, wave, lane);
+    stage_two_windows(fa, wa, sha, fb, wb, shb, lds.dump, wave, lane);
     __syncthreads();
     if (col >= nu) return;
     const DestColumn dc = dest_column(g, side, col);
@@ -937,8 +921,8 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
             const double x = g[0] + rg.y * dc.scu, y = g[1] + rg.y * dc.u;
             const ChildTap ca = child_tap_px<INTERP>(fa, x, y, rr, kt);
             const ChildTap cb = child_tap_px<INTERP>(fb, x, y, rr, kt);
-            const float2 va = tile_sample<INTERP, COH, PITCH>(fa, ca, wa, sha, kpoly);
-            const float2 vb = tile_sample<INTERP, COH, PITCH>(fb, cb, wb, shb, kpoly);
+            const float2 va = tile_sample<INTERP, PITCH>(fa, ca, wa, sha, kpoly);
+            const float2 vb = tile_sample<INTERP, PITCH>(fb, cb, wb, shb, kpoly);
             st_c64<COH>(out + (long long)row * nu + col,
                         make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y)));
         }
@@ -1408,8 +1392,14 @@ struct PersistentArgs {
     double k;
 };
 
+// Counters of the sync array, addressed as global memory (a flat atomic
+// would also wait for the wave's LDS accesses).
+typedef __attribute__((address_space(1))) int global_i32;
 __device__ __forceinline__ int load_count(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load((global_i32*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int add_count(int* p, int v) {
+    return __hip_atomic_fetch_add((global_i32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ BpArgs uniform_bp_args(const BpArgs& v) {
@@ -1466,28 +1456,40 @@ struct PersistentLoop {
     int go;
     int pend_idx, pend_cnt;  // pending completion count and its counter index
     int ready0, ready1;      // last dependency pair seen complete
+    int ready_kind, ready_idx;  // the last item found runnable (kind, node or job)
 };
 
+// A fresh read of loop state in LDS (v must live in LDS): addressed as LDS
+// explicitly, since a volatile read through a generic pointer is a flat,
+// system-coherent load.
 template <typename T>
 __device__ __forceinline__ T lds_reload(const T& v) {
-    return *(const volatile T*)&v;
+    return *(const volatile __attribute__((address_space(3))) T*)&v;
 }
 
 
 // Thread 0: add the pending completion count to its counter (a job's
 // counter reaching its block count stamps the job's completion time).
+// A node's count is added without waiting for the old value (nothing reads
+// it here).
 __device__ __forceinline__ void flush_pending(PersistentLoop& st) {
     if (st.pend_cnt <= 0) return;
     const BPMI_CONST_AS PersistentArgs& pa = *as_const(st.tab);
-    const int old = atomicAdd(st.sync + PS_NODES + st.pend_idx, st.pend_cnt);
+    int* counter = st.sync + PS_NODES + st.pend_idx;
     const int job = st.pend_idx - pa.nodes;
-    if (job >= 0 && old + st.pend_cnt == as_const(pa.job_blocks)[job])
-        st.stamps[1 + job] = wall_clock64();
+    if (job >= 0) {
+        const int old = add_count(counter, st.pend_cnt);
+        if (old + st.pend_cnt == as_const(pa.job_blocks)[job]) st.stamps[1 + job] = wall_clock64();
+    } else {
+        add_count(counter, st.pend_cnt);
+    }
     st.pend_cnt = 0;
 }
 
 // Thread 0: true when item w may run, false after an abort.
 __device__ __forceinline__ bool persistent_wait(PersistentLoop& st, int4 w) {
+    // another tile of the item found runnable last: no table reads
+    if (w.x == st.ready_kind && w.y == st.ready_idx) return true;
     const BPMI_CONST_AS PersistentArgs& pa = *as_const(st.tab);
     int n0 = -1, n1 = -1;
     if (w.x == PW_MERGE) {
@@ -1496,7 +1498,11 @@ __device__ __forceinline__ bool persistent_wait(PersistentLoop& st, int4 w) {
     } else if (w.x == PW_EVAL) {
         n0 = as_const(pa.job_root)[w.y];
     }
-    if (n0 < 0 || (n0 == st.ready0 && n1 == st.ready1)) return true;
+    if (n0 < 0 || (n0 == st.ready0 && n1 == st.ready1)) {
+        st.ready_kind = w.x;
+        st.ready_idx = w.y;
+        return true;
+    }
     const int* done = st.sync + PS_NODES;
     const int need0 = as_const(pa.node_tiles)[n0];
     const int need1 = n1 >= 0 ? as_const(pa.node_tiles)[n1] : 0;
@@ -1505,12 +1511,15 @@ __device__ __forceinline__ bool persistent_wait(PersistentLoop& st, int4 w) {
         if (load_count(done + n0) >= need0 && (n1 < 0 || load_count(done + n1) >= need1)) {
             st.ready0 = n0;
             st.ready1 = n1;
+            st.ready_kind = w.x;
+            st.ready_idx = w.y;
             return true;
         }
         if (first) flush_pending(st);
         if (load_count(st.sync + PS_ABORT)) return false;
         if (wall_clock64() - t0 > pa.spin_ticks) {
-            __hip_atomic_store(st.sync + PS_ABORT, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store((global_i32*)(st.sync + PS_ABORT), 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
         __builtin_amdgcn_s_sleep(8);
@@ -1543,6 +1552,7 @@ __global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
         st.pend_idx = -1;
         st.pend_cnt = 0;
         st.ready0 = st.ready1 = -2;
+        st.ready_kind = st.ready_idx = -1;
         atomicMin(stamps_arg, wall_clock64());
     }
     // Per item: thread 0 has checked the item's dependencies (st.go) before
@@ -1558,7 +1568,7 @@ __global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
         // claim the next chunk; an aborted launch claims nothing more
         if (t == 0) {
             const bool abort = load_count(st.sync + PS_ABORT) != 0;
-            const int first = abort ? st.nitems : atomicAdd(st.sync + PS_TICKET, PW_CHUNK);
+            const int first = abort ? st.nitems : add_count(st.sync + PS_TICKET, PW_CHUNK);
             st.item = first;
             st.last = min(first + PW_CHUNK, st.nitems);
             if (first < st.nitems) {
