@@ -413,8 +413,7 @@ __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double
     const double dx = x - f.ox, dy = y - f.oy;
     const double s2 = dx * dx + dy * dy;
     const double uu = dy * rsqrt64(s2);
-    const double rr2 = s2 + f.oz2;
-    const double rr = rr2 * rsqrt64(rr2);
+    const double rr = sqrt64_newton(s2 + f.oz2);
     const double qr = (rr - f.r0) * f.inv_dr, qu = (uu - f.u0) * f.inv_du;
     const double fr = floor(qr), fu = floor(qu);
     const bool defined = fabs(fr) < 2147483000.0 && fabs(fu) < 2147483000.0;
@@ -424,7 +423,7 @@ __device__ __forceinline__ ChildTap child_tap(const FastSrc& f, double x, double
     c.ok = defined && InterpShape<INTERP>::valid(c.ir, f.nr) && InterpShape<INTERP>::valid(c.iu, f.nu);
     c.tr = (float)(qr - fr);
     c.tu = (float)(qu - fu);
-    c.rot = cis_turns64(kt * (rr - r));
+    c.rot = cis_turns64(fma(kt, rr, -kt * r));
     return c;
 }
 
@@ -442,8 +441,10 @@ __device__ __forceinline__ ChildTap child_tap_px(const FastSrc& f, double x, dou
     const double dx = x - f.ox, dy = y - f.oy;
     const double s2 = dx * dx + dy * dy;
     const double uu = dy * rsqrt64(s2);
-    const double rr2 = s2 + f.oz2;
-    const double rr = rr2 * rsqrt64(rr2);
+    const double rr = sqrt64_newton(s2 + f.oz2);
+    // (q - origin) * (1 / spacing): one scalar operand per instruction (an
+    // FMA with the origin folded in would take two and hold one of them in
+    // vector registers)
     const double qr = (rr - f.r0) * f.inv_dr, qu = (uu - f.u0) * f.inv_du;
     ChildTap c;
     c.ir = cvt_i32_sat(qr);
@@ -451,7 +452,7 @@ __device__ __forceinline__ ChildTap child_tap_px(const FastSrc& f, double x, dou
     c.tr = (float)hw_fract64(qr);
     c.tu = (float)hw_fract64(qu);
     c.ok = false;
-    c.rot = cis_turns64(kt * (rr - r));
+    c.rot = cis_turns64(fma(kt, rr, -kt * r));
     return c;
 }
 
