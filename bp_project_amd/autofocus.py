@@ -72,17 +72,32 @@ class GridOperator:
                    range_start=scene.r_start, data_dr=scene.data_dr, k=scene.k, mode=mode,
                    interp=interp)
 
+    def _forward_workspace(self):
+        """The pulse-split partials of bpmi_bp_grid_ws for this grid (kept)."""
+        ws = self.__dict__.get("_ws")
+        if ws is None:
+            torch = _torch()
+            nbytes = ctypes.c_size_t()
+            _lib.call("bpmi_bp_grid_workspace_bytes", self.nx, self.ny, self.pulses,
+                      ctypes.byref(nbytes))
+            ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.data.device)
+            self.__dict__["_ws"] = ws
+        return ws
+
     def image(self, phase, pos):
-        """Forward BP (no autograd) into a new complex64 [ny, nx] tensor."""
+        """Forward BP (no autograd) into a new complex64 [ny, nx] tensor
+        (pulse-split into chunks where the grid has few tiles)."""
         torch = _torch()
         out = torch.empty((self.ny, self.nx), dtype=torch.complex64, device=self.data.device)
         phase = phase.detach().to(torch.float64).contiguous()
         pos = pos.detach().to(torch.float64).contiguous()
-        _lib.call("bpmi_bp_grid", _vp(self.data.data_ptr()), int(self.data.shape[1]),
+        ws = self._forward_workspace()
+        _lib.call("bpmi_bp_grid_ws", _vp(self.data.data_ptr()), int(self.data.shape[1]),
                   _vp(pos.data_ptr()), _vp(self.weight.data_ptr()), _vp(phase.data_ptr()), 0,
                   self.pulses, self.x0, self.dx, self.nx, self.y0, self.dy, self.ny, self.z0,
                   self.range_start, self.data_dr, self.k, _lib.mode_code(self.mode),
-                  _lib.interp_code(self.interp), 0, _vp(out.data_ptr()), _stream())
+                  _lib.interp_code(self.interp), 0, _vp(out.data_ptr()), _vp(ws.data_ptr()),
+                  int(ws.numel()), _stream())
         return out
 
     def image64(self, phase, pos):

@@ -119,8 +119,11 @@ static long long tiles_for(int rows, int cols, int tc_log2) {
 // into chunks, each tile and chunk its own workgroup writing a partial image,
 // fills the chip, and one pass sums the partials in chunk order (the result
 // does not depend on scheduling).  Chunks keep at least SPLIT_MIN_PULSES
-// pulses so the per-tile set-up stays amortised.
-constexpr long long SPLIT_TARGET_BLOCKS = 2048;
+// pulses so the per-tile set-up stays amortised.  Grids below 2048 tiles
+// split: 1024 tiles (1024^2) make one round of 4 workgroups per CU whose
+// slowest workgroup sets the time; in 4 chunks the same forward takes 3.36
+// instead of 3.87 ms (4096 pulses, 1024^2 at 0.5 m).
+constexpr long long SPLIT_TARGET_BLOCKS = 4096;
 constexpr int SPLIT_MIN_PULSES = 64;
 
 static int pulse_chunks(long long tiles, int pulses) {
@@ -135,13 +138,13 @@ static int pulse_chunks(long long tiles, int pulses) {
 // passes the array's name to a kernel (on the host it is only a stand-in
 // for the symbol), it uses the device address that grid_scratch_address()
 // resolves once per device.  Size: a grid splits its pulses only when it has
-// T < 1024 tiles of 1024 pixels, into at most ceil(2048 / T) chunks, so its
-// partials take at most (2048 + T - 1) * 1024 * 8 bytes < 24 MiB; 24 MiB
+// T < 2048 tiles of 1024 pixels, into at most ceil(4096 / T) chunks, so its
+// partials take at most (4096 + T - 1) * 1024 * 8 bytes < 48 MiB; 48 MiB
 // therefore holds every grid's partials (256 x 256 over 1024 pulses: 16
-// partials, 8 MiB), and the size check in bpmi_bp_grid_noalloc guards that
-// bound.  Every chunk overwrites all pixels of its partial, so nothing in it
+// partials, 8 MiB; 1024 x 1024 over 4096 pulses: 4 partials, 32 MiB), and
+// the size check in bpmi_bp_grid_noalloc guards that bound.  Every chunk overwrites all pixels of its partial, so nothing in it
 // has to start at zero.
-constexpr size_t GRID_SCRATCH_BYTES = (size_t)24 << 20;
+constexpr size_t GRID_SCRATCH_BYTES = (size_t)48 << 20;
 static __device__ __align__(256) unsigned char scratch[GRID_SCRATCH_BYTES];
 
 constexpr int GRID_SCRATCH_MAX_DEVICES = 64;
@@ -677,9 +680,12 @@ struct AdjSplitLds {
 };
 
 template <int INTERP>
+// Workgroup (tile, chunk): the pulses [pulse_begin + chunk * chunk_pulses,
+// + chunk_pulses) of one tile; every pulse's partials are written by exactly
+// one workgroup per tile, whatever the chunk count.
 __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
         BpArgs a, SrcArgs s, int tc_log2, const float2* __restrict__ grad,
-        float* __restrict__ partials) {
+        float* __restrict__ partials, int chunk_pulses) {
     static_assert(ADJ_THREADS == SP_THREADS && ADJ_PPT == SP_PPT, "split tile shape");
     static_assert(ADJ_SNB % ADJ_NB == 0, "whole pulse groups per block");
     constexpr int F4 = SlotSize<INTERP>::F4;
@@ -728,8 +734,10 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
     const int vlo = InterpShape<INTERP>::FIRST < 0 ? -InterpShape<INTERP>::FIRST : 0;
     const int vhi = S - (InterpShape<INTERP>::TAPS + InterpShape<INTERP>::FIRST);
 
-    for (int nb0 = a.pulse_begin; nb0 < a.pulse_end; nb0 += ADJ_SNB) {
-        const int nb = min(ADJ_SNB, a.pulse_end - nb0);
+    const int cbegin = a.pulse_begin + (int)blockIdx.y * chunk_pulses;
+    const int cend = min(a.pulse_end, cbegin + chunk_pulses);
+    for (int nb0 = cbegin; nb0 < cend; nb0 += ADJ_SNB) {
+        const int nb = min(ADJ_SNB, cend - nb0);
         if (t < nb) {
             const int n = nb0 + t;
             lds.prm[t] = split_pulse_params<false>(a, lds.ref, n, kt, inv_ddr, vlo, vhi);
@@ -894,31 +902,52 @@ __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
 }
 
 // Fixed-order FP64 reduction over tiles: out_phase[n] = -Im A, out_pos[n] = G.
-__global__ void adjoint_reduce_kernel(const float* __restrict__ partials, int tiles, int P,
-                                      double* __restrict__ grad_phase,
-                                      double* __restrict__ grad_pos) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= P) return;
-    double s[5] = {0, 0, 0, 0, 0};
-    for (int b = 0; b < tiles; ++b) {
-        const float* p = partials + ((size_t)b * P + n) * 5;
+// A workgroup takes AR_PULSES pulses; wave c sums part c of the tile range
+// (tiles in order, four loads in flight), then lane n of wave 0 adds the
+// AR_PARTS part sums in order, so the result is the same on every call.
+// (One thread per pulse over all tiles ran 16 workgroups, latency-bound:
+// 0.32 ms for the 1024 x 4096 partials of a 1024^2 grid.)
+constexpr int AR_PULSES = 64, AR_PARTS = 16;
+__global__ __launch_bounds__(AR_PULSES * AR_PARTS) void adjoint_reduce_kernel(
+        const float* __restrict__ partials, int tiles, int P, double* __restrict__ grad_phase,
+        double* __restrict__ grad_pos) {
+    __shared__ double part[AR_PARTS][ADJ_VALS][AR_PULSES];
+    const int lane = threadIdx.x % AR_PULSES, c = threadIdx.x / AR_PULSES;
+    const int n = blockIdx.x * AR_PULSES + lane;
+    const int per = (tiles + AR_PARTS - 1) / AR_PARTS;
+    const int b0 = min(tiles, c * per), b1 = min(tiles, b0 + per);
+    double s[ADJ_VALS] = {0, 0, 0, 0, 0};
+    if (n < P) {
+#pragma unroll 4
+        for (int b = b0; b < b1; ++b) {
+            const float* p = partials + ((size_t)b * P + n) * ADJ_VALS;
 #pragma unroll
-        for (int j = 0; j < 5; ++j) s[j] += (double)p[j];
+            for (int j = 0; j < ADJ_VALS; ++j) s[j] += (double)p[j];
+        }
     }
+#pragma unroll
+    for (int j = 0; j < ADJ_VALS; ++j) part[c][j][lane] = s[j];
+    __syncthreads();
+    if (c != 0 || n >= P) return;
+    double t[ADJ_VALS] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < AR_PARTS; ++q)
+#pragma unroll
+        for (int j = 0; j < ADJ_VALS; ++j) t[j] += part[q][j][lane];
     // dL/dphase = Re(j A) = -Im A
-    if (grad_phase) grad_phase[n] = -s[1];
+    if (grad_phase) grad_phase[n] = -t[1];
     if (grad_pos) {
-        grad_pos[3 * n] = s[2];
-        grad_pos[3 * n + 1] = s[3];
-        grad_pos[3 * n + 2] = s[4];
+        grad_pos[3 * n] = t[2];
+        grad_pos[3 * n + 1] = t[3];
+        grad_pos[3 * n + 2] = t[4];
     }
 }
 
 static int adjoint_reduce(const BpArgs& a, long long tiles, const void* partials,
                           double* grad_phase, double* grad_pos, hipStream_t hs) {
     const int P = a.pulse_end - a.pulse_begin;
-    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(grid_1d(P, 256)), dim3(256), 0, hs,
-                       (const float*)partials, (int)tiles, P, grad_phase, grad_pos);
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(grid_1d(P, AR_PULSES)),
+                       dim3(AR_PULSES * AR_PARTS), 0, hs, (const float*)partials, (int)tiles, P,
+                       grad_phase, grad_pos);
     return check_launch("adjoint_reduce_kernel");
 }
 
@@ -970,11 +999,16 @@ static int adjoint_grid_launch(int interp, const BpArgs& a, const SrcArgs& s, co
     if (partials_bytes < need)
         return set_error(BPMI_ERR_INVALID, "partials workspace too small: need %zu bytes", need);
     if (tiles > 0x7fffffffLL) return set_error(BPMI_ERR_INVALID, "image too large");
-    const dim3 g((unsigned)tiles);
+    // pulse chunks by the forward's rule, whole pulse blocks each
+    int chunks = pulse_chunks(tiles, P);
+    int cp = (P + chunks - 1) / chunks;
+    cp = (cp + ADJ_SNB - 1) / ADJ_SNB * ADJ_SNB;
+    chunks = (P + cp - 1) / cp;
+    const dim3 g((unsigned)tiles, (unsigned)chunks);
     if (interp == BPMI_INTERP_LAGRANGE4)
-        hipLaunchKernelGGL((bp_adjoint_split_kernel<INTERP_LAGRANGE4>), g, dim3(ADJ_THREADS), 0, hs, a, s, tcl, (const float2*)grad, (float*)partials);
+        hipLaunchKernelGGL((bp_adjoint_split_kernel<INTERP_LAGRANGE4>), g, dim3(ADJ_THREADS), 0, hs, a, s, tcl, (const float2*)grad, (float*)partials, cp);
     else
-        hipLaunchKernelGGL((bp_adjoint_split_kernel<INTERP_LINEAR>), g, dim3(ADJ_THREADS), 0, hs, a, s, tcl, (const float2*)grad, (float*)partials);
+        hipLaunchKernelGGL((bp_adjoint_split_kernel<INTERP_LINEAR>), g, dim3(ADJ_THREADS), 0, hs, a, s, tcl, (const float2*)grad, (float*)partials, cp);
     st = check_launch("bp_adjoint_split_kernel");
     if (st) return st;
     return adjoint_reduce(a, tiles, partials, grad_phase, grad_pos, hs);

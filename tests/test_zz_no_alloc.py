@@ -3,7 +3,8 @@ library's static device array) against bpmi_bp_grid_ws with a workspace of
 bpmi_bp_grid_workspace_bytes(): the same chunks and the same kernels, so the
 images must be equal bit for bit.  Two calls in a row, the second on a
 smaller grid with fewer pulses (the array still holds the first call's
-partials), then a grid too large to split (no scratch) and a non-split mode."""
+partials), then a grid too large to split (no scratch), a 1024^2 grid whose
+partials take 32 MiB of the array, and a non-split mode."""
 import ctypes
 
 import numpy as np
@@ -71,9 +72,14 @@ def test_noalloc_equals_workspace_entry_two_calls(c1):
 
 def test_noalloc_without_scratch_and_other_modes(c1):
     s, dev, p = c1
-    # 1024 x 1024 has 1024 tiles: no pulse split, no scratch
-    na, ws, nbytes = _pair(s, dev, p, 1024, 1024, 256)
+    # 2048 x 1024 has 2048 tiles: no pulse split, no scratch
+    na, ws, nbytes = _pair(s, dev, p, 2048, 1024, 256)
     assert nbytes == 0
+    assert np.array_equal(na, ws)
+    # 1024 x 1024 (1024 tiles) splits 256 pulses into 4 chunks: 32 MiB of
+    # the 48 MiB array
+    na, ws, nbytes = _pair(s, dev, p, 1024, 1024, 256)
+    assert nbytes == 4 * 1024 * 1024 * 8
     assert np.array_equal(na, ws)
     # the FP64 per-pixel kernel takes no scratch in any case
     na, ws, _ = _pair(s, dev, p, 64, 48, 300, mode=_lib.MODE_FP64)
