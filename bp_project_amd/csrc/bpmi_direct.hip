@@ -119,11 +119,13 @@ static long long tiles_for(int rows, int cols, int tc_log2) {
 // into chunks, each tile and chunk its own workgroup writing a partial image,
 // fills the chip, and one pass sums the partials in chunk order (the result
 // does not depend on scheduling).  Chunks keep at least SPLIT_MIN_PULSES
-// pulses so the per-tile set-up stays amortised.  Grids below 2048 tiles
+// pulses so the per-tile set-up stays amortised.  Grids below 4096 tiles
 // split: 1024 tiles (1024^2) make one round of 4 workgroups per CU whose
 // slowest workgroup sets the time; in 4 chunks the same forward takes 3.36
-// instead of 3.87 ms (4096 pulses, 1024^2 at 0.5 m).
-constexpr long long SPLIT_TARGET_BLOCKS = 4096;
+// instead of 3.87 ms (4096 pulses, 1024^2 at 0.5 m), in 8 chunks the
+// autofocus gradient step (forward and adjoint) 9.37 instead of 9.67 ms.
+// 32-pulse chunks were slower on 256^2 over 1024 pulses (0.079 vs 0.074 ms).
+constexpr long long SPLIT_TARGET_BLOCKS = 8192;
 constexpr int SPLIT_MIN_PULSES = 64;
 
 static int pulse_chunks(long long tiles, int pulses) {
@@ -138,13 +140,13 @@ static int pulse_chunks(long long tiles, int pulses) {
 // passes the array's name to a kernel (on the host it is only a stand-in
 // for the symbol), it uses the device address that grid_scratch_address()
 // resolves once per device.  Size: a grid splits its pulses only when it has
-// T < 2048 tiles of 1024 pixels, into at most ceil(4096 / T) chunks, so its
-// partials take at most (4096 + T - 1) * 1024 * 8 bytes < 48 MiB; 48 MiB
+// T < 4096 tiles of 1024 pixels, into at most ceil(8192 / T) chunks, so its
+// partials take at most (8192 + T - 1) * 1024 * 8 bytes < 96 MiB; 96 MiB
 // therefore holds every grid's partials (256 x 256 over 1024 pulses: 16
-// partials, 8 MiB; 1024 x 1024 over 4096 pulses: 4 partials, 32 MiB), and
+// partials, 8 MiB; 1024 x 1024 over 4096 pulses: 8 partials, 64 MiB), and
 // the size check in bpmi_bp_grid_noalloc guards that bound.  Every chunk overwrites all pixels of its partial, so nothing in it
 // has to start at zero.
-constexpr size_t GRID_SCRATCH_BYTES = (size_t)48 << 20;
+constexpr size_t GRID_SCRATCH_BYTES = (size_t)96 << 20;
 static __device__ __align__(256) unsigned char scratch[GRID_SCRATCH_BYTES];
 
 constexpr int GRID_SCRATCH_MAX_DEVICES = 64;

@@ -124,7 +124,7 @@ int bpmi_bp_grid(const void* data, int samples, const double* pos, const double*
                  void* stream);
 
 /* bpmi_bp_grid with a caller workspace (device, workspace_bytes long).  In
- * split mode a grid of few tiles (fewer than 2048 tiles of 1024 pixels)
+ * split mode a grid of few tiles (fewer than 4096 tiles of 1024 pixels)
  * has its pulse range split into chunks, each written by its own
  * workgroups to a partial image in the workspace, and the partials are
  * summed in chunk order into out (added to it with BPMI_ACCUMULATE): the
@@ -144,12 +144,12 @@ int bpmi_bp_grid_ws(const void* data, int samples, const double* pos, const doub
 int bpmi_bp_grid_workspace_bytes(int nx, int ny, int pulses, size_t* bytes);
 
 /* bpmi_bp_grid_ws that allocates nothing and takes no workspace: the
- * pulse-split partial images live in a static 48 MiB device array of the
+ * pulse-split partial images live in a static 96 MiB device array of the
  * library (one per GPU, held until the process ends), so the output equals
  * bpmi_bp_grid_ws with bpmi_bp_grid_workspace_bytes() bytes bit for bit.
- * Partials above 48 MiB are rejected (BPMI_ERR_INVALID); by the chunk rule no
- * grid needs more (at most (4096 + T - 1) * 8 KiB for T < 2048 tiles of 1024
- * pixels; grids of 2048 tiles or more need none).  No counterpart in the
+ * Partials above 96 MiB are rejected (BPMI_ERR_INVALID); by the chunk rule no
+ * grid needs more (at most (8192 + T - 1) * 8 KiB for T < 4096 tiles of 1024
+ * pixels; grids of 4096 tiles or more need none).  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops).  Calls that overlap on one GPU share
  * the array and race; the first call on a device resolves the array's

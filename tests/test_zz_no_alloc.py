@@ -72,12 +72,12 @@ def test_noalloc_equals_workspace_entry_two_calls(c1):
 
 def test_noalloc_without_scratch_and_other_modes(c1):
     s, dev, p = c1
-    # 2048 x 1024 has 2048 tiles: no pulse split, no scratch
-    na, ws, nbytes = _pair(s, dev, p, 2048, 1024, 256)
+    # 4096 x 1024 has 4096 tiles: no pulse split, no scratch
+    na, ws, nbytes = _pair(s, dev, p, 4096, 1024, 256)
     assert nbytes == 0
     assert np.array_equal(na, ws)
-    # 1024 x 1024 (1024 tiles) splits 256 pulses into 4 chunks: 32 MiB of
-    # the 48 MiB array
+    # 1024 x 1024 (1024 tiles) splits 256 pulses into 4 chunks of 64: 32 MiB
+    # of the 96 MiB array
     na, ws, nbytes = _pair(s, dev, p, 1024, 1024, 256)
     assert nbytes == 4 * 1024 * 1024 * 8
     assert np.array_equal(na, ws)
