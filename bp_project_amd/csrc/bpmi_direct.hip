@@ -144,8 +144,9 @@ static int pulse_chunks(long long tiles, int pulses) {
 // partials take at most (8192 + T - 1) * 1024 * 8 bytes < 96 MiB; 96 MiB
 // therefore holds every grid's partials (256 x 256 over 1024 pulses: 16
 // partials, 8 MiB; 1024 x 1024 over 4096 pulses: 8 partials, 64 MiB), and
-// the size check in bpmi_bp_grid_noalloc guards that bound.  Every chunk overwrites all pixels of its partial, so nothing in it
-// has to start at zero.
+// the size check in bpmi_bp_grid_noalloc guards that bound.  Every chunk
+// overwrites all pixels of its partial, so nothing in it has to start at
+// zero.
 constexpr size_t GRID_SCRATCH_BYTES = (size_t)96 << 20;
 static __device__ __align__(256) unsigned char scratch[GRID_SCRATCH_BYTES];
 
@@ -681,10 +682,10 @@ struct AdjSplitLds {
     float4 red[ADJ_ROWS][ADJ_THREADS / 4 + 1];  // rows 16-byte aligned
 };
 
-template <int INTERP>
 // Workgroup (tile, chunk): the pulses [pulse_begin + chunk * chunk_pulses,
 // + chunk_pulses) of one tile; every pulse's partials are written by exactly
 // one workgroup per tile, whatever the chunk count.
+template <int INTERP>
 __global__ __launch_bounds__(ADJ_THREADS, 4) void bp_adjoint_split_kernel(
         BpArgs a, SrcArgs s, int tc_log2, const float2* __restrict__ grad,
         float* __restrict__ partials, int chunk_pulses) {
