@@ -374,7 +374,7 @@ def run_ffbp(args, scene):
         cfg, report, cached = prog.autotune(
             execute=lambda mark: step(None if mark is None else SimpleNamespace(mark=mark)))
         tuned = autotune_summary(cfg, report, cached,
-                                 {"pulse_block": 32, "merge_shapes": [3, 2176]})
+                                 {"pulse_block": 32, "merge_shapes": [4, 2176]})
         log(f"autotuned in {time.perf_counter() - t:.1f} s:", json.dumps(tuned["config"]),
             json.dumps(tuned["default_vs_tuned"]))
     for _ in range(args.warmup):
@@ -920,7 +920,7 @@ def run_multipatch(args):
             "sequential_gpix_pulse_per_s": round(work / (seq_total * 1e-3) / 1e9, 2),
             "batched_equals_sequential_bitwise": bool(same),
             "autotune": autotune_summary(tuned_cfg, tune_report, tune_cached,
-                                         {"leaf": [6, 32], "merge_shapes": [3, 2176]}),
+                                         {"leaf": [6, 32], "merge_shapes": [4, 2176]}),
             "launches_per_batch": prog.launch_count,
             "latency_groups": groups},
         "roofline": roofline,

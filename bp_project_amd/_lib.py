@@ -25,7 +25,7 @@ KAISER_TABLE_LEN = 4 * 2048 + 2
 # block depth) and bpmi_ffbp_merges_tiled_ex ((waves per SIMD hint, window
 # cap)); the first of each is the shape of the entry points without _ex.
 PULSE_BLOCKS = (32, 16, 64)
-MERGE_SHAPES = ((3, 2176), (5, 1664), (5, 1920), (7, 1088))
+MERGE_SHAPES = ((4, 2176), (5, 1664), (5, 1920), (7, 1088))
 DEFAULT_PULSE_BLOCK = PULSE_BLOCKS[0]
 DEFAULT_MERGE_SHAPE = MERGE_SHAPES[0]
 

@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void batched_merge_fast_kernel(
 constexpr int MT_ROWS = 32, MT_COLS = 32, MT_THREADS = 256;
 constexpr int MT_PPT = MT_ROWS * MT_COLS / MT_THREADS;
 constexpr int MT_CAP = 2176;
-constexpr int MT_WAVES = 3;
+constexpr int MT_WAVES = 4;
 
 
 struct Window {
@@ -1768,7 +1768,7 @@ using namespace bpmi;
 // the default shape at the root, where a tile whose window exceeds the cap
 // reads global memory (a 32752 B workgroup, cap 2000, measured like 4
 // waves: it does not fit 5 per CU).
-#define BPMI_MERGE_SHAPE_LIST(X) X(3, 2176) X(5, 1664) X(5, 1920) X(7, 1088)
+#define BPMI_MERGE_SHAPE_LIST(X) X(4, 2176) X(5, 1664) X(5, 1920) X(7, 1088)
 
 static bool merge_shape_known(int waves, int cap) {
 #define BPMI_MERGE_SHAPE(W, C) \

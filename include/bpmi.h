@@ -269,7 +269,7 @@ int bpmi_ffbp_merges_tiled(const int* node_int, const double* node_geo,
                            int nblocks, double k, int interp, void* stream);
 /* bpmi_ffbp_merges_tiled with an explicit launch shape, for the autotuner:
  * (waves, window_cap) = the occupancy hint of the launch bounds and the
- * per-child LDS window cap in samples, one of (3, 2176) (the default shape of
+ * per-child LDS window cap in samples, one of (4, 2176) (the default shape of
  * bpmi_ffbp_merges_tiled), (5, 1664), (5, 1920), (7, 1088); anything else
  * is BPMI_ERR_INVALID.  Every shape computes the same values
  * (a window above the cap is read from global memory with the same

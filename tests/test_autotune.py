@@ -146,16 +146,16 @@ def test_keys_and_size_classes():
 def test_launch_shape_validation():
     pb, shapes = check_launch_shapes("split", None, None, 3)
     assert pb == _lib.DEFAULT_PULSE_BLOCK == 32
-    assert shapes == [_lib.DEFAULT_MERGE_SHAPE] * 3 == [(3, 2176)] * 3
+    assert shapes == [_lib.DEFAULT_MERGE_SHAPE] * 3 == [(4, 2176)] * 3
     assert check_launch_shapes("split", 64, (5, 1664), 2) == (64, [(5, 1664)] * 2)
     assert check_launch_shapes("split", 16, [[7, 1088], [5, 1920]], 2)[1] == [(7, 1088),
                                                                                (5, 1920)]
     with pytest.raises(ValueError, match="pulse_block must be one of"):
         check_launch_shapes("split", 48, None, 2)
     with pytest.raises(ValueError, match="merge shape must be one of"):
-        check_launch_shapes("split", None, (4, 2176), 2)
+        check_launch_shapes("split", None, (3, 2176), 2)
     with pytest.raises(ValueError, match="one shape per merge height"):
-        check_launch_shapes("split", None, [[3, 2176]] * 3, 2)
+        check_launch_shapes("split", None, [[4, 2176]] * 3, 2)
     with pytest.raises(ValueError, match="split mode only"):
         check_launch_shapes("mixed", 16, None, 2)
     assert set(_lib.PULSE_BLOCKS) == {16, 32, 64}
@@ -171,7 +171,7 @@ def test_c_abi_shape_lists_match_the_python_constants():
     # lagrange4 merges need no Kaiser table, so the shape check is the first
     # to fail
     st = lib.bpmi_ffbp_merges_tiled_ex(one, one, one, one, one, 1, 1.0,
-                                       _lib.INTERP_CODES["lagrange4"], 4, 2176, None)
+                                       _lib.INTERP_CODES["lagrange4"], 3, 2176, None)
     assert st == _lib.ERR_INVALID
     msg = _lib.last_error()
     listed = [tuple(int(v) for v in m) for m in

@@ -35,7 +35,7 @@ def test_every_launch_shape_matches_the_default(scenes, merge_interp):
     prog = MultiPatchProgram(scenes, mode="split", merge_interp=merge_interp, tile_cols_log2=6)
     ref = run(prog)
     heights = len(prog.merge_tables)
-    assert prog.launch_config() == {"leaf": [6, 32], "merge_shapes": [[3, 2176]] * heights}
+    assert prog.launch_config() == {"leaf": [6, 32], "merge_shapes": [[4, 2176]] * heights}
     for shape in _lib.MERGE_SHAPES[1:]:
         prog.set_launch_config({"leaf": [6, 32], "merge_shapes": [list(shape)] * heights})
         out = run(prog)
@@ -132,7 +132,7 @@ def test_single_plan_launch_shapes_and_tuner(tmp_path, monkeypatch):
     synchronize()
     assert np.linalg.norm(prog.output.get() - ref) <= 1e-5 * np.linalg.norm(ref)
     with pytest.raises(ValueError, match="merge shape must be one of"):
-        prog.set_launch_shapes(merge_shapes=(4, 2176))
+        prog.set_launch_shapes(merge_shapes=(3, 2176))
     prog.close()
 
 
@@ -141,7 +141,7 @@ def test_shape_arguments_rejected_at_the_boundary():
     from bp_project_amd import _lib
     lib = _lib.load()
     one = ctypes.c_void_p(1)
-    st = lib.bpmi_ffbp_merges_tiled_ex(one, one, one, one, one, 1, 1.0, 0, 4, 2176, None)
+    st = lib.bpmi_ffbp_merges_tiled_ex(one, one, one, one, one, 1, 1.0, 0, 3, 2176, None)
     assert st == _lib.ERR_INVALID and "merge shape" in _lib.last_error()
     st = lib.bpmi_ffbp_leaves_multi_ex(one, one, one, one, one, one, 1, 4, 0, 6, 48, None)
     assert st == _lib.ERR_INVALID and "pulse_block must be 16, 32 or 64" in _lib.last_error()
