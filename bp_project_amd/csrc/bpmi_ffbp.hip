@@ -752,10 +752,46 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
 // lane) instead of the window: every store is unconditional, so the
 // compiler moves no load into a branch where it would be waited for alone.
 // Windows wider than 64 columns take stage_window.
+//
+// In the narrow layout (narrow_windows: both windows staged, at most
+// NARROW_COLS wide, row pitch NARROW_PITCH) one load covers two window rows:
+// lanes 0-31 take the columns of row u, lanes 32-63 those of row u + 1.
+// Lanes past a window's width load its last column and store into the
+// row's padding (columns w .. NARROW_PITCH - 1, which no stencil reads), and
+// u is clamped to h - 2 so that both rows exist (a pair past the end stores
+// rows h - 2 and h - 1 again, with the same values).  Row offsets and clamps
+// are then wave-uniform: no per-lane address arithmetic besides the lane's
+// fixed offsets, and half the load and store instructions.
 __device__ __forceinline__ void stage_two_windows(const FastSrc& fa, const Window& wa, float2* da,
                                                   const FastSrc& fb, const Window& wb, float2* db,
-                                                  float2* dump, int wave, int lane) {
+                                                  float2* dump, bool narrow, int wave, int lane) {
     constexpr int WAVES = MT_THREADS / 64;
+    static_assert(2 * NARROW_COLS == 64 && NARROW_PITCH > NARROW_COLS, "two rows per wave load");
+    if (narrow) {
+        wave = uniform_i32(wave);
+        const int half = lane >> 5, col = lane & (NARROW_COLS - 1);
+        const float2* const ia = fa.img + (long long)wa.r0 * fa.ld + wa.c0;
+        const float2* const ib = fb.img + (long long)wb.r0 * fb.ld + wb.c0;
+        const int oa = half * fa.ld + min(col, wa.w - 1), ob = half * fb.ld + min(col, wb.w - 1);
+        const int os = half * NARROW_PITCH + col;
+        const int rows = max(wa.h, wb.h);
+        for (int i = 2 * wave; i < rows; i += 2 * STAGE_ROWS * WAVES) {
+            float2 va[STAGE_ROWS], vb[STAGE_ROWS];
+#pragma unroll
+            for (int b = 0; b < STAGE_ROWS; ++b) {
+                const int ua = min(i + 2 * b * WAVES, wa.h - 2), ub = min(i + 2 * b * WAVES, wb.h - 2);
+                va[b] = ld_global_c64(ia + (long long)ua * fa.ld + oa);
+                vb[b] = ld_global_c64(ib + (long long)ub * fb.ld + ob);
+            }
+#pragma unroll
+            for (int b = 0; b < STAGE_ROWS; ++b) {
+                const int ua = min(i + 2 * b * WAVES, wa.h - 2), ub = min(i + 2 * b * WAVES, wb.h - 2);
+                da[ua * NARROW_PITCH + os] = va[b];
+                db[ub * NARROW_PITCH + os] = vb[b];
+            }
+        }
+        return;
+    }
     if (!(wa.lds && wb.lds && wa.w <= 64 && wb.w <= 64)) {
         if (wa.lds) stage_window(fa, wa, da, wave, lane);
         if (wb.lds) stage_window(fb, wb, db, wave, lane);
@@ -905,7 +941,7 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
     Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
     const bool narrow = narrow_windows<CAP>(wa, wb);
-    stage_two_windows(fa, wa, sha, fb, wb, shb, lds.dump, wave, lane);
+    stage_two_windows(fa, wa, sha, fb, wb, shb, lds.dump, narrow, wave, lane);
     __syncthreads();
     if (col >= nu) return;
     const DestColumn dc = dest_column(g, side, col);
