@@ -947,9 +947,14 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     const DestColumn dc = dest_column(g, side, col);
     // The pixel loop, with the windows' LDS pitch fixed (narrow layout) or
     // read from the window: the same arithmetic either way.
+    // output address of the thread's first pixel, stepped by ROW_STEP rows
+    // (no per-pixel 64-bit multiply)
+    float2* const out_first = out + (long long)row0 * nu + col;
+    const long long out_step = (long long)ROW_STEP * nu;
     auto pixels = [&]<int PITCH>() {
+        float2* o = out_first;
 #pragma unroll 1
-        for (int j = 0; j < MT_PPT; ++j) {
+        for (int j = 0; j < MT_PPT; ++j, o += out_step) {
             const int row = row0 + j * ROW_STEP;
             if (row >= nr) break;
             // dest_pixel_col with the row terms from LDS (the same values)
@@ -960,8 +965,7 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
             const ChildTap cb = child_tap_px<INTERP>(fb, x, y, rr, kt);
             const float2 va = tile_sample<INTERP, PITCH>(fa, ca, wa, sha, kpoly);
             const float2 vb = tile_sample<INTERP, PITCH>(fb, cb, wb, shb, kpoly);
-            st_c64<COH>(out + (long long)row * nu + col,
-                        make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y)));
+            st_c64<COH>(o, make_float2(__fadd_rn(va.x, vb.x), __fadd_rn(va.y, vb.y)));
         }
     };
     if (narrow)
