@@ -92,6 +92,8 @@ SIGNATURES = {
     "bpmi_split_tiles_ex": [_i, _i, _i, POINTER(c_int)],
     "bpmi_ffbp_merges_tiled": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _vp],
     "bpmi_ffbp_merges_tiled_ex": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _i, _i, _vp],
+    "bpmi_ffbp_merge_windows": [_vp, _vp, _vp, _i, _i, _vp, _vp],
+    "bpmi_ffbp_merges_tiled_win": [_vp, _vp, _vp, _vp, _vp, _i, _d, _i, _i, _i, _vp, _vp],
     "bpmi_merge_tiles": [_i, _i, POINTER(c_int)],
     "bpmi_ffbp_required_tiles": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "bpmi_eval_required_tiles": [_vp, _i, _i, _i, _i, _i, _i, _d, _d, _i, _d, _d, _i, _i, _vp,

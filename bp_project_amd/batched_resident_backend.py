@@ -100,11 +100,31 @@ def check_launch_shapes(mode, pulse_block, merge_shapes, merge_heights):
     return pb, shapes
 
 
+def merge_window_table(node_int_ptr, node_geo_ptr, blocks, merge_interp):
+    """Window bounds of every entry of a split-mode block table
+    (``bpmi_ffbp_merge_windows``: int32 [entries, 8], built once, from the
+    node geometry only); the tiled merges read them instead of deriving
+    them in every workgroup."""
+    windows = DeviceArray.empty((blocks.shape[0], 8), np.int32)
+    _lib.call("bpmi_ffbp_merge_windows", _vp(node_int_ptr), _vp(node_geo_ptr), _vp(blocks.ptr),
+              int(blocks.shape[0]), _lib.interp_code(merge_interp), _vp(windows.ptr), None)
+    synchronize()
+    return windows
+
+
 def enqueue_merges(node_int_ptr, node_geo_ptr, offsets_ptr, arena_ptr, blocks_ptr, nblocks, k,
-                   mode, merge_interp, stream, shape=None):
+                   mode, merge_interp, stream, shape=None, windows_ptr=None):
     """One merge launch over a block table built by ``merge_block_table``;
-    ``shape`` = (waves, window_cap) of the split-mode tiled merge."""
+    ``shape`` = (waves, window_cap) of the split-mode tiled merge,
+    ``windows_ptr`` its ``merge_window_table`` (None: derived per tile)."""
     if mode == "split":
+        if windows_ptr is not None:
+            sh = _lib.DEFAULT_MERGE_SHAPE if shape is None else tuple(shape)
+            _lib.call("bpmi_ffbp_merges_tiled_win", _vp(node_int_ptr), _vp(node_geo_ptr),
+                      _vp(offsets_ptr), _vp(arena_ptr), _vp(blocks_ptr), int(nblocks), float(k),
+                      _lib.interp_code(merge_interp), int(sh[0]), int(sh[1]), _vp(windows_ptr),
+                      stream)
+            return
         if shape is not None and tuple(shape) != _lib.DEFAULT_MERGE_SHAPE:
             _lib.call("bpmi_ffbp_merges_tiled_ex", _vp(node_int_ptr), _vp(node_geo_ptr),
                       _vp(offsets_ptr), _vp(arena_ptr), _vp(blocks_ptr), int(nblocks), float(k),
@@ -231,13 +251,19 @@ class BatchedResidentProgram(ResidentProgram):
                     mapping = [(-1, 0)]
                 blocks = self._table(np.asarray(mapping, dtype=np.int32))
                 if self.mode == "split":
+                    windows = merge_window_table(self._node_table.ptr, self._geometry_table.ptr,
+                                                 blocks, self.backend.merge_interp)
+                    self._table_buffers.append(windows)
                     merge_args = (
                         _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
                         _vp(self._offset_table.ptr), _vp(arena.ptr), _vp(blocks.ptr),
                         len(mapping), self.k, self._merge_interp())
-                    self._enqueue("bpmi_ffbp_merges_tiled", len(mapping) * MERGE_BLOCK,
-                                  merge_args, name=f"batched_merge_regrid_{self.mode}")
-                    self._merge_slots.append((len(self._schedule) - 1, merge_args))
+                    self._enqueue("bpmi_ffbp_merges_tiled_win", len(mapping) * MERGE_BLOCK,
+                                  merge_args + tuple(_lib.DEFAULT_MERGE_SHAPE)
+                                  + (_vp(windows.ptr),),
+                                  name=f"batched_merge_regrid_{self.mode}")
+                    self._merge_slots.append((len(self._schedule) - 1, merge_args,
+                                              _vp(windows.ptr)))
                 else:
                     self._enqueue("bpmi_ffbp_merges", len(mapping) * MERGE_BLOCK, (
                         _vp(self._node_table.ptr), _vp(self._geometry_table.ptr),
@@ -278,11 +304,8 @@ class BatchedResidentProgram(ResidentProgram):
             self._set_entry(index, "bpmi_ffbp_leaves_multi", args)
         else:
             self._set_entry(index, "bpmi_ffbp_leaves_multi_ex", args + (pb,))
-        for (index, args), shape in zip(self._merge_slots, shapes):
-            if shape == _lib.DEFAULT_MERGE_SHAPE:
-                self._set_entry(index, "bpmi_ffbp_merges_tiled", args)
-            else:
-                self._set_entry(index, "bpmi_ffbp_merges_tiled_ex", args + tuple(shape))
+        for (index, args, windows), shape in zip(self._merge_slots, shapes):
+            self._set_entry(index, "bpmi_ffbp_merges_tiled_win", args + tuple(shape) + (windows,))
         self.pulse_block, self.merge_shapes = pb, shapes
         if self._graph is not None:
             self._graph.close()
@@ -322,7 +345,7 @@ class BatchedResidentProgram(ResidentProgram):
                         mark(label)
         names = stage_keys([e[0] for e in self._schedule])
         leaf_stage = names[self._leaf_slot[0]]
-        merge_stages = [names[i] for i, _ in self._merge_slots]
+        merge_stages = [names[i] for i, _, _ in self._merge_slots]
         groups = [Group("pulse_block", list(pulse_blocks or _lib.PULSE_BLOCKS), [leaf_stage]),
                   Group("merge_shapes", [list(s) for s in (merge_shapes or _lib.MERGE_SHAPES)],
                         merge_stages, per_stage=True)]

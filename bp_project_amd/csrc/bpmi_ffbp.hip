@@ -648,12 +648,19 @@ struct Window {
 // rows of a wave, one child row apart, off a common bank.
 constexpr int NARROW_COLS = 32, NARROW_PITCH = 33;
 
-// Window from the corner stencil origins ir[4], iu[4] (INT_MAX if undefined).
-template <int INTERP, int CAP = MT_CAP>
-__device__ __forceinline__ Window corner_window(const int* ir, const int* iu, int nr, int nu) {
+// Window bounds from the corner stencil origins ir[4], iu[4] (INT_MAX if
+// undefined), before any LDS cap: rows [r0, r0 + h), columns [c0, c0 + w)
+// of the child, the origins' box widened by one cell, covering the stencils
+// and clipped to the image; h = w = 0 when an origin is undefined or the
+// window is narrower than one stencil (it holds no stencil).
+struct WinBounds {
+    int r0, c0, h, w;
+};
+
+template <int INTERP>
+__device__ __forceinline__ WinBounds window_bounds(const int* ir, const int* iu, int nr, int nu) {
     constexpr int TAPS = FastShape<INTERP>::TAPS, FIRST = FastShape<INTERP>::FIRST;
     constexpr int LIM = 0x3fffffff;  // stencil origins of any real image are far below
-    Window w;
     int rlo = ir[0], rhi = ir[0], clo = iu[0], chi = iu[0];
 #pragma unroll
     for (int j = 1; j < 4; ++j) {
@@ -661,18 +668,36 @@ __device__ __forceinline__ Window corner_window(const int* ir, const int* iu, in
         clo = min(clo, iu[j]); chi = max(chi, iu[j]);
     }
     const bool defined = rlo > -LIM && rhi < LIM && clo > -LIM && chi < LIM;
-    // widen by one cell, cover the stencils, clip to the image
     const int r0 = max(rlo - 1 - FIRST, 0), r1 = min(rhi + 1 + (TAPS - 1 - FIRST), nr - 1);
     const int c0 = max(clo - 1 - FIRST, 0), c1 = min(chi + 1 + (TAPS - 1 - FIRST), nu - 1);
     const int h = r1 - r0 + 1, wd = c1 - c0 + 1;
-    // a window narrower than one stencil holds no stencil: not staged
-    w.lds = defined && h >= TAPS && wd >= TAPS && h <= CAP && wd <= CAP && h * wd <= CAP;
-    w.r0 = r0;
-    w.c0 = c0;
-    w.h = w.lds ? h : 0;
-    w.w = w.lds ? wd : 0;
+    const bool holds = defined && h >= TAPS && wd >= TAPS;
+    WinBounds b;
+    b.r0 = r0;
+    b.c0 = c0;
+    b.h = holds ? h : 0;
+    b.w = holds ? wd : 0;
+    return b;
+}
+
+// The window as a launch with CAP samples of LDS per child stages it: in LDS
+// when the bounds fit the cap, else read from global memory (lds false).
+template <int CAP>
+__device__ __forceinline__ Window cap_window(const WinBounds& b) {
+    Window w;
+    // the product is formed only once both sides are at most CAP
+    w.lds = b.h > 0 && b.h <= CAP && b.w <= CAP && b.h * b.w <= CAP;
+    w.r0 = b.r0;
+    w.c0 = b.c0;
+    w.h = w.lds ? b.h : 0;
+    w.w = w.lds ? b.w : 0;
     w.pitch = w.w;
     return w;
+}
+
+template <int INTERP, int CAP = MT_CAP>
+__device__ __forceinline__ Window corner_window(const int* ir, const int* iu, int nr, int nu) {
+    return cap_window<CAP>(window_bounds<INTERP>(ir, iu, nr, nu));
 }
 
 // Switches both windows of a merge tile to the narrow layout when both are
@@ -839,16 +864,18 @@ __device__ __forceinline__ float2 tile_sample(const FastSrc& f, const ChildTap& 
     return stencil_global<INTERP>(f, c, kp);
 }
 
-// LDS of one merge tile: both child windows (CAP samples each), the corner
-// stencil origins, the children's reciprocal spacings and a copy of the
-// Kaiser weight polynomials (read as vector operands, they free the 36 scalar
-// registers whose pressure otherwise spills scalars into vector lanes in the
-// pixel loop).
+// LDS of one merge tile: both child windows (CAP samples each), the
+// destination row and column terms, both windows' placement, the children's
+// reciprocal spacings and a copy of the Kaiser weight polynomials (read as
+// vector operands, they free the 36 scalar registers whose pressure
+// otherwise spills scalars into vector lanes in the pixel loop).
 template <int CAP>
 struct MergeLdsCap {
     float2 sh[2 * CAP];
     double2 row[MT_ROWS];  // per tile row: slant range r, ground range sqrt(r^2 - z^2)
-    int corner[4][4];      // corner j: ir_a, iu_a, ir_b, iu_b
+    double2 col[MT_COLS];  // per tile column: u, side * sqrt(1 - u^2) (dest_column)
+    int win[2][6];         // windows a, b: r0, c0, h, w, pitch, staged
+    int narrow;            // both windows in the narrow layout
     double inv[4];         // 1/dr, 1/du of both children
     float kpoly[4 * KPOLY_STRIDE];
     float2 dump[64];       // staging stores of lanes outside a window
@@ -857,14 +884,16 @@ using MergeLds = MergeLdsCap<MT_CAP>;
 
 // One 32 x 32 tile of parent node ``node`` (all MT_THREADS threads call it;
 // every thread reaches both barriers).  kp_src: the Kaiser weight
-// polynomials (c_kaiser_poly).  A caller that runs several tiles in a row
-// must place a barrier between them.
+// polynomials (c_kaiser_poly).  win: the tile's window bounds from a table
+// of merge_windows_kernel (children a and b: r0, c0, h, w), or null to
+// derive them here.  A caller that runs several tiles in a row must place a
+// barrier between them.
 template <int INTERP, bool COH = false, int CAP = MT_CAP>
 __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
                                            const double* __restrict__ node_geo,
                                            const long long* __restrict__ offsets, float2* arena,
                                            int node, int tile, double k, const float* kp_src,
-                                           MergeLdsCap<CAP>& lds) {
+                                           const int* __restrict__ win, MergeLdsCap<CAP>& lds) {
     // The second window's base offset is opaque to the compiler, so the
     // stencil reads address it from a register rather than through
     // per-instruction offset additions (ds_read2 offsets reach 2 KiB).
@@ -900,51 +929,103 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     constexpr int ROW_STEP = MT_THREADS / MT_COLS;
 
     const double kt = k * INV_TWO_PI;
-    // The first wave alone computes the children's reciprocal spacings and
-    // the tile's corner pixels (clamped to the node), which locate the
-    // windows: lanes 0..3 take one corner each.
+    // The first wave alone computes the children's reciprocal spacings and,
+    // without a window table, places both windows from the stencil origins
+    // of the tile's corner pixels (clamped to the node; lanes 0..3 take one
+    // corner each).  The second wave computes the destination row and column
+    // terms.  The other waves then read the windows' placement instead of
+    // deriving it.
+    const bool table = win != nullptr;  // uniform
     if (uniform_i32(wave) == 0) {
         const FastSrc fa = make_fast_src(img_a, ga, lnr, lnu, lnu);
         const FastSrc fb = make_fast_src(img_b, gb, rnr, rnu, rnu);
-        if (t < 4) {
-            const int crow = min(tr * MT_ROWS + (t >> 1) * (MT_ROWS - 1), nr - 1);
-            const int ccol = min(tu * MT_COLS + (t & 1) * (MT_COLS - 1), nu - 1);
-            double x, y, rr;
-            dest_pixel_rc(g, side, crow, ccol, &x, &y, &rr);
-            const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
-            const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
-            lds.corner[t][0] = ca.ir; lds.corner[t][1] = ca.iu;
-            lds.corner[t][2] = cb.ir; lds.corner[t][3] = cb.iu;
-        }
         if (t == 0) {
             lds.inv[0] = fa.inv_dr; lds.inv[1] = fa.inv_du;
             lds.inv[2] = fb.inv_dr; lds.inv[3] = fb.inv_du;
         }
         if (t < 4 * KPOLY_STRIDE) lds.kpoly[t] = kp_src[t];
-    } else if (uniform_i32(wave) == 1 && lane < MT_ROWS) {
-        // the row terms of dest_pixel_col, once per tile row
-        const double rr = g[3] + (double)(tr * MT_ROWS + lane) * g[4];
-        const double g2 = rr * rr - g[2] * g[2];
-        lds.row[lane] = make_double2(rr, g2 > 0.0 ? g2 * rsqrt64(g2) : 0.0);
+        if (!table) {
+            int car = 0, cau = 0, cbr = 0, cbu = 0;
+            if (t < 4) {
+                const int crow = min(tr * MT_ROWS + (t >> 1) * (MT_ROWS - 1), nr - 1);
+                const int ccol = min(tu * MT_COLS + (t & 1) * (MT_COLS - 1), nu - 1);
+                double x, y, rr;
+                dest_pixel_rc(g, side, crow, ccol, &x, &y, &rr);
+                const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, kt);
+                const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, kt);
+                car = ca.ir; cau = ca.iu;
+                cbr = cb.ir; cbu = cb.iu;
+            }
+            int ira[4], iua[4], irb[4], iub[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                ira[j] = __builtin_amdgcn_readlane(car, j);
+                iua[j] = __builtin_amdgcn_readlane(cau, j);
+                irb[j] = __builtin_amdgcn_readlane(cbr, j);
+                iub[j] = __builtin_amdgcn_readlane(cbu, j);
+            }
+            Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
+            Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
+            const int narrow = narrow_windows<CAP>(wa, wb) ? 1 : 0;
+            if (t == 0) {
+                const Window* w2[2] = {&wa, &wb};
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    lds.win[i][0] = w2[i]->r0; lds.win[i][1] = w2[i]->c0;
+                    lds.win[i][2] = w2[i]->h; lds.win[i][3] = w2[i]->w;
+                    lds.win[i][4] = w2[i]->pitch; lds.win[i][5] = w2[i]->lds ? 1 : 0;
+                }
+                lds.narrow = narrow;
+            }
+        }
+    } else if (uniform_i32(wave) == 1) {
+        static_assert(MT_ROWS + MT_COLS <= 64, "one wave computes the row and column terms");
+        if (lane < MT_ROWS) {
+            // the row terms of dest_pixel_col, once per tile row
+            const double rr = g[3] + (double)(tr * MT_ROWS + lane) * g[4];
+            const double g2 = rr * rr - g[2] * g[2];
+            lds.row[lane] = make_double2(rr, g2 > 0.0 ? g2 * rsqrt64(g2) : 0.0);
+        } else if (lane < MT_ROWS + MT_COLS) {
+            // dest_column, once per tile column (columns past the node give
+            // values no thread uses)
+            const DestColumn c = dest_column(g, side, tu * MT_COLS + lane - MT_ROWS);
+            lds.col[lane - MT_ROWS] = make_double2(c.u, c.scu);
+        }
     }
     __syncthreads();
     const FastSrc fa = make_fast_src_inv(img_a, ga, lnr, lnu, lnu, uniform_f64(lds.inv[0]),
                                          uniform_f64(lds.inv[1]));
     const FastSrc fb = make_fast_src_inv(img_b, gb, rnr, rnu, rnu, uniform_f64(lds.inv[2]),
                                          uniform_f64(lds.inv[3]));
-    int ira[4], iua[4], irb[4], iub[4];
+    Window wa, wb;
+    bool narrow;
+    if (table) {
+        // scalar loads of the table entry; the cap test in scalar registers
+        const BPMI_CONST_AS int* e = as_const(win);
+        WinBounds ba, bb;
+        ba.r0 = e[0]; ba.c0 = e[1]; ba.h = e[2]; ba.w = e[3];
+        bb.r0 = e[4]; bb.c0 = e[5]; bb.h = e[6]; bb.w = e[7];
+        wa = cap_window<CAP>(ba);
+        wb = cap_window<CAP>(bb);
+        narrow = narrow_windows<CAP>(wa, wb);
+    } else {
+        Window* const w2[2] = {&wa, &wb};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        ira[j] = uniform_i32(lds.corner[j][0]); iua[j] = uniform_i32(lds.corner[j][1]);
-        irb[j] = uniform_i32(lds.corner[j][2]); iub[j] = uniform_i32(lds.corner[j][3]);
+        for (int i = 0; i < 2; ++i) {
+            w2[i]->r0 = uniform_i32(lds.win[i][0]); w2[i]->c0 = uniform_i32(lds.win[i][1]);
+            w2[i]->h = uniform_i32(lds.win[i][2]); w2[i]->w = uniform_i32(lds.win[i][3]);
+            w2[i]->pitch = uniform_i32(lds.win[i][4]);
+            w2[i]->lds = uniform_i32(lds.win[i][5]) != 0;
+        }
+        narrow = uniform_i32(lds.narrow) != 0;
     }
-    Window wa = corner_window<INTERP, CAP>(ira, iua, fa.nr, fa.nu);
-    Window wb = corner_window<INTERP, CAP>(irb, iub, fb.nr, fb.nu);
-    const bool narrow = narrow_windows<CAP>(wa, wb);
     stage_two_windows(fa, wa, sha, fb, wb, shb, lds.dump, narrow, wave, lane);
     __syncthreads();
     if (col >= nu) return;
-    const DestColumn dc = dest_column(g, side, col);
+    const double2 cterm = lds.col[t % MT_COLS];
+    DestColumn dc;
+    dc.u = cterm.x;
+    dc.scu = cterm.y;
     // The pixel loop, with the windows' LDS pitch fixed (narrow layout) or
     // read from the window: the same arithmetic either way.
     // output address of the thread's first pixel, stepped by ROW_STEP rows
@@ -974,16 +1055,71 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         pixels.template operator()<0>();
 }
 
+// wins: null, or the block table's window bounds (merge_windows_kernel).
 template <int INTERP, int WAVES = MT_WAVES, int CAP = MT_CAP>
 __global__ __launch_bounds__(MT_THREADS, WAVES) void batched_merge_tiled_kernel(
         const int* __restrict__ node_int, const double* __restrict__ node_geo,
         const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
-        double k) {
+        double k, const int* __restrict__ wins) {
     __shared__ MergeLdsCap<CAP> lds;
     const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
     if (node < 0) return;  // padding entry of a die-ordered table (uniform)
     merge_tile<INTERP, false, CAP>(node_int, node_geo, offsets, arena, node, tile, k,
-                                   &c_kaiser_poly[0][0], lds);
+                                   &c_kaiser_poly[0][0],
+                                   wins ? wins + 8 * (long long)blockIdx.x : nullptr, lds);
+}
+
+// Window bounds of every (node, tile) entry of a tiled-merge block table,
+// the same corner-stencil placement merge_tile derives without a table (the
+// bounds depend on the plan's geometry only, not on the pulses):
+// windows[8 e + 0..3] = child a's r0, c0, h, w, [8 e + 4..7] child b's;
+// padding entries (node < 0) get empty windows.  One thread per entry.
+template <int INTERP>
+__global__ __launch_bounds__(256) void merge_windows_kernel(const int* __restrict__ node_int,
+                                                            const double* __restrict__ node_geo,
+                                                            const int* __restrict__ blocks,
+                                                            int nblocks, int* __restrict__ windows) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nblocks) return;
+    int* const out = windows + 8 * (long long)e;
+    const int node = blocks[2 * e], tile = blocks[2 * e + 1];
+    if (node < 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[j] = 0;
+        return;
+    }
+    const int* const ni = node_int + 8 * node;
+    const int nr = ni[2], nu = ni[3], side = ni[4], l = ni[5], r = ni[6];
+    const int lnr = node_int[8 * l + 2], lnu = node_int[8 * l + 3];
+    const int rnr = node_int[8 * r + 2], rnu = node_int[8 * r + 3];
+    double g[7], ga[7], gb[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        g[j] = node_geo[7 * node + j];
+        ga[j] = node_geo[7 * l + j];
+        gb[j] = node_geo[7 * r + j];
+    }
+    const FastSrc fa = make_fast_src(nullptr, ga, lnr, lnu, lnu);
+    const FastSrc fb = make_fast_src(nullptr, gb, rnr, rnu, rnu);
+    const int tiles_u = (nu + MT_COLS - 1) / MT_COLS;
+    const int tr = tile / tiles_u, tu = tile - tr * tiles_u;
+    int ira[4], iua[4], irb[4], iub[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int crow = min(tr * MT_ROWS + (j >> 1) * (MT_ROWS - 1), nr - 1);
+        const int ccol = min(tu * MT_COLS + (j & 1) * (MT_COLS - 1), nu - 1);
+        double x, y, rr;
+        dest_pixel_rc(g, side, crow, ccol, &x, &y, &rr);
+        // the stencil origins do not depend on the phase term (kt = 0)
+        const ChildTap ca = child_tap<INTERP>(fa, x, y, rr, 0.0);
+        const ChildTap cb = child_tap<INTERP>(fb, x, y, rr, 0.0);
+        ira[j] = ca.ir; iua[j] = ca.iu;
+        irb[j] = cb.ir; iub[j] = cb.iu;
+    }
+    const WinBounds ba = window_bounds<INTERP>(ira, iua, lnr, lnu);
+    const WinBounds bb = window_bounds<INTERP>(irb, iub, rnr, rnu);
+    out[0] = ba.r0; out[1] = ba.c0; out[2] = ba.h; out[3] = ba.w;
+    out[4] = bb.r0; out[5] = bb.c0; out[6] = bb.h; out[7] = bb.w;
 }
 
 // ---- required-tile marking ----------------------------------------------------
@@ -1651,7 +1787,7 @@ __global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
                 } else if (w.x == PW_MERGE) {
                     merge_tile<MERGE_INTERP, true>(node_int, node_geo, offsets, arena, w.y, w.z,
                                                    uniform_f64(tab->k), &c_kaiser_poly[0][0],
-                                                   lds.merge);
+                                                   nullptr, lds.merge);
                 } else {
                     // one 64 x 4-pixel tile of the job's grid, as the level
                     // programs' tiled evaluation computes it
@@ -1827,12 +1963,14 @@ static const char* merge_shape_names() {
 template <int INTERP>
 static void launch_merge_shape(int waves, int cap, int nblocks, hipStream_t hs,
                                const int* node_int, const double* node_geo,
-                               const long long* offsets, float2* ar, const int* blocks, double k) {
-#define BPMI_MERGE_SHAPE(W, C)                                                                  \
-    if (waves == W && cap == C) {                                                               \
-        hipLaunchKernelGGL((batched_merge_tiled_kernel<INTERP, W, C>), dim3(nblocks),           \
-                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar, blocks, k); \
-        return;                                                                                 \
+                               const long long* offsets, float2* ar, const int* blocks, double k,
+                               const int* wins) {
+#define BPMI_MERGE_SHAPE(W, C)                                                         \
+    if (waves == W && cap == C) {                                                      \
+        hipLaunchKernelGGL((batched_merge_tiled_kernel<INTERP, W, C>), dim3(nblocks),  \
+                           dim3(MT_THREADS), 0, hs, node_int, node_geo, offsets, ar,   \
+                           blocks, k, wins);                                           \
+        return;                                                                        \
     }
     BPMI_MERGE_SHAPE_LIST(BPMI_MERGE_SHAPE)
 #undef BPMI_MERGE_SHAPE
@@ -1975,6 +2113,32 @@ int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
                               const long long* offsets, void* arena, const int* blocks,
                               int nblocks, double k, int interp, int waves, int window_cap,
                               void* stream) {
+    return bpmi_ffbp_merges_tiled_win(node_int, node_geo, offsets, arena, blocks, nblocks, k,
+                                      interp, waves, window_cap, nullptr, stream);
+}
+
+int bpmi_ffbp_merge_windows(const int* node_int, const double* node_geo, const int* blocks,
+                            int nblocks, int interp, int* windows, void* stream) {
+    int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
+    if (st) return st;
+    if (nblocks <= 0) return BPMI_OK;
+    if (!node_int || !node_geo || !blocks || !windows)
+        return set_error(BPMI_ERR_INVALID, "null table");
+    hipStream_t hs = (hipStream_t)stream;
+    const dim3 grid(grid_1d(nblocks, 256));
+    if (interp == BPMI_INTERP_KAISER8)
+        hipLaunchKernelGGL(merge_windows_kernel<INTERP_KAISER8>, grid, dim3(256), 0, hs, node_int,
+                           node_geo, blocks, nblocks, windows);
+    else
+        hipLaunchKernelGGL(merge_windows_kernel<INTERP_LAGRANGE4>, grid, dim3(256), 0, hs,
+                           node_int, node_geo, blocks, nblocks, windows);
+    return check_launch("merge_windows_kernel");
+}
+
+int bpmi_ffbp_merges_tiled_win(const int* node_int, const double* node_geo,
+                               const long long* offsets, void* arena, const int* blocks,
+                               int nblocks, double k, int interp, int waves, int window_cap,
+                               const int* windows, void* stream) {
     int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
     if (st) return st;
     if (!merge_shape_known(waves, window_cap))
@@ -1987,10 +2151,10 @@ int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
     hipStream_t hs = (hipStream_t)stream;
     if (interp == BPMI_INTERP_KAISER8)
         launch_merge_shape<INTERP_KAISER8>(waves, window_cap, nblocks, hs, node_int, node_geo,
-                                           offsets, ar, blocks, k);
+                                           offsets, ar, blocks, k, windows);
     else
         launch_merge_shape<INTERP_LAGRANGE4>(waves, window_cap, nblocks, hs, node_int, node_geo,
-                                             offsets, ar, blocks, k);
+                                             offsets, ar, blocks, k, windows);
     return check_launch("batched_merge_tiled_kernel");
 }
 

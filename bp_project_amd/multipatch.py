@@ -24,7 +24,8 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Event, Graph, Stream, current_stream, elapsed_ms, stream_handle, synchronize
-from .batched_resident_backend import check_launch_shapes, enqueue_merges, merge_block_table
+from .batched_resident_backend import (check_launch_shapes, enqueue_merges, merge_block_table,
+                                       merge_window_table)
 from .culling import merge_tiles_of, split_tiles_of
 from .gpu_backend import build_ffbp_plan, ensure_kaiser_table
 from .plan import NI_HEIGHT, NI_LEFT, NI_NR, NI_NU, NI_RIGHT, flatten_plan
@@ -161,6 +162,7 @@ class MultiPatchProgram:
             else (None, None))
         self.set_tile_width(tile_cols_log2)
         self.merge_tables = []
+        self.merge_windows = []  # split mode: the window bounds of each table
         tiles = (None if self._masks is None else
                  {n: merge_tiles_of(m) for n, m in enumerate(self._masks)})
         self.merge_pixels = []  # parent pixels each merge launch computes
@@ -168,6 +170,10 @@ class MultiPatchProgram:
             nodes = np.flatnonzero(heights == h)
             mapping = merge_block_table(self.node_int, nodes, self.mode, tiles) or [(-1, 0)]
             self.merge_tables.append(DeviceArray.from_host(np.asarray(mapping, np.int32)))
+            if self.mode == "split":
+                self.merge_windows.append(merge_window_table(
+                    self.t_node_int.ptr, self.t_node_geo.ptr, self.merge_tables[-1],
+                    self.merge_interp))
             self.merge_pixels.append(self._mapping_pixels(mapping))
         # Cartesian evaluation jobs (crop as a view into each root)
         jobs = np.zeros(len(self.patches), EVAL_JOB)
@@ -329,7 +335,8 @@ class MultiPatchProgram:
             enqueue_merges(self.t_node_int.ptr, self.t_node_geo.ptr, self.t_offsets.ptr,
                            self.arena.ptr, table.ptr, table.shape[0], self.k, self.mode,
                            self.merge_interp, s,
-                           shape=self.merge_shapes[h] if self.merge_shapes else None)
+                           shape=self.merge_shapes[h] if self.merge_shapes else None,
+                           windows_ptr=self.merge_windows[h].ptr if self.merge_windows else None)
             if mark:
                 mark("merges")
         if self.mode == "split":

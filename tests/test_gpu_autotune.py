@@ -123,7 +123,7 @@ def test_single_plan_launch_shapes_and_tuner(tmp_path, monkeypatch):
     got = built.output.get()
     assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
     assert [e[1].cname for e in built._schedule[:2]] == ["bpmi_ffbp_leaves_multi_ex",
-                                                         "bpmi_ffbp_merges_tiled_ex"]
+                                                         "bpmi_ffbp_merges_tiled_win"]
     built.close()
     config, report, cached = prog.autotune(force=True)
     assert not cached and config["pulse_block"] in _lib.PULSE_BLOCKS

@@ -1,5 +1,7 @@
-"""Split-mode LDS-tiled merge (bpmi_ffbp_merges_tiled) on hand-built node
-tables: bitwise equal to the untiled split merge (bpmi_ffbp_merges) and
+"""Split-mode LDS-tiled merge (bpmi_ffbp_merges_tiled, and
+bpmi_ffbp_merges_tiled_win reading a bpmi_ffbp_merge_windows table) on
+hand-built node tables: bitwise equal to the untiled split merge
+(bpmi_ffbp_merges) and
 within 1e-4 of the FP64-mode merge, both when each child's footprint fits
 the LDS tile (coarse child u sampling) and when it does not and the kernel
 gathers from global memory (child u sampling 8x finer than the parent's).
@@ -37,6 +39,28 @@ def _tables(child_du):
     return nodes, node_int, node_geo, offsets, cursor
 
 
+def _window_table(t_int, t_geo, blocks, interp, node_int, table):
+    """bpmi_ffbp_merge_windows of a block table, checked on the host: every
+    window lies inside its child, and a stageable one is at least one
+    stencil wide."""
+    from bp_project_amd.batched_resident_backend import merge_window_table
+    from bp_project_amd.plan import NI_LEFT, NI_NR, NI_NU, NI_RIGHT
+    windows = merge_window_table(t_int.ptr, t_geo.ptr, blocks, interp)
+    w = windows.get()
+    taps = 8 if interp == "kaiser8" else 4
+    for (node, _), e in zip(table, w):
+        if node < 0:
+            assert not e.any()
+            continue
+        for child, (r0, c0, h, wd) in zip((node_int[node, NI_LEFT], node_int[node, NI_RIGHT]),
+                                          e.reshape(2, 4)):
+            assert (h == 0) == (wd == 0)
+            if h:
+                assert h >= taps and wd >= taps and r0 >= 0 and c0 >= 0
+                assert r0 + h <= node_int[child, NI_NR] and c0 + wd <= node_int[child, NI_NU]
+    return windows
+
+
 @pytest.mark.parametrize("child_du", [2e-3, 1.25e-4])
 @pytest.mark.parametrize("interp", ["kaiser8", "lagrange4"])
 def test_tiled_merge_matches_untiled(child_du, interp):
@@ -57,15 +81,21 @@ def test_tiled_merge_matches_untiled(child_du, interp):
     k = 4 * np.pi * 9.6e9 / 299792458.0
     p0 = nodes[0]["nr"] * nodes[0]["nu"]
     results = {}
-    for name in ("tiled", "untiled", "fp64"):
+    for name in ("tiled", "tiled_win", "untiled", "fp64"):
         arena = DeviceArray.from_host(arena_host)
         mode = "fp64" if name == "fp64" else "split"
-        table = np.asarray(merge_block_table(node_int, [0], "split" if name == "tiled" else "x"),
-                           np.int32)
+        kind = "split" if name.startswith("tiled") else "x"
+        table = np.asarray(merge_block_table(node_int, [0], kind), np.int32)
         blocks = DeviceArray.from_host(table)
         if name == "tiled":
             _lib.call("bpmi_ffbp_merges_tiled", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
                       _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.interp_code(interp),
+                      None)
+        elif name == "tiled_win":
+            windows = _window_table(t_int, t_geo, blocks, interp, node_int, table)
+            _lib.call("bpmi_ffbp_merges_tiled_win", _vp(t_int.ptr), _vp(t_geo.ptr),
+                      _vp(t_off.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(table), k,
+                      _lib.interp_code(interp), *_lib.DEFAULT_MERGE_SHAPE, _vp(windows.ptr),
                       None)
         else:
             _lib.call("bpmi_ffbp_merges", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
@@ -78,6 +108,7 @@ def test_tiled_merge_matches_untiled(child_du, interp):
     print(f"child du {child_du} {interp}: {zero} of {p0} parent pixels outside the children")
     assert 0 < zero < p0 // 2
     assert np.array_equal(a, b)
+    assert np.array_equal(results["tiled_win"], a)
     assert np.linalg.norm(a - c) <= 1e-4 * np.linalg.norm(c)
 
 
@@ -117,14 +148,20 @@ def test_tile_with_a_pixel_on_a_child_origin(interp):
     t_off = DeviceArray.from_host(offsets)
     k = 4 * np.pi * 9.6e9 / 299792458.0
     out = {}
-    for name in ("tiled", "untiled"):
+    for name in ("tiled", "tiled_win", "untiled"):
         arena = DeviceArray.from_host(arena_host)
-        table = np.asarray(merge_block_table(node_int, [0], "split" if name == "tiled" else "x"),
-                           np.int32)
+        kind = "split" if name.startswith("tiled") else "x"
+        table = np.asarray(merge_block_table(node_int, [0], kind), np.int32)
         blocks = DeviceArray.from_host(table)
         if name == "tiled":
             _lib.call("bpmi_ffbp_merges_tiled", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
                       _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.interp_code(interp),
+                      None)
+        elif name == "tiled_win":
+            windows = _window_table(t_int, t_geo, blocks, interp, node_int, table)
+            _lib.call("bpmi_ffbp_merges_tiled_win", _vp(t_int.ptr), _vp(t_geo.ptr),
+                      _vp(t_off.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(table), k,
+                      _lib.interp_code(interp), *_lib.DEFAULT_MERGE_SHAPE, _vp(windows.ptr),
                       None)
         else:
             _lib.call("bpmi_ffbp_merges", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
@@ -136,3 +173,4 @@ def test_tile_with_a_pixel_on_a_child_origin(interp):
     assert out["tiled"][3, 2] == 0
     assert np.count_nonzero(out["tiled"]) >= 6
     assert np.array_equal(out["tiled"], out["untiled"])
+    assert np.array_equal(out["tiled_win"], out["tiled"])
