@@ -101,11 +101,12 @@ def check_launch_shapes(mode, pulse_block, merge_shapes, merge_heights):
 
 
 def merge_window_table(node_int_ptr, node_geo_ptr, blocks, merge_interp):
-    """Window bounds of every entry of a split-mode block table
-    (``bpmi_ffbp_merge_windows``: int32 [entries, 8], built once, from the
-    node geometry only); the tiled merges read them instead of deriving
-    them in every workgroup."""
-    windows = DeviceArray.empty((blocks.shape[0], 8), np.int32)
+    """Window table of a split-mode block table (``bpmi_ffbp_merge_windows``:
+    one 64-byte entry per table entry, int32 window bounds (r0, c0, h, w) of
+    both children, then their float64 reciprocal spacings; built once, from
+    the node geometry only); the tiled merges read it instead of deriving
+    the entries in every workgroup."""
+    windows = DeviceArray.empty((blocks.shape[0], 16), np.int32)
     _lib.call("bpmi_ffbp_merge_windows", _vp(node_int_ptr), _vp(node_geo_ptr), _vp(blocks.ptr),
               int(blocks.shape[0]), _lib.interp_code(merge_interp), _vp(windows.ptr), None)
     synchronize()

@@ -680,6 +680,14 @@ __device__ __forceinline__ WinBounds window_bounds(const int* ir, const int* iu,
     return b;
 }
 
+// One entry of a merge window table (merge_windows_kernel): both children's
+// window bounds and reciprocal spacings (1/dr, 1/du of child a, then b).
+struct MergeWindowEntry {
+    WinBounds a, b;
+    double inv[4];
+};
+static_assert(sizeof(MergeWindowEntry) == 64, "64-byte table entries");
+
 // The window as a launch with CAP samples of LDS per child stages it: in LDS
 // when the bounds fit the cap, else read from global memory (lds false).
 template <int CAP>
@@ -884,16 +892,17 @@ using MergeLds = MergeLdsCap<MT_CAP>;
 
 // One 32 x 32 tile of parent node ``node`` (all MT_THREADS threads call it;
 // every thread reaches both barriers).  kp_src: the Kaiser weight
-// polynomials (c_kaiser_poly).  win: the tile's window bounds from a table
-// of merge_windows_kernel (children a and b: r0, c0, h, w), or null to
-// derive them here.  A caller that runs several tiles in a row must place a
+// polynomials (c_kaiser_poly).  win: the tile's entry of a
+// merge_windows_kernel table (window bounds and reciprocal spacings of both
+// children), or null to derive them here.  A caller that runs several tiles in a row must place a
 // barrier between them.
 template <int INTERP, bool COH = false, int CAP = MT_CAP>
 __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
                                            const double* __restrict__ node_geo,
                                            const long long* __restrict__ offsets, float2* arena,
                                            int node, int tile, double k, const float* kp_src,
-                                           const int* __restrict__ win, MergeLdsCap<CAP>& lds) {
+                                           const MergeWindowEntry* __restrict__ win,
+                                           MergeLdsCap<CAP>& lds) {
     // The second window's base offset is opaque to the compiler, so the
     // stencil reads address it from a register rather than through
     // per-instruction offset additions (ds_read2 offsets reach 2 KiB).
@@ -937,14 +946,14 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
     // deriving it.
     const bool table = win != nullptr;  // uniform
     if (uniform_i32(wave) == 0) {
-        const FastSrc fa = make_fast_src(img_a, ga, lnr, lnu, lnu);
-        const FastSrc fb = make_fast_src(img_b, gb, rnr, rnu, rnu);
-        if (t == 0) {
-            lds.inv[0] = fa.inv_dr; lds.inv[1] = fa.inv_du;
-            lds.inv[2] = fb.inv_dr; lds.inv[3] = fb.inv_du;
-        }
         if (t < 4 * KPOLY_STRIDE) lds.kpoly[t] = kp_src[t];
         if (!table) {
+            const FastSrc fa = make_fast_src(img_a, ga, lnr, lnu, lnu);
+            const FastSrc fb = make_fast_src(img_b, gb, rnr, rnu, rnu);
+            if (t == 0) {
+                lds.inv[0] = fa.inv_dr; lds.inv[1] = fa.inv_du;
+                lds.inv[2] = fb.inv_dr; lds.inv[3] = fb.inv_du;
+            }
             int car = 0, cau = 0, cbr = 0, cbu = 0;
             if (t < 4) {
                 const int crow = min(tr * MT_ROWS + (t >> 1) * (MT_ROWS - 1), nr - 1);
@@ -993,18 +1002,21 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         }
     }
     __syncthreads();
-    const FastSrc fa = make_fast_src_inv(img_a, ga, lnr, lnu, lnu, uniform_f64(lds.inv[0]),
-                                         uniform_f64(lds.inv[1]));
-    const FastSrc fb = make_fast_src_inv(img_b, gb, rnr, rnu, rnu, uniform_f64(lds.inv[2]),
-                                         uniform_f64(lds.inv[3]));
+    // the table entry through scalar loads (the cap test then runs in
+    // scalar registers), else what the first wave left in LDS
+    const BPMI_CONST_AS MergeWindowEntry* e = as_const(win);
+    const FastSrc fa = make_fast_src_inv(img_a, ga, lnr, lnu, lnu,
+                                         table ? e->inv[0] : uniform_f64(lds.inv[0]),
+                                         table ? e->inv[1] : uniform_f64(lds.inv[1]));
+    const FastSrc fb = make_fast_src_inv(img_b, gb, rnr, rnu, rnu,
+                                         table ? e->inv[2] : uniform_f64(lds.inv[2]),
+                                         table ? e->inv[3] : uniform_f64(lds.inv[3]));
     Window wa, wb;
     bool narrow;
     if (table) {
-        // scalar loads of the table entry; the cap test in scalar registers
-        const BPMI_CONST_AS int* e = as_const(win);
         WinBounds ba, bb;
-        ba.r0 = e[0]; ba.c0 = e[1]; ba.h = e[2]; ba.w = e[3];
-        bb.r0 = e[4]; bb.c0 = e[5]; bb.h = e[6]; bb.w = e[7];
+        ba.r0 = e->a.r0; ba.c0 = e->a.c0; ba.h = e->a.h; ba.w = e->a.w;
+        bb.r0 = e->b.r0; bb.c0 = e->b.c0; bb.h = e->b.h; bb.w = e->b.w;
         wa = cap_window<CAP>(ba);
         wb = cap_window<CAP>(bb);
         narrow = narrow_windows<CAP>(wa, wb);
@@ -1055,37 +1067,38 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         pixels.template operator()<0>();
 }
 
-// wins: null, or the block table's window bounds (merge_windows_kernel).
+// wins: null, or the block table's window table (merge_windows_kernel).
 template <int INTERP, int WAVES = MT_WAVES, int CAP = MT_CAP>
 __global__ __launch_bounds__(MT_THREADS, WAVES) void batched_merge_tiled_kernel(
         const int* __restrict__ node_int, const double* __restrict__ node_geo,
         const long long* __restrict__ offsets, float2* arena, const int* __restrict__ blocks,
-        double k, const int* __restrict__ wins) {
+        double k, const MergeWindowEntry* __restrict__ wins) {
     __shared__ MergeLdsCap<CAP> lds;
     const int node = blocks[2 * blockIdx.x], tile = blocks[2 * blockIdx.x + 1];
     if (node < 0) return;  // padding entry of a die-ordered table (uniform)
     merge_tile<INTERP, false, CAP>(node_int, node_geo, offsets, arena, node, tile, k,
-                                   &c_kaiser_poly[0][0],
-                                   wins ? wins + 8 * (long long)blockIdx.x : nullptr, lds);
+                                   &c_kaiser_poly[0][0], wins ? wins + blockIdx.x : nullptr,
+                                   lds);
 }
 
-// Window bounds of every (node, tile) entry of a tiled-merge block table,
-// the same corner-stencil placement merge_tile derives without a table (the
-// bounds depend on the plan's geometry only, not on the pulses):
-// windows[8 e + 0..3] = child a's r0, c0, h, w, [8 e + 4..7] child b's;
-// padding entries (node < 0) get empty windows.  One thread per entry.
+// Window table of a tiled-merge block table: for every (node, tile) entry
+// the same corner-stencil window placement and the same reciprocal child
+// spacings as merge_tile derives without a table (they depend on the plan's
+// geometry only, not on the pulses).  Padding entries (node < 0) get empty
+// windows.  One thread per entry.
 template <int INTERP>
 __global__ __launch_bounds__(256) void merge_windows_kernel(const int* __restrict__ node_int,
                                                             const double* __restrict__ node_geo,
                                                             const int* __restrict__ blocks,
-                                                            int nblocks, int* __restrict__ windows) {
+                                                            int nblocks,
+                                                            MergeWindowEntry* __restrict__ table) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nblocks) return;
-    int* const out = windows + 8 * (long long)e;
+    MergeWindowEntry& out = table[e];
     const int node = blocks[2 * e], tile = blocks[2 * e + 1];
     if (node < 0) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) out[j] = 0;
+        out.a = out.b = WinBounds{0, 0, 0, 0};
+        out.inv[0] = out.inv[1] = out.inv[2] = out.inv[3] = 0.0;
         return;
     }
     const int* const ni = node_int + 8 * node;
@@ -1116,10 +1129,10 @@ __global__ __launch_bounds__(256) void merge_windows_kernel(const int* __restric
         ira[j] = ca.ir; iua[j] = ca.iu;
         irb[j] = cb.ir; iub[j] = cb.iu;
     }
-    const WinBounds ba = window_bounds<INTERP>(ira, iua, lnr, lnu);
-    const WinBounds bb = window_bounds<INTERP>(irb, iub, rnr, rnu);
-    out[0] = ba.r0; out[1] = ba.c0; out[2] = ba.h; out[3] = ba.w;
-    out[4] = bb.r0; out[5] = bb.c0; out[6] = bb.h; out[7] = bb.w;
+    out.a = window_bounds<INTERP>(ira, iua, lnr, lnu);
+    out.b = window_bounds<INTERP>(irb, iub, rnr, rnu);
+    out.inv[0] = fa.inv_dr; out.inv[1] = fa.inv_du;
+    out.inv[2] = fb.inv_dr; out.inv[3] = fb.inv_du;
 }
 
 // ---- required-tile marking ----------------------------------------------------
@@ -1964,7 +1977,7 @@ template <int INTERP>
 static void launch_merge_shape(int waves, int cap, int nblocks, hipStream_t hs,
                                const int* node_int, const double* node_geo,
                                const long long* offsets, float2* ar, const int* blocks, double k,
-                               const int* wins) {
+                               const MergeWindowEntry* wins) {
 #define BPMI_MERGE_SHAPE(W, C)                                                         \
     if (waves == W && cap == C) {                                                      \
         hipLaunchKernelGGL((batched_merge_tiled_kernel<INTERP, W, C>), dim3(nblocks),  \
@@ -2118,7 +2131,7 @@ int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
 }
 
 int bpmi_ffbp_merge_windows(const int* node_int, const double* node_geo, const int* blocks,
-                            int nblocks, int interp, int* windows, void* stream) {
+                            int nblocks, int interp, void* windows, void* stream) {
     int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
     if (st) return st;
     if (nblocks <= 0) return BPMI_OK;
@@ -2128,17 +2141,17 @@ int bpmi_ffbp_merge_windows(const int* node_int, const double* node_geo, const i
     const dim3 grid(grid_1d(nblocks, 256));
     if (interp == BPMI_INTERP_KAISER8)
         hipLaunchKernelGGL(merge_windows_kernel<INTERP_KAISER8>, grid, dim3(256), 0, hs, node_int,
-                           node_geo, blocks, nblocks, windows);
+                           node_geo, blocks, nblocks, (MergeWindowEntry*)windows);
     else
         hipLaunchKernelGGL(merge_windows_kernel<INTERP_LAGRANGE4>, grid, dim3(256), 0, hs,
-                           node_int, node_geo, blocks, nblocks, windows);
+                           node_int, node_geo, blocks, nblocks, (MergeWindowEntry*)windows);
     return check_launch("merge_windows_kernel");
 }
 
 int bpmi_ffbp_merges_tiled_win(const int* node_int, const double* node_geo,
                                const long long* offsets, void* arena, const int* blocks,
                                int nblocks, double k, int interp, int waves, int window_cap,
-                               const int* windows, void* stream) {
+                               const void* windows, void* stream) {
     int st = check_mode_interp(BPMI_MODE_SPLIT, interp);
     if (st) return st;
     if (!merge_shape_known(waves, window_cap))
@@ -2151,10 +2164,12 @@ int bpmi_ffbp_merges_tiled_win(const int* node_int, const double* node_geo,
     hipStream_t hs = (hipStream_t)stream;
     if (interp == BPMI_INTERP_KAISER8)
         launch_merge_shape<INTERP_KAISER8>(waves, window_cap, nblocks, hs, node_int, node_geo,
-                                           offsets, ar, blocks, k, windows);
+                                           offsets, ar, blocks, k,
+                                           (const MergeWindowEntry*)windows);
     else
         launch_merge_shape<INTERP_LAGRANGE4>(waves, window_cap, nblocks, hs, node_int, node_geo,
-                                             offsets, ar, blocks, k, windows);
+                                             offsets, ar, blocks, k,
+                                             (const MergeWindowEntry*)windows);
     return check_launch("batched_merge_tiled_kernel");
 }
 

@@ -279,22 +279,23 @@ int bpmi_ffbp_merges_tiled_ex(const int* node_int, const double* node_geo,
                               const long long* offsets, void* arena, const int* blocks,
                               int nblocks, double k, int interp, int waves, int window_cap,
                               void* stream);
-/* Window table of a tiled-merge block table: windows int32 [nblocks, 8] =
- * the LDS window bounds (r0, c0, h, w) in child a, then in child b, of each
- * (node, tile) entry -- where the tile's stencils read the children, from
- * the node geometry alone (no pulse data), as each merge workgroup would
- * derive them from its corner pixels.  Built once per block table; no
- * reference counterpart (the reference's merge reads its children through
- * per-pixel global loads, resident_kernels.cu:35-157). */
+/* Window table of a tiled-merge block table: ``windows`` holds nblocks
+ * 64-byte entries, each the LDS window bounds (int32 r0, c0, h, w) of the
+ * (node, tile) entry's stencils in child a, then in child b, followed by
+ * the children's reciprocal spacings (double 1/dr, 1/du of a, then of b) --
+ * computed from the node geometry alone (no pulse data), exactly as each
+ * merge workgroup would derive them.  Built once per block table; no
+ * reference counterpart (the reference's merge gathers every tap from
+ * global memory, resident_kernels.cu:35-157). */
 int bpmi_ffbp_merge_windows(const int* node_int, const double* node_geo, const int* blocks,
-                            int nblocks, int interp, int* windows, void* stream);
-/* bpmi_ffbp_merges_tiled_ex reading each tile's window bounds from a
+                            int nblocks, int interp, void* windows, void* stream);
+/* bpmi_ffbp_merges_tiled_ex reading each tile's entry of a
  * bpmi_ffbp_merge_windows table of the same block table instead of deriving
- * them in the workgroup (null: derive them); outputs bitwise equal. */
+ * it in the workgroup (null: derive it); outputs bitwise equal. */
 int bpmi_ffbp_merges_tiled_win(const int* node_int, const double* node_geo,
                                const long long* offsets, void* arena, const int* blocks,
                                int nblocks, double k, int interp, int waves, int window_cap,
-                               const int* windows, void* stream);
+                               const void* windows, void* stream);
 int bpmi_merge_tiles(int nr, int nu, int* tiles);
 
 /* Required-tile marking for culled schedules (no reference counterpart: the

@@ -41,19 +41,23 @@ def _tables(child_du):
 
 def _window_table(t_int, t_geo, blocks, interp, node_int, table):
     """bpmi_ffbp_merge_windows of a block table, checked on the host: every
-    window lies inside its child, and a stageable one is at least one
-    stencil wide."""
+    window lies inside its child, a stageable one is at least one stencil
+    wide, and the reciprocal spacings are the children's to FP64 rounding."""
     from bp_project_amd.batched_resident_backend import merge_window_table
     from bp_project_amd.plan import NI_LEFT, NI_NR, NI_NU, NI_RIGHT
     windows = merge_window_table(t_int.ptr, t_geo.ptr, blocks, interp)
     w = windows.get()
+    geo = t_geo.get()
     taps = 8 if interp == "kaiser8" else 4
     for (node, _), e in zip(table, w):
         if node < 0:
             assert not e.any()
             continue
-        for child, (r0, c0, h, wd) in zip((node_int[node, NI_LEFT], node_int[node, NI_RIGHT]),
-                                          e.reshape(2, 4)):
+        kids = (node_int[node, NI_LEFT], node_int[node, NI_RIGHT])
+        inv = e[8:].view(np.float64).reshape(2, 2)
+        for child, (idr, idu) in zip(kids, inv):
+            assert abs(idr * geo[child, 4] - 1) < 1e-15 and abs(idu * geo[child, 6] - 1) < 1e-15
+        for child, (r0, c0, h, wd) in zip(kids, e[:8].reshape(2, 4)):
             assert (h == 0) == (wd == 0)
             if h:
                 assert h >= taps and wd >= taps and r0 >= 0 and c0 >= 0
