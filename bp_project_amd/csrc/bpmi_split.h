@@ -583,11 +583,13 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
     double px[SP_PPT], py[SP_PPT], pz[SP_PPT], pr[SP_PPT];
     double lo_x = 1e300, lo_y = 1e300, lo_z = 1e300, hi_x = -1e300, hi_y = -1e300, hi_z = -1e300;
     double rmin = 1e300, rmax = -1e300;
+    int col0 = 0;  // the thread's column (grid and polar sources: the same for every slot)
 #pragma unroll
     for (int k = 0; k < SP_PPT; ++k) {
         int row = 0, col = 0;
         pix[k] = split_pixel<SRC>(s, tile, tc_log2, t, k, &val[k], &row, &col);
         if (!val[k]) row = col = 0;
+        if (k == 0) col0 = col;
         split_pixel64<SRC>(s, val[k] ? pix[k] : 0, row, col, &px[k], &py[k], &pz[k], &pr[k]);
         if (!ORIGIN_REF && val[k]) {
             lo_x = fmin(lo_x, px[k]); hi_x = fmax(hi_x, px[k]);
@@ -619,10 +621,33 @@ __device__ __forceinline__ void bp_split_tile(BpArgs a, SrcArgs s, int tc_log2, 
     double dxk[SP_PPT], dyk[SP_PPT], dzk[SP_PPT];
     float ek[SP_PPT], qpk[SP_PPT], sck[SP_PPT], irk[SP_PPT];
     float2 acc[SP_PPT];
+    // A polar image about the reference origin: the unit vector to pixel
+    // (row, col) is (side sqrt(1 - u^2) g / r, u g / r, -z / r) with the
+    // ground range g = sqrt(r^2 - z^2) -- a column factor (the thread's
+    // column is the same for all its pixels) times row terms, instead of
+    // the pixel's coordinates minus the origin.
+    double col_x = 0.0, col_y = 0.0;
+    if (ORIGIN_REF && SRC == SRC_POLAR) {
+        // (slot 0 invalid: every slot of the thread is, and nothing uses these)
+        const double u = s.b2 + (double)col0 * s.b3;
+        col_x = (double)s.side * sqrt64_fast(1.0 - u * u);
+        col_y = u;
+    }
 #pragma unroll
     for (int k = 0; k < SP_PPT; ++k) {
         acc[k] = make_float2(0.f, 0.f);
-        if (ORIGIN_REF) {
+        if (ORIGIN_REF && SRC == SRC_POLAR) {
+            const double r = pr[k] > 0.0 ? pr[k] : 1.0;
+            const double ir = rcp64(r);
+            const double gi = sqrt64_fast(pr[k] * pr[k] - s.a2 * s.a2) * ir;
+            dxk[k] = col_x * gi;
+            dyk[k] = col_y * gi;
+            dzk[k] = -s.a2 * ir;
+            ek[k] = 0.f;
+            qpk[k] = (float)((pr[k] - a.range_start) * inv_ddr - (double)ref.itile);
+            sck[k] = (float)(0.5 * ir);
+            irk[k] = (float)ir;
+        } else if (ORIGIN_REF) {
             // unit vector from the node origin to the pixel; |p - o| = r
             const double r = pr[k] > 0.0 ? pr[k] : 1.0;
             const double ir = rcp64(r);
