@@ -688,6 +688,21 @@ struct MergeWindowEntry {
 };
 static_assert(sizeof(MergeWindowEntry) == 64, "64-byte table entries");
 
+// Bounds read from a table, made safe for the child they are applied to
+// (a table built for another block table then costs speed, not
+// correctness): clipped to the nr x nu image and emptied when that leaves
+// less than one stencil.  Scalar arithmetic.
+template <int INTERP>
+__device__ __forceinline__ WinBounds clip_bounds(WinBounds b, int nr, int nu) {
+    constexpr int TAPS = FastShape<INTERP>::TAPS;
+    b.r0 = max(b.r0, 0);
+    b.c0 = max(b.c0, 0);
+    b.h = min(b.h, nr - b.r0);
+    b.w = min(b.w, nu - b.c0);
+    if (b.h < TAPS || b.w < TAPS) b.h = b.w = 0;
+    return b;
+}
+
 // The window as a launch with CAP samples of LDS per child stages it: in LDS
 // when the bounds fit the cap, else read from global memory (lds false).
 template <int CAP>
@@ -1017,8 +1032,8 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
         WinBounds ba, bb;
         ba.r0 = e->a.r0; ba.c0 = e->a.c0; ba.h = e->a.h; ba.w = e->a.w;
         bb.r0 = e->b.r0; bb.c0 = e->b.c0; bb.h = e->b.h; bb.w = e->b.w;
-        wa = cap_window<CAP>(ba);
-        wb = cap_window<CAP>(bb);
+        wa = cap_window<CAP>(clip_bounds<INTERP>(ba, lnr, lnu));
+        wb = cap_window<CAP>(clip_bounds<INTERP>(bb, rnr, rnu));
         narrow = narrow_windows<CAP>(wa, wb);
     } else {
         Window* const w2[2] = {&wa, &wb};

@@ -291,7 +291,10 @@ int bpmi_ffbp_merge_windows(const int* node_int, const double* node_geo, const i
                             int nblocks, int interp, void* windows, void* stream);
 /* bpmi_ffbp_merges_tiled_ex reading each tile's entry of a
  * bpmi_ffbp_merge_windows table of the same block table instead of deriving
- * it in the workgroup (null: derive it); outputs bitwise equal. */
+ * it in the workgroup (null: derive it); outputs bitwise equal.  Window
+ * bounds are clipped to the child they are applied to, so a table of
+ * another block table gives the same outputs, only slower; the reciprocal
+ * spacings must be the children's (a table of this plan's nodes). */
 int bpmi_ffbp_merges_tiled_win(const int* node_int, const double* node_geo,
                                const long long* offsets, void* arena, const int* blocks,
                                int nblocks, double k, int interp, int waves, int window_cap,

@@ -1,7 +1,8 @@
 """Split-mode LDS-tiled merge (bpmi_ffbp_merges_tiled, and
 bpmi_ffbp_merges_tiled_win reading a bpmi_ffbp_merge_windows table) on
-hand-built node tables: bitwise equal to the untiled split merge
-(bpmi_ffbp_merges) and
+hand-built node tables (also with a window table whose bounds belong to
+other tiles, which the kernel clips to each child): bitwise equal to the
+untiled split merge (bpmi_ffbp_merges) and
 within 1e-4 of the FP64-mode merge, both when each child's footprint fits
 the LDS tile (coarse child u sampling) and when it does not and the kernel
 gathers from global memory (child u sampling 8x finer than the parent's).
@@ -85,7 +86,7 @@ def test_tiled_merge_matches_untiled(child_du, interp):
     k = 4 * np.pi * 9.6e9 / 299792458.0
     p0 = nodes[0]["nr"] * nodes[0]["nu"]
     results = {}
-    for name in ("tiled", "tiled_win", "untiled", "fp64"):
+    for name in ("tiled", "tiled_win", "tiled_moved", "untiled", "fp64"):
         arena = DeviceArray.from_host(arena_host)
         mode = "fp64" if name == "fp64" else "split"
         kind = "split" if name.startswith("tiled") else "x"
@@ -95,8 +96,17 @@ def test_tiled_merge_matches_untiled(child_du, interp):
             _lib.call("bpmi_ffbp_merges_tiled", _vp(t_int.ptr), _vp(t_geo.ptr), _vp(t_off.ptr),
                       _vp(arena.ptr), _vp(blocks.ptr), len(table), k, _lib.interp_code(interp),
                       None)
-        elif name == "tiled_win":
+        elif name.startswith("tiled_"):
             windows = _window_table(t_int, t_geo, blocks, interp, node_int, table)
+            if name == "tiled_moved":
+                # bounds of other tiles, a huge window and one past the
+                # image's corner: clipped to each child, only slower
+                host = windows.get()
+                bounds = host[:, :8].copy()
+                host[:, :8] = np.roll(bounds, 1, axis=0)
+                host[0, :4] = (0, 0, 1 << 20, 1 << 20)
+                host[-1, 4:8] = (-50, 10_000, 30, 30)
+                windows = DeviceArray.from_host(host)
             _lib.call("bpmi_ffbp_merges_tiled_win", _vp(t_int.ptr), _vp(t_geo.ptr),
                       _vp(t_off.ptr), _vp(arena.ptr), _vp(blocks.ptr), len(table), k,
                       _lib.interp_code(interp), *_lib.DEFAULT_MERGE_SHAPE, _vp(windows.ptr),
@@ -113,6 +123,7 @@ def test_tiled_merge_matches_untiled(child_du, interp):
     assert 0 < zero < p0 // 2
     assert np.array_equal(a, b)
     assert np.array_equal(results["tiled_win"], a)
+    assert np.array_equal(results["tiled_moved"], a)
     assert np.linalg.norm(a - c) <= 1e-4 * np.linalg.norm(c)
 
 
