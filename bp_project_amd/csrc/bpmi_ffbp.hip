@@ -756,6 +756,9 @@ constexpr int STAGE_ROWS = 4;
 // Window loads are global loads (ld_global_c64) also where the image pointer
 // was read from a table in memory: as flat loads they could alias LDS, and
 // the compiler would order them behind every LDS access of the wave.
+// ROWS: rows per load batch (the evaluation, with one window and one pixel
+// per thread, stages a whole window in one batch).
+template <int ROWS = STAGE_ROWS>
 __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, float2* dst,
                                              int wave, int lane) {
     constexpr int WAVES = MT_THREADS / 64;
@@ -767,20 +770,20 @@ __device__ __forceinline__ void stage_window(const FastSrc& f, const Window& w, 
     const int dst_step = WAVES * w.pitch;
     if (w.w <= 64) {
         if (lane < w.w) {
-            for (int i = wave; i < w.h; i += STAGE_ROWS * WAVES) {
+            for (int i = wave; i < w.h; i += ROWS * WAVES) {
                 // rows i + b * WAVES of this batch: nb of them lie in the
                 // window; the loads of the others repeat its last row (no
                 // branch around a load), only the nb rows are stored
-                const int nb = min(STAGE_ROWS, (w.h - i + WAVES - 1) / WAVES);
-                float2 v[STAGE_ROWS];
+                const int nb = min(ROWS, (w.h - i + WAVES - 1) / WAVES);
+                float2 v[ROWS];
 #pragma unroll
-                for (int b = 0; b < STAGE_ROWS; ++b)
+                for (int b = 0; b < ROWS; ++b)
                     v[b] = ld_global_c64(src + min(b, nb - 1) * src_step + lane);
 #pragma unroll
-                for (int b = 0; b < STAGE_ROWS; ++b)
+                for (int b = 0; b < ROWS; ++b)
                     if (b < nb) d[b * dst_step + lane] = v[b];
-                src += STAGE_ROWS * src_step;
-                d += STAGE_ROWS * dst_step;
+                src += ROWS * src_step;
+                d += ROWS * dst_step;
             }
         }
         return;
@@ -1333,6 +1336,9 @@ __global__ __launch_bounds__(256) void eval_grid_multi_fast_kernel(
 // window above ET_CAP, reads global memory with identical arithmetic, so the
 // output equals eval_grid_fast_kernel's bit for bit.
 constexpr int ET_X = 64, ET_Y = 4, ET_CAP = 4096;
+// rows per staging batch: a 64-column window of ET_CAP samples has 64 rows,
+// 16 per wave, all in one batch
+constexpr int ET_STAGE_ROWS = 16;
 static_assert(ET_X * ET_Y == MT_THREADS, "one pixel per thread");
 
 struct EvalLds {
@@ -1342,8 +1348,10 @@ struct EvalLds {
 };
 
 // Tile (tx, ty) of an nx x ny grid over the polar view f; every thread of
-// the workgroup calls it (two barriers).
-template <int INTERP>
+// the workgroup calls it (two barriers).  SROWS: window rows per staging
+// batch (the persistent kernel, whose registers its other bodies share,
+// keeps the merge's batches).
+template <int INTERP, int SROWS = ET_STAGE_ROWS>
 __device__ __forceinline__ void eval_tile(const FastSrc& f, double x0, double dx, int nx, double y0,
                                           double dy, int ny, double kt, int tx, int ty,
                                           float2* __restrict__ out, EvalLds& lds) {
@@ -1364,7 +1372,7 @@ __device__ __forceinline__ void eval_tile(const FastSrc& f, double x0, double dx
         iu[j] = uniform_i32(lds.corner[j][1]);
     }
     const Window w = corner_window<INTERP, ET_CAP>(ir, iu, f.nr, f.nu);
-    if (w.lds) stage_window(f, w, lds.sh, wave, lane);
+    if (w.lds) stage_window<SROWS>(f, w, lds.sh, wave, lane);
     __syncthreads();
     const int ix = tx * ET_X + (t % ET_X), iy = ty * ET_Y + t / ET_X;
     if (ix < nx && iy < ny) {
@@ -1829,7 +1837,7 @@ __global__ __launch_bounds__(MT_THREADS, PW_WAVES) void ffbp_persistent_kernel(
                     const FastSrc f = make_fast_src(uniform_ptr((const float2*)jp->img), geo,
                                                     uniform_i32(jp->nr), uniform_i32(jp->nu),
                                                     uniform_i32(jp->ld));
-                    eval_tile<MERGE_INTERP>(f, uniform_f64(jp->x0), uniform_f64(jp->dx), nx,
+                    eval_tile<MERGE_INTERP, STAGE_ROWS>(f, uniform_f64(jp->x0), uniform_f64(jp->dx), nx,
                                             uniform_f64(jp->y0), uniform_f64(jp->dy), ny,
                                             uniform_f64(jp->k) * INV_TWO_PI, tx, ty,
                                             uniform_ptr((float2*)jp->out), lds.eval);
