@@ -1019,7 +1019,11 @@ __device__ __forceinline__ void merge_tile(const int* __restrict__ node_int,
             lds.col[lane - MT_ROWS] = make_double2(c.u, c.scu);
         }
     }
-    __syncthreads();
+    // With a table nothing below reads LDS before the staging barrier (the
+    // windows come from the table), so the waves start staging at once and
+    // one barrier orders the Kaiser polynomials, row and column terms and
+    // windows; without one, the first wave's window placement is waited for.
+    if (!table) __syncthreads();
     // the table entry through scalar loads (the cap test then runs in
     // scalar registers), else what the first wave left in LDS
     const BPMI_CONST_AS MergeWindowEntry* e = as_const(win);
